@@ -1,0 +1,140 @@
+/*
+ * yatscf_gpu.h -- C-ABI of the MI355X (gfx950) DF-RHF hot path.
+ *
+ * The library libyatscf_gpu.so owns all device state of one TinySCF engine
+ * behind an opaque handle.  Host code (the drop-in build_DF_tensor.c,
+ * build_Fock.c, build_density.c, DIIS.c and TinySCF.c of
+ * yatscf-df_amd/host/) keeps the reference's TinySCF struct and calls these
+ * functions with plain host pointers.  All matrices are row-major FP64, as in
+ * the reference (utils.h:6-7, TinySCF.h:62-77).  Every function is
+ * synchronous at return (host outputs are valid) and returns YGPU_OK or an
+ * error code; ygpu_last_error() gives the message of the calling thread's
+ * last failure.  A handle is not re-entrant: calls on one handle must not
+ * overlap (the reference's engine is not re-entrant either, TinySCF.h:12).
+ */
+#ifndef YATSCF_GPU_H
+#define YATSCF_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YGPU_OK            0
+#define YGPU_ERR_INVALID   1   /* bad argument or call order */
+#define YGPU_ERR_HIP       2   /* a HIP runtime call failed */
+#define YGPU_ERR_NOMEM     3   /* device allocation failed */
+#define YGPU_ERR_SOLVER    4   /* rocSOLVER eigensolver failed */
+#define YGPU_ERR_NUMERIC   5   /* non-positive metric eigenvalue, NaN */
+#define YGPU_ERR_NODEVICE  6   /* no HIP device */
+
+typedef struct ygpu_ctx ygpu_ctx_t;
+
+/* Phase timers in seconds, named after the reference's printouts
+ * (build_DF_tensor.c:263-284, build_Fock.c:390-395). */
+typedef struct {
+    double t_3c, t_2c, t_invsqrt, t_dfgemm;
+} ygpu_df_timing_t;
+
+typedef struct {
+    double t_tempJ, t_Jmat, t_tempK, t_Kmat, t_symm, t_total;
+} ygpu_fock_timing_t;
+
+/* Message of the calling thread's last error ("" if none). */
+const char *ygpu_last_error(void);
+
+/* Number of visible HIP devices. */
+int ygpu_device_count(int *count);
+
+/* Create / destroy the device engine on `device` (replaces the device-free
+ * state set up by init_TinySCF, TinySCF_init_free.c:14-208, and freed by
+ * free_TinySCF, TinySCF_init_free.c:211-268). */
+int ygpu_create(int device, ygpu_ctx_t **ctx);
+int ygpu_destroy(ygpu_ctx_t *ctx);
+
+/* Shells of the orbital basis (which = 0) or of the density-fitting basis
+ * (which = 1), in the reference's shell order (CMS_parse_molecule,
+ * CMS_BasisSet.c:192-299) with normalised contraction coefficients (the
+ * Simint shells of CMS_createSimint, CMS_Simint.c:79-145).  xyz is 3*nshells
+ * Bohr; alpha/coef are indexed by prim_off[shell]. */
+int ygpu_set_basis(ygpu_ctx_t *ctx, int which, int nshells, const int *am, const int *nprim,
+                   const int *prim_off, const double *xyz, const double *alpha, const double *coef);
+
+/* Nuclear charges and positions (Bohr) for the core Hamiltonian. */
+int ygpu_set_atoms(ygpu_ctx_t *ctx, int natoms, const double *charge, const double *xyz);
+
+/* S and Hcore = T + V, nbf x nbf (replaces the Simint calls of
+ * TinySCF_compute_Hcore_Ovlp_mat, TinySCF.c:20-54). */
+int ygpu_ovlp_hcore(ygpu_ctx_t *ctx, double *S, double *Hcore);
+
+/* X = A^{-1/2} = U diag(lambda^{-1/2}) U^T of a symmetric positive definite
+ * n x n matrix, FP64 (TinySCF.c:57-76; build_DF_tensor.c:205-237).  min_eig
+ * (nullable) receives the smallest eigenvalue; a non-positive one is
+ * YGPU_ERR_NUMERIC. */
+int ygpu_sym_inv_sqrt(ygpu_ctx_t *ctx, int n, const double *A, double *X, double *min_eig);
+
+/* Schwarz screening values (replaces the Simint calls of
+ * TinySCF_compute_sq_Schwarz_scrvals, TinySCF.c:112-177):
+ *   bf_pair_scrval[i*nbf+j] = |(MN|MN) element| for i in shell M, j in N,
+ *     read at the reference's index when quirk_a5 != 0 (TinySCF.c:153),
+ *     at the diagonal (ij|ij) otherwise;
+ *   sp_scrval[M*nshells+N] = block maximum;
+ *   df_sp_scrval[P] = max_k (P_k|P_k).  Any output may be NULL. */
+int ygpu_schwarz(ygpu_ctx_t *ctx, int quirk_a5, double *sp_scrval, double *bf_pair_scrval,
+                 double *df_sp_scrval);
+
+/* The screened pair layout (TinySCF_prepare_sparsity, TinySCF.c:228-280)
+ * and the unique shell-pair list (TinySCF.c:183-221).  bf_pair_mask must be
+ * symmetric in which entries are >= 0.  The device stores only rows with
+ * i <= j ("up" rows), in the reference's CSR order restricted to j >= i. */
+int ygpu_set_sparsity(ygpu_ctx_t *ctx, const int *bf_pair_mask, int num_uniq_sp,
+                      const int *uniq_sp_lid, const int *uniq_sp_rid, const double *sp_scrval,
+                      const double *df_sp_scrval, double shell_scrtol2);
+
+/* The DF tensor B = (mn|P) J^{-1/2} on the device (TinySCF_build_DF_tensor,
+ * build_DF_tensor.c:256-287).  Jpq_invsqrt (nullable, naux x naux) and
+ * df_tensor (nullable, the reference's host layout nnz x naux with both
+ * triangles, TinySCF.c:268-273) receive host copies. */
+int ygpu_build_df_tensor(ygpu_ctx_t *ctx, double *Jpq_invsqrt, double *df_tensor,
+                         ygpu_df_timing_t *timing);
+
+/* As ygpu_build_df_tensor, also returning the 3-centre integrals (mn|P)
+ * before the metric contraction (nullable, the reference's pqA layout,
+ * build_DF_tensor.c:14-47). */
+int ygpu_build_df_tensor_ex(ygpu_ctx_t *ctx, double *pqA, double *Jpq_invsqrt, double *df_tensor,
+                            ygpu_df_timing_t *timing);
+
+/* Upload a DF tensor already built on the host in the reference layout
+ * (nnz x naux, both triangles) instead of building it on the device. */
+int ygpu_set_df_tensor(ygpu_ctx_t *ctx, const double *df_tensor);
+
+/* J, K and F = Hcore + 2J - K from D and C_occ (TinySCF_build_FockMat,
+ * build_Fock.c:349-396).  D, Hcore: nbf x nbf; Cocc: nbf x n_occ.  J, K
+ * (nullable) and F receive full symmetric matrices; J entries of screened
+ * pairs are zero. */
+int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *Cocc,
+                    const double *Hcore, double *J, double *K, double *F,
+                    ygpu_fock_timing_t *timing);
+
+/* Density from the orthogonal-basis Fock matrix F_orth = X^T F X
+ * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
+ * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.
+ * ev_idx (nullable) receives the order used (identity for the ascending
+ * eigenvalues of the eigensolver). */
+int ygpu_build_density(ygpu_ctx_t *ctx, int n_occ, const double *F_orth, const double *X,
+                       double *D, double *Cocc, double *eigval, int *ev_idx);
+
+/* Row-major FP64 GEMM on host operands, C = alpha op(A) op(B) + beta C
+ * (the cblas_dgemm calls of DIIS.c:34-107). */
+int ygpu_dgemm(ygpu_ctx_t *ctx, int transa, int transb, int m, int n, int k, double alpha,
+               const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc);
+
+/* Device-resident pair layout sizes: rows of the packed tensor (i <= j),
+ * rows of the reference layout (both triangles) and the padded aux width. */
+int ygpu_layout_info(ygpu_ctx_t *ctx, long *n_up, long *nnz, int *naux_pad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
+same inputs.  Bars: FP64 Fock matrix within 1e-10 max-abs, converged total
+energy within 1e-8 Eh; integral and tensor stages within the tolerances
+written in each test."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import system, xyz_path
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ygpu():
+    import ygpu as m
+    assert m.device_count() >= 1, "no GPU visible"
+    return m
+
+
+class OracleState:
+    def __init__(self, name):
+        import oracle
+        self.name = name
+        self.sys = system(name)
+        self.o = oracle.OracleSCF(self.sys, niters=60)
+        self.o.setup()
+        self.o.build_df_tensor()
+        n = self.sys.nbf
+        self.D0 = self.o.mat("D_mat").copy()
+        self.C0 = self.o.d("Cocc_mat", (n, self.sys.n_occ)).copy()
+        self.o.build_fock()
+        self.F0 = self.o.mat("F_mat").copy()
+        self.J0 = self.o.mat("J_mat").copy()
+        self.K0 = self.o.mat("K_mat").copy()
+
+
+_states = {}
+
+
+def oracle_state(name):
+    if name not in _states:
+        _states[name] = OracleState(name)
+    return _states[name]
+
+
+def make_engine(ygpu, st):
+    e = ygpu.Engine(0)
+    e.set_system(st.sys)
+    return e
+
+
+def set_oracle_sparsity(e, st):
+    o = st.o
+    e.set_sparsity(o.i("bf_pair_mask"), o.i("uniq_sp_lid"), o.i("uniq_sp_rid"), o.d("sp_scrval"),
+                   o.d("df_sp_scrval"))
+
+
+def test_dgemm_all_transposes(ygpu):
+    log("case: dgemm transposes")
+    rng = np.random.RandomState(7)
+    e = ygpu.Engine(0)
+    for (m, n, k) in [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 129, 67)]:
+        for ta in (0, 1):
+            for tb in (0, 1):
+                A = rng.randn(*((k, m) if ta else (m, k)))
+                B = rng.randn(*((n, k) if tb else (k, n)))
+                C0 = rng.randn(m, n)
+                ref = 1.5 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.5 * C0
+                got = e.dgemm(ta, tb, A, B, alpha=1.5, beta=-0.5, C=C0)
+                assert np.max(np.abs(got - ref)) < 1e-12 * max(1, k), (m, n, k, ta, tb)
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_one_electron_and_schwarz(ygpu, name):
+    log("case: S/H/X/Schwarz", name)
+    st = oracle_state(name)
+    e = make_engine(ygpu, st)
+    S, H = e.ovlp_hcore()
+    assert np.max(np.abs(S - st.o.mat("S_mat"))) < 1e-12
+    assert np.max(np.abs(H - st.o.mat("Hcore_mat"))) < 1e-10
+    X, mine = e.sym_inv_sqrt(S)
+    assert mine > 0
+    # X = S^-1/2 is unique: X S X = I
+    assert np.max(np.abs(X @ S @ X - np.eye(len(S)))) < 1e-9
+    assert np.max(np.abs(X - st.o.mat("X_mat"))) < 1e-8 * max(1.0, np.abs(X).max())
+    sp, bf, df = e.schwarz(quirk_a5=True)
+    ref_bf = st.o.d("bf_pair_scrval")
+    assert np.allclose(bf, ref_bf, rtol=1e-10, atol=1e-20)
+    assert np.allclose(sp, st.o.d("sp_scrval"), rtol=1e-10, atol=1e-20)
+    assert np.allclose(df, st.o.d("df_sp_scrval"), rtol=1e-12, atol=0)
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_df_tensor(ygpu, name):
+    log("case: DF tensor", name)
+    st = oracle_state(name)
+    e = make_engine(ygpu, st)
+    set_oracle_sparsity(e, st)
+    r = e.build_df_tensor(want_pqA=True, want_Jpq=True, want_B=True)
+    pqA_ref = st.o.d("pqA").reshape(r["pqA"].shape)
+    err3 = np.max(np.abs(r["pqA"] - pqA_ref))
+    log("  3c max abs err %.3e (max |value| %.3e)" % (err3, np.abs(pqA_ref).max()))
+    assert err3 < 1e-11
+    Jinv_ref = st.o.d("Jpq").reshape(r["Jpq"].shape)
+    errJ = np.max(np.abs(r["Jpq"] - Jinv_ref)) / np.abs(Jinv_ref).max()
+    log("  J^-1/2 rel err %.3e" % errJ)
+    assert errJ < 1e-8
+    B_ref = st.o.d("df_tensor").reshape(r["B"].shape)
+    errB = np.max(np.abs(r["B"] - B_ref))
+    log("  B max abs err %.3e" % errB)
+    assert errB < 1e-8
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_fock_same_inputs(ygpu, name):
+    """Same D, C_occ and the oracle's own B: F within 1e-10."""
+    log("case: Fock with oracle B", name)
+    st = oracle_state(name)
+    e = make_engine(ygpu, st)
+    set_oracle_sparsity(e, st)
+    e.set_df_tensor(st.o.d("df_tensor"))
+    r = e.build_fock(st.D0, st.C0, st.o.mat("Hcore_mat"))
+    errJ = np.max(np.abs(r["J"] - st.J0))
+    errK = np.max(np.abs(r["K"] - st.K0))
+    errF = np.max(np.abs(r["F"] - st.F0))
+    log("  max|dJ| %.3e max|dK| %.3e max|dF| %.3e" % (errJ, errK, errF))
+    assert errF < 1e-10 and errJ < 1e-10 and errK < 1e-10
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_fock_device_tensor(ygpu, name):
+    """End to end on the device (integrals, metric, B, J/K/F) from the same D, C_occ."""
+    log("case: Fock with device B", name)
+    st = oracle_state(name)
+    e = make_engine(ygpu, st)
+    set_oracle_sparsity(e, st)
+    e.build_df_tensor()
+    r = e.build_fock(st.D0, st.C0, st.o.mat("Hcore_mat"))
+    errF = np.max(np.abs(r["F"] - st.F0))
+    log("  max|dF| %.3e" % errF)
+    assert errF < 1e-10
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_density(ygpu, name):
+    log("case: density", name)
+    st = oracle_state(name)
+    o = st.o
+    n, nocc = st.sys.nbf, st.sys.n_occ
+    X = o.mat("X_mat").copy()
+    Fo = X.T @ st.F0 @ X
+    e = make_engine(ygpu, st)
+    D, C, ev, idx = e.build_density(Fo, X, nocc)
+    evref = np.linalg.eigvalsh(Fo)
+    assert np.max(np.abs(ev - evref)) < 1e-10
+    w, V = np.linalg.eigh(Fo)
+    Cref = X @ V[:, :nocc]
+    Dref = Cref @ Cref.T
+    errD = np.max(np.abs(D - Dref))
+    log("  max|dD| %.3e" % errD)
+    assert errD < 1e-10
+    assert np.max(np.abs(C @ C.T - D)) < 1e-12
+    assert np.all(idx == np.arange(n))
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_full_scf_energy(ygpu, name):
+    """Drop-in TinySCF engine (host C + device) against the oracle SCF."""
+    import oracle
+    log("case: full SCF", name)
+    s = system(name)
+    o = oracle.OracleSCF(s, niters=60)
+    o.setup()
+    o.build_df_tensor()
+    o.do_scf()
+    e_ref = o.getd("total_energy")
+    it_ref = o.geti("iter")
+    T = ygpu.TinySCF(xyz_path(name), niters=60)
+    T.setup()
+    T.build_df_tensor()
+    T.do_scf()
+    e_gpu = T.getd("total_energy")
+    log("  E gpu %.10f oracle %.10f dE %.3e iters %d/%d" % (e_gpu, e_ref, e_gpu - e_ref, T.geti("iter"), it_ref))
+    assert T.geti("converged") == 1
+    assert abs(e_gpu - e_ref) < 1e-8
+    T.close()

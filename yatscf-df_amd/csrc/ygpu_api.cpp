@@ -1,0 +1,623 @@
+// ygpu_api.cpp -- the C-ABI of libyatscf_gpu.so (declared in include/yatscf_gpu.h).
+//
+// Owns the device state of one DF-RHF engine: shells, Boys table, the packed
+// pair layout, the DF tensor B (rows i <= j only, aux index padded to a
+// multiple of 64), J^{-1/2}, and the per-iteration work buffers.  FP64
+// eigensolves use rocSOLVER dsyevd on the device; every product is one of the
+// MFMA kernels of ygpu_gemm.hip.
+#include "ygpu_internal.h"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+static thread_local std::string g_err;
+
+void ygpu_set_error(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = std::string("[ygpu ") + std::to_string(code) + "] " + buf;
+}
+
+int DBuf::ensure(size_t n)
+{
+    if (n <= bytes && p) return YGPU_OK;
+    if (p) {
+        (void) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    if (n == 0) n = 8;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        ygpu_set_error(YGPU_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
+        return YGPU_ERR_NOMEM;
+    }
+    bytes = n;
+    return YGPU_OK;
+}
+
+void DBuf::release()
+{
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+namespace {
+
+constexpr int ROUND_AUX = 64;
+
+// F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)), all terms positive
+long double boys_series(int n, long double T)
+{
+    long double term = 1.0L / (2 * n + 1), sum = term;
+    for (int k = 1; k < 5000; ++k) {
+        term *= 2.0L * T / (long double) (2 * n + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+    }
+    return expl(-T) * sum;
+}
+
+int upload(DBuf &b, const void *src, size_t bytes, hipStream_t s)
+{
+    YG_TRY(b.ensure(bytes));
+    if (bytes) YG_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+    return YGPU_OK;
+}
+
+int check_ctx(ygpu_ctx *c)
+{
+    if (!c) {
+        ygpu_set_error(YGPU_ERR_INVALID, "null context");
+        return YGPU_ERR_INVALID;
+    }
+    YG_HIP(hipSetDevice(c->device));
+    return YGPU_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    (void) hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// In-place A (n x n, leading dimension ld) -> out = A^{-1/2}; A is destroyed.
+int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *dOut, double *min_eig)
+{
+    YG_TRY(c->d_eig.ensure((size_t) n * sizeof(double)));
+    YG_TRY(c->d_evE.ensure((size_t) n * sizeof(double)));
+    YG_TRY(c->d_info.ensure(sizeof(int)));
+    rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, dA, ld,
+                                         c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
+    if (st != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", n, (int) st);
+        return YGPU_ERR_SOLVER;
+    }
+    int info = 0;
+    std::vector<double> ev(n);
+    YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(ev.data(), c->d_eig.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (info != 0) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) info=%d", n, info);
+        return YGPU_ERR_SOLVER;
+    }
+    if (min_eig) *min_eig = ev[0];
+    if (!(ev[0] > 0.0)) {
+        ygpu_set_error(YGPU_ERR_NUMERIC, "matrix not positive definite: smallest eigenvalue %.3e", ev[0]);
+        return YGPU_ERR_NUMERIC;
+    }
+    // dA holds V column-major: dA[i + j*ld] = V(i,j), i.e. row k of the row-major view is
+    // eigenvector k.  W = diag(lambda^{-1/2}) V^T (rows scaled); out = V W = (V^T)^T W.
+    YG_HIP(hipMemcpy2DAsync(dW, (size_t) ld * sizeof(double), dA, (size_t) ld * sizeof(double), n * sizeof(double),
+                            n, hipMemcpyDeviceToDevice, c->stream));
+    YG_TRY(yg_scale_rows(c->stream, n, n, dW, ld, c->d_eig.as<double>(), 1));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, dA, ld, dW, ld, 0.0, dOut, ld));
+    return YGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ygpu_last_error(void) { return g_err.c_str(); }
+
+int ygpu_device_count(int *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return YGPU_OK;
+}
+
+int ygpu_create(int device, ygpu_ctx_t **out)
+{
+    if (!out) {
+        ygpu_set_error(YGPU_ERR_INVALID, "null output pointer");
+        return YGPU_ERR_INVALID;
+    }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        ygpu_set_error(YGPU_ERR_NODEVICE, "no HIP device visible");
+        return YGPU_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "device %d out of range (%d visible)", device, n);
+        return YGPU_ERR_INVALID;
+    }
+    ygpu_ctx *c = new ygpu_ctx();
+    c->device = device;
+    YG_HIP(hipSetDevice(device));
+    YG_HIP(hipStreamCreate(&c->stream));
+    for (auto &e : c->ev) YG_HIP(hipEventCreate(&e));
+    if (rocblas_create_handle(&c->rb) != rocblas_status_success ||
+        rocblas_set_stream(c->rb, c->stream) != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocblas handle creation failed");
+        return YGPU_ERR_SOLVER;
+    }
+    std::vector<double> tab((size_t) YG_BOYS_NPTS * (YG_BOYS_NMAX + 1));
+    for (int k = 0; k < YG_BOYS_NPTS; ++k)
+        for (int m = 0; m <= YG_BOYS_NMAX; ++m)
+            tab[(size_t) k * (YG_BOYS_NMAX + 1) + m] = (double) boys_series(m, (long double) k * (long double) YG_BOYS_H);
+    YG_TRY(upload(c->boys, tab.data(), tab.size() * sizeof(double), c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    *out = c;
+    return YGPU_OK;
+}
+
+int ygpu_destroy(ygpu_ctx_t *c)
+{
+    if (!c) return YGPU_OK;
+    (void) hipSetDevice(c->device);
+    (void) hipStreamSynchronize(c->stream);
+    DBuf *bufs[] = {&c->boys, &c->d_charge, &c->d_atom_xyz, &c->d_upmask, &c->d_up_i, &c->d_up_j, &c->d_xptr,
+                    &c->d_xup, &c->d_xpartner, &c->d_sp_scr, &c->d_df_scr, &c->d_pqA, &c->d_B, &c->d_Jpq,
+                    &c->d_D, &c->d_Cocc, &c->d_H, &c->d_J, &c->d_K, &c->d_F, &c->d_tempJ, &c->d_tempJpart,
+                    &c->d_X, &c->d_Kpart, &c->d_mat0, &c->d_mat1, &c->d_mat2, &c->d_eig, &c->d_info, &c->d_evE,
+                    &c->d_gA, &c->d_gB, &c->d_gC};
+    for (DBuf *b : bufs) b->release();
+    for (BasisDev *b : {&c->bas, &c->aux}) {
+        DBuf *bb[] = {&b->d_am, &b->d_nprim, &b->d_prim_off, &b->d_bf_start, &b->d_xyz, &b->d_alpha, &b->d_coef};
+        for (DBuf *x : bb) x->release();
+    }
+    if (c->rb) rocblas_destroy_handle(c->rb);
+    for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+    if (c->stream) (void) hipStreamDestroy(c->stream);
+    delete c;
+    return YGPU_OK;
+}
+
+int ygpu_set_basis(ygpu_ctx_t *c, int which, int nshells, const int *am, const int *nprim, const int *prim_off,
+                   const double *xyz, const double *alpha, const double *coef)
+{
+    YG_TRY(check_ctx(c));
+    if ((which != 0 && which != 1) || nshells <= 0 || !am || !nprim || !prim_off || !xyz || !alpha || !coef) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_basis: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    BasisDev &b = which == 0 ? c->bas : c->aux;
+    b.nshells = nshells;
+    b.am.assign(am, am + nshells);
+    b.nprim.assign(nprim, nprim + nshells);
+    b.prim_off.assign(prim_off, prim_off + nshells);
+    b.xyz.assign(xyz, xyz + 3 * (size_t) nshells);
+    b.bf_start.resize(nshells + 1);
+    int nbf = 0, nprim_end = 0;
+    b.max_am = 0;
+    b.max_nprim = 0;
+    for (int i = 0; i < nshells; ++i) {
+        if (am[i] < 0 || am[i] > 4 || nprim[i] <= 0 || prim_off[i] < 0) {
+            ygpu_set_error(YGPU_ERR_INVALID, "shell %d: am %d nprim %d", i, am[i], nprim[i]);
+            return YGPU_ERR_INVALID;
+        }
+        b.bf_start[i] = nbf;
+        nbf += (am[i] + 1) * (am[i] + 2) / 2;
+        nprim_end = std::max(nprim_end, prim_off[i] + nprim[i]);
+        b.max_am = std::max(b.max_am, am[i]);
+        b.max_nprim = std::max(b.max_nprim, nprim[i]);
+    }
+    b.bf_start[nshells] = nbf;
+    b.nbf = nbf;
+    b.alpha.assign(alpha, alpha + nprim_end);
+    b.coef.assign(coef, coef + nprim_end);
+    YG_TRY(upload(b.d_am, b.am.data(), b.am.size() * sizeof(int), c->stream));
+    YG_TRY(upload(b.d_nprim, b.nprim.data(), b.nprim.size() * sizeof(int), c->stream));
+    YG_TRY(upload(b.d_prim_off, b.prim_off.data(), b.prim_off.size() * sizeof(int), c->stream));
+    YG_TRY(upload(b.d_bf_start, b.bf_start.data(), b.bf_start.size() * sizeof(int), c->stream));
+    YG_TRY(upload(b.d_xyz, b.xyz.data(), b.xyz.size() * sizeof(double), c->stream));
+    YG_TRY(upload(b.d_alpha, b.alpha.data(), b.alpha.size() * sizeof(double), c->stream));
+    YG_TRY(upload(b.d_coef, b.coef.data(), b.coef.size() * sizeof(double), c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (which == 0) c->nbf = nbf;
+    else c->naux = nbf;
+    return YGPU_OK;
+}
+
+int ygpu_set_atoms(ygpu_ctx_t *c, int natoms, const double *charge, const double *xyz)
+{
+    YG_TRY(check_ctx(c));
+    if (natoms <= 0 || !charge || !xyz) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_atoms: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    c->natoms = natoms;
+    YG_TRY(upload(c->d_charge, charge, natoms * sizeof(double), c->stream));
+    YG_TRY(upload(c->d_atom_xyz, xyz, 3 * (size_t) natoms * sizeof(double), c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+}
+
+int ygpu_ovlp_hcore(ygpu_ctx_t *c, double *S, double *H)
+{
+    YG_TRY(check_ctx(c));
+    if (c->bas.nshells == 0 || c->natoms == 0 || !S || !H) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_ovlp_hcore: basis/atoms not set or null output");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
+    YG_TRY(c->d_mat0.ensure(ms));
+    YG_TRY(c->d_mat1.ensure(ms));
+    YG_TRY(yg_launch_ovlp_hcore(c, c->d_mat0.as<double>(), c->d_mat1.as<double>()));
+    YG_HIP(hipMemcpyAsync(S, c->d_mat0.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(H, c->d_mat1.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+}
+
+int ygpu_sym_inv_sqrt(ygpu_ctx_t *c, int n, const double *A, double *X, double *min_eig)
+{
+    YG_TRY(check_ctx(c));
+    if (n <= 0 || !A || !X) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_sym_inv_sqrt: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) n * n * sizeof(double);
+    YG_TRY(c->d_mat0.ensure(ms));
+    YG_TRY(c->d_mat1.ensure(ms));
+    YG_TRY(c->d_mat2.ensure(ms));
+    YG_HIP(hipMemcpyAsync(c->d_mat0.p, A, ms, hipMemcpyHostToDevice, c->stream));
+    YG_TRY(inv_sqrt_device(c, n, n, c->d_mat0.as<double>(), c->d_mat1.as<double>(), c->d_mat2.as<double>(), min_eig));
+    YG_HIP(hipMemcpyAsync(X, c->d_mat2.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+}
+
+int ygpu_schwarz(ygpu_ctx_t *c, int quirk_a5, double *sp_scrval, double *bf_pair_scrval, double *df_sp_scrval)
+{
+    YG_TRY(check_ctx(c));
+    if (c->bas.nshells == 0 || c->aux.nshells == 0) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_schwarz: basis sets not set");
+        return YGPU_ERR_INVALID;
+    }
+    const int nbf = c->nbf, ns = c->bas.nshells;
+    const size_t ms = (size_t) nbf * nbf * sizeof(double);
+    YG_TRY(c->d_mat0.ensure(ms));
+    YG_TRY(c->d_mat1.ensure(c->aux.nshells * sizeof(double)));
+    YG_TRY(yg_launch_schwarz(c, quirk_a5, c->d_mat0.as<double>(), c->d_mat1.as<double>()));
+    std::vector<double> bfs((size_t) nbf * nbf), dfs(c->aux.nshells);
+    YG_HIP(hipMemcpyAsync(bfs.data(), c->d_mat0.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(dfs.data(), c->d_mat1.p, dfs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (bf_pair_scrval) std::memcpy(bf_pair_scrval, bfs.data(), ms);
+    if (df_sp_scrval) std::memcpy(df_sp_scrval, dfs.data(), dfs.size() * sizeof(double));
+    if (sp_scrval)
+        for (int M = 0; M < ns; ++M)
+            for (int N = 0; N < ns; ++N) {
+                double m = 0.0;
+                for (int i = c->bas.bf_start[M]; i < c->bas.bf_start[M + 1]; ++i)
+                    for (int j = c->bas.bf_start[N]; j < c->bas.bf_start[N + 1]; ++j)
+                        m = std::max(m, bfs[(size_t) i * nbf + j]);
+                sp_scrval[(size_t) M * ns + N] = m;
+            }
+    return YGPU_OK;
+}
+
+int ygpu_set_sparsity(ygpu_ctx_t *c, const int *mask, int num_uniq_sp, const int *lid, const int *rid,
+                      const double *sp_scrval, const double *df_sp_scrval, double scrtol2)
+{
+    YG_TRY(check_ctx(c));
+    const int nbf = c->nbf, ns = c->bas.nshells;
+    if (!mask || num_uniq_sp < 0 || (num_uniq_sp > 0 && (!lid || !rid)) || !sp_scrval || !df_sp_scrval || nbf == 0 ||
+        c->naux == 0) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_sparsity: bad arguments or basis sets not set");
+        return YGPU_ERR_INVALID;
+    }
+    c->naux_pad = (c->naux + ROUND_AUX - 1) / ROUND_AUX * ROUND_AUX;
+    c->h_mask.assign(mask, mask + (size_t) nbf * nbf);
+    c->h_upmask.assign((size_t) nbf * nbf, -1);
+    std::vector<int> up_i, up_j;
+    long nnz = 0;
+    for (int i = 0; i < nbf; ++i)
+        for (int j = 0; j < nbf; ++j) {
+            const bool a = mask[(size_t) i * nbf + j] >= 0, b = mask[(size_t) j * nbf + i] >= 0;
+            if (a != b) {
+                ygpu_set_error(YGPU_ERR_INVALID, "pair mask not symmetric at (%d,%d)", i, j);
+                return YGPU_ERR_INVALID;
+            }
+            if (!a) continue;
+            ++nnz;
+            if (j >= i) {
+                c->h_upmask[(size_t) i * nbf + j] = (int) up_i.size();
+                up_i.push_back(i);
+                up_j.push_back(j);
+            }
+        }
+    for (int i = 0; i < nbf; ++i)
+        if (mask[(size_t) i * nbf + i] < 0) {
+            ygpu_set_error(YGPU_ERR_INVALID, "diagonal pair (%d,%d) screened out", i, i);
+            return YGPU_ERR_INVALID;
+        }
+    c->n_up = (long) up_i.size();
+    c->nnz = nnz;
+    std::vector<int> xptr(nbf + 1), xup, xpartner;
+    xup.reserve(nnz);
+    xpartner.reserve(nnz);
+    c->max_row_len = 0;
+    for (int i = 0; i < nbf; ++i) {
+        xptr[i] = (int) xup.size();
+        for (int j = 0; j < nbf; ++j) {
+            if (mask[(size_t) i * nbf + j] < 0) continue;
+            const int lo = std::min(i, j), hi = std::max(i, j);
+            xup.push_back(c->h_upmask[(size_t) lo * nbf + hi]);
+            xpartner.push_back(j);
+        }
+        c->max_row_len = std::max(c->max_row_len, (int) xup.size() - xptr[i]);
+    }
+    xptr[nbf] = (int) xup.size();
+    c->num_uniq_sp = num_uniq_sp;
+    c->h_uniq_lid.assign(lid, lid + num_uniq_sp);
+    c->h_uniq_rid.assign(rid, rid + num_uniq_sp);
+    c->h_sp_scrval.assign(sp_scrval, sp_scrval + (size_t) ns * ns);
+    c->h_df_sp_scrval.assign(df_sp_scrval, df_sp_scrval + c->aux.nshells);
+    c->scrtol2 = scrtol2;
+    YG_TRY(upload(c->d_upmask, c->h_upmask.data(), c->h_upmask.size() * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_up_i, up_i.data(), std::max<size_t>(1, up_i.size()) * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_up_j, up_j.data(), std::max<size_t>(1, up_j.size()) * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_xptr, xptr.data(), xptr.size() * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_xup, xup.data(), std::max<size_t>(1, xup.size()) * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_xpartner, xpartner.data(), std::max<size_t>(1, xpartner.size()) * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_sp_scr, c->h_sp_scrval.data(), c->h_sp_scrval.size() * sizeof(double), c->stream));
+    YG_TRY(upload(c->d_df_scr, c->h_df_sp_scrval.data(), c->h_df_sp_scrval.size() * sizeof(double), c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    c->have_sparsity = 1;
+    c->have_df = 0;
+    return YGPU_OK;
+}
+
+static int unpack_to_reference(ygpu_ctx *c, const double *dsrc, double *host_out)
+{
+    // device packed rows (n_up x naux_pad) -> host reference layout (nnz x naux, both triangles)
+    const int nbf = c->nbf, naux = c->naux;
+    std::vector<double> up((size_t) c->n_up * c->naux_pad);
+    YG_HIP(hipMemcpyAsync(up.data(), dsrc, up.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nbf; ++i)
+        for (int j = 0; j < nbf; ++j) {
+            const int r = c->h_mask[(size_t) i * nbf + j];
+            if (r < 0) continue;
+            const int u = c->h_upmask[(size_t) std::min(i, j) * nbf + std::max(i, j)];
+            std::memcpy(host_out + (size_t) r * naux, up.data() + (size_t) u * c->naux_pad, naux * sizeof(double));
+        }
+    return YGPU_OK;
+}
+
+int ygpu_build_df_tensor_ex(ygpu_ctx_t *c, double *pqA_out, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
+{
+    YG_TRY(check_ctx(c));
+    if (!c->have_sparsity) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_df_tensor: call ygpu_set_sparsity first");
+        return YGPU_ERR_INVALID;
+    }
+    const long ld = c->naux_pad;
+    const size_t tsz = (size_t) c->n_up * ld * sizeof(double);
+    YG_TRY(c->d_pqA.ensure(tsz));
+    YG_TRY(c->d_B.ensure(tsz));
+    YG_TRY(c->d_Jpq.ensure((size_t) ld * ld * sizeof(double)));
+    YG_TRY(c->d_mat0.ensure((size_t) ld * ld * sizeof(double)));
+    YG_TRY(c->d_mat1.ensure((size_t) ld * ld * sizeof(double)));
+    YG_HIP(hipEventRecord(c->ev[0], c->stream));
+    YG_HIP(hipMemsetAsync(c->d_pqA.p, 0, tsz, c->stream));
+    YG_TRY(yg_launch_eri3(c, c->d_pqA.as<double>()));
+    YG_HIP(hipEventRecord(c->ev[1], c->stream));
+    YG_HIP(hipMemsetAsync(c->d_mat0.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
+    YG_TRY(yg_launch_eri2(c, c->d_mat0.as<double>(), (int) ld));
+    YG_HIP(hipEventRecord(c->ev[2], c->stream));
+    YG_HIP(hipMemsetAsync(c->d_Jpq.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
+    double mine = 0.0;
+    YG_TRY(inv_sqrt_device(c, c->naux, (int) ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
+                           c->d_Jpq.as<double>(), &mine));
+    YG_HIP(hipEventRecord(c->ev[3], c->stream));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, (int) c->n_up, (int) ld, (int) ld, 1.0, c->d_pqA.as<double>(), (int) ld,
+                    c->d_Jpq.as<double>(), (int) ld, 0.0, c->d_B.as<double>(), (int) ld));
+    YG_HIP(hipEventRecord(c->ev[4], c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (t) {
+        t->t_3c = elapsed(c->ev[0], c->ev[1]) * 1e-3;
+        t->t_2c = elapsed(c->ev[1], c->ev[2]) * 1e-3;
+        t->t_invsqrt = elapsed(c->ev[2], c->ev[3]) * 1e-3;
+        t->t_dfgemm = elapsed(c->ev[3], c->ev[4]) * 1e-3;
+    }
+    if (Jpq_out)
+        YG_HIP(hipMemcpy2D(Jpq_out, c->naux * sizeof(double), c->d_Jpq.p, ld * sizeof(double), c->naux * sizeof(double),
+                           c->naux, hipMemcpyDeviceToHost));
+    if (pqA_out) YG_TRY(unpack_to_reference(c, c->d_pqA.as<double>(), pqA_out));
+    if (df_out) YG_TRY(unpack_to_reference(c, c->d_B.as<double>(), df_out));
+    c->have_df = 1;
+    return YGPU_OK;
+}
+
+int ygpu_build_df_tensor(ygpu_ctx_t *c, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
+{
+    return ygpu_build_df_tensor_ex(c, nullptr, Jpq_out, df_out, t);
+}
+
+int ygpu_set_df_tensor(ygpu_ctx_t *c, const double *df)
+{
+    YG_TRY(check_ctx(c));
+    if (!c->have_sparsity || !df) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_df_tensor: call ygpu_set_sparsity first / null tensor");
+        return YGPU_ERR_INVALID;
+    }
+    const int nbf = c->nbf, naux = c->naux;
+    const long ld = c->naux_pad;
+    std::vector<double> up((size_t) c->n_up * ld, 0.0);
+    for (int i = 0; i < nbf; ++i)
+        for (int j = i; j < nbf; ++j) {
+            const int u = c->h_upmask[(size_t) i * nbf + j];
+            if (u < 0) continue;
+            const int r = c->h_mask[(size_t) i * nbf + j];
+            std::memcpy(up.data() + (size_t) u * ld, df + (size_t) r * naux, naux * sizeof(double));
+        }
+    YG_TRY(upload(c->d_B, up.data(), up.size() * sizeof(double), c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    c->have_df = 1;
+    return YGPU_OK;
+}
+
+int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Cocc, const double *H, double *J,
+                    double *K, double *F, ygpu_fock_timing_t *t)
+{
+    YG_TRY(check_ctx(c));
+    if (!c->have_df || n_occ <= 0 || !D || !Cocc || !H || !F) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built or bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    const int nbf = c->nbf;
+    const size_t ms = (size_t) nbf * nbf * sizeof(double);
+    YG_TRY(c->d_D.ensure(ms));
+    YG_TRY(c->d_H.ensure(ms));
+    YG_TRY(c->d_J.ensure(ms));
+    YG_TRY(c->d_K.ensure(ms));
+    YG_TRY(c->d_F.ensure(ms));
+    YG_TRY(c->d_Cocc.ensure((size_t) nbf * n_occ * sizeof(double)));
+    YG_TRY(c->d_X.ensure((size_t) nbf * n_occ * c->naux_pad * sizeof(double)));
+    YG_HIP(hipMemcpyAsync(c->d_D.p, D, ms, hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_H.p, H, ms, hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_Cocc.p, Cocc, (size_t) nbf * n_occ * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    float tj1 = 0.f, tj2 = 0.f;
+    YG_HIP(hipEventRecord(c->ev[5], c->stream));
+    YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr));
+    YG_HIP(hipEventRecord(c->ev[3], c->stream));
+    YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
+                     c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B.as<double>(), c->d_X.as<double>(),
+                     c->max_row_len));
+    YG_HIP(hipEventRecord(c->ev[4], c->stream));
+    YG_TRY(yg_syrk_upper(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<double>(), c->d_K.as<double>(),
+                         c->d_Kpart));
+    YG_HIP(hipEventRecord(c->ev[6], c->stream));
+    YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
+                            c->d_F.as<double>()));
+    YG_HIP(hipEventRecord(c->ev[7], c->stream));
+    YG_HIP(hipMemcpyAsync(F, c->d_F.p, ms, hipMemcpyDeviceToHost, c->stream));
+    if (J) YG_HIP(hipMemcpyAsync(J, c->d_J.p, ms, hipMemcpyDeviceToHost, c->stream));
+    if (K) YG_HIP(hipMemcpyAsync(K, c->d_K.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (t) {
+        tj1 = elapsed(c->ev[0], c->ev[1]);
+        tj2 = elapsed(c->ev[1], c->ev[2]);
+        t->t_tempJ = tj1 * 1e-3;
+        t->t_Jmat = tj2 * 1e-3;
+        t->t_tempK = elapsed(c->ev[3], c->ev[4]) * 1e-3;
+        t->t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
+        t->t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
+        t->t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
+    }
+    return YGPU_OK;
+}
+
+int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double *X, double *D, double *Cocc,
+                       double *eigval, int *ev_idx)
+{
+    YG_TRY(check_ctx(c));
+    const int nbf = c->nbf;
+    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || !Fo || !X || !D || !Cocc || !eigval) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_density: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) nbf * nbf * sizeof(double);
+    YG_TRY(c->d_mat0.ensure(ms));
+    YG_TRY(c->d_mat1.ensure(ms));
+    YG_TRY(c->d_mat2.ensure(ms));
+    YG_TRY(c->d_D.ensure(ms));
+    YG_TRY(c->d_Cocc.ensure((size_t) nbf * n_occ * sizeof(double)));
+    YG_TRY(c->d_eig.ensure(nbf * sizeof(double)));
+    YG_TRY(c->d_evE.ensure(nbf * sizeof(double)));
+    YG_TRY(c->d_info.ensure(sizeof(int)));
+    YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
+    rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf, c->d_mat0.as<double>(),
+                                         nbf, c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
+    if (st != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", nbf, (int) st);
+        return YGPU_ERR_SOLVER;
+    }
+    // C = X V: with V column-major in d_mat0, V(k,n) = d_mat0[n*nbf+k] = op_T access
+    YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, c->d_mat1.as<double>(), nbf, c->d_mat0.as<double>(), nbf, 0.0,
+                    c->d_mat2.as<double>(), nbf));
+    YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
+    YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, c->d_Cocc.as<double>(), n_occ,
+                    0.0, c->d_D.as<double>(), nbf));
+    int info = 0;
+    YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(eigval, c->d_eig.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(D, c->d_D.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(Cocc, c->d_Cocc.p, (size_t) nbf * n_occ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    if (info != 0) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) info=%d", nbf, info);
+        return YGPU_ERR_SOLVER;
+    }
+    if (ev_idx)
+        for (int i = 0; i < nbf; ++i) ev_idx[i] = i;
+    return YGPU_OK;
+}
+
+int ygpu_dgemm(ygpu_ctx_t *c, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+               const double *B, int ldb, double beta, double *C, int ldc)
+{
+    YG_TRY(check_ctx(c));
+    if (m < 0 || n < 0 || k < 0 || !A || !B || !C) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_dgemm: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    if (m == 0 || n == 0) return YGPU_OK;
+    const size_t arows = ta ? k : m, brows = tb ? n : k;
+    const size_t asz = arows * lda * sizeof(double), bsz = brows * ldb * sizeof(double), csz = (size_t) m * ldc * sizeof(double);
+    YG_TRY(c->d_gA.ensure(asz));
+    YG_TRY(c->d_gB.ensure(bsz));
+    YG_TRY(c->d_gC.ensure(csz));
+    YG_HIP(hipMemcpyAsync(c->d_gA.p, A, asz, hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_gB.p, B, bsz, hipMemcpyHostToDevice, c->stream));
+    if (beta != 0.0) YG_HIP(hipMemcpyAsync(c->d_gC.p, C, csz, hipMemcpyHostToDevice, c->stream));
+    YG_TRY(yg_dgemm(c->stream, ta, tb, m, n, k, alpha, c->d_gA.as<double>(), lda, c->d_gB.as<double>(), ldb, beta,
+                    c->d_gC.as<double>(), ldc));
+    YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+}
+
+int ygpu_layout_info(ygpu_ctx_t *c, long *n_up, long *nnz, int *naux_pad)
+{
+    YG_TRY(check_ctx(c));
+    if (n_up) *n_up = c->n_up;
+    if (nnz) *nnz = c->nnz;
+    if (naux_pad) *naux_pad = c->naux_pad;
+    return YGPU_OK;
+}
+
+}  // extern "C"

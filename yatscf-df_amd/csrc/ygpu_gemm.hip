@@ -1,0 +1,273 @@
+// ygpu_gemm.hip -- FP64 MFMA GEMM family for gfx950.
+//
+// One tile engine serves every matrix product of the hot path:
+//   B = pqA * J^{-1/2}                       (build_DF_tensor.c:239-254)
+//   X_i = C_occ(partners of i)^T B(rows of i) (build_Fock.c:201-285, "aux. K tensor")
+//   K = X X^T, upper tiles, split-K          (build_Fock.c:317-341, "K matrix")
+//   C = X C0, D = C_occ C_occ^T               (build_density.c:56-68)
+//   U diag U^T of the inverse square roots    (build_DF_tensor.c:205-237)
+//   the DIIS products                         (DIIS.c:34-107)
+// Workgroup tile 64x64, k-step 16, 256 threads = 4 wavefronts; each wavefront
+// owns a 32x32 sub-tile computed as 2x2 v_mfma_f64_16x16x4f64 blocks from LDS.
+// Operands are staged global -> registers -> LDS with the next k-step's loads
+// issued before the current step's MFMAs.
+#include "ygpu_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16, NTHR = 256, PAD = 2;
+
+enum { MODE_N = 0, MODE_T = 1, MODE_GATHER_T = 2, MODE_GATHER_N = 3 };
+
+// element (m,k) of A / (k,n) of B for the four storage modes
+struct Operand {
+    const double *p;
+    long ld;
+    const int *rows;   // gather: row index per k
+};
+
+// A tile: As[k][m].  N: A[m*ld+k].  T: A[k*ld+m].  GATHER_T: A[rows[k]*ld+m]
+template <int MODE>
+__device__ __forceinline__ void load_a(const Operand &A, int M, int K, int m0, int k0, double (&r)[4])
+{
+    const int t = threadIdx.x;
+    if (MODE == MODE_N) {
+        const int m = m0 + (t >> 2), kq = k0 + (t & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = (m < M && kq + q < K) ? A.p[(size_t) m * A.ld + kq + q] : 0.0;
+    } else {
+        const int k = k0 + (t >> 4), mq = m0 + (t & 15) * 4;
+        const bool kin = k < K;
+        size_t row = 0;
+        if (kin) row = (MODE == MODE_GATHER_T) ? (size_t) A.rows[k] : (size_t) k;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = (kin && mq + q < M) ? A.p[row * A.ld + mq + q] : 0.0;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void store_a(double (*As)[BM + PAD], const double (&r)[4])
+{
+    const int t = threadIdx.x;
+    if (MODE == MODE_N) {
+        const int m = t >> 2, kq = (t & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) As[kq + q][m] = r[q];
+    } else {
+        const int k = t >> 4, mq = (t & 15) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) As[k][mq + q] = r[q];
+    }
+}
+
+// B tile: Bs[k][n].  N: B[k*ld+n].  T: B[n*ld+k].  GATHER_N: B[rows[k]*ld+n]
+template <int MODE>
+__device__ __forceinline__ void load_b(const Operand &B, int N, int K, int n0, int k0, double (&r)[4])
+{
+    const int t = threadIdx.x;
+    if (MODE == MODE_T) {
+        const int n = n0 + (t >> 2), kq = k0 + (t & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = (n < N && kq + q < K) ? B.p[(size_t) n * B.ld + kq + q] : 0.0;
+    } else {
+        const int k = k0 + (t >> 4), nq = n0 + (t & 15) * 4;
+        const bool kin = k < K;
+        size_t row = 0;
+        if (kin) row = (MODE == MODE_GATHER_N) ? (size_t) B.rows[k] : (size_t) k;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = (kin && nq + q < N) ? B.p[row * B.ld + nq + q] : 0.0;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&r)[4])
+{
+    const int t = threadIdx.x;
+    if (MODE == MODE_T) {
+        const int n = t >> 2, kq = (t & 3) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Bs[kq + q][n] = r[q];
+    } else {
+        const int k = t >> 4, nq = (t & 15) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Bs[k][nq + q] = r[q];
+    }
+}
+
+// Core: accumulate a 64x64 tile over k in [kbeg, kend)
+template <int MA, int MB>
+__device__ __forceinline__ void tile_mma(const Operand &A, const Operand &B, int M, int N, int K, int m0, int n0,
+                                         int kbeg, int kend, v4d (&acc)[2][2])
+{
+    __shared__ double As[BK][BM + PAD];
+    __shared__ double Bs[BK][BN + PAD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    double ra[4], rb[4];
+    if (kbeg < kend) {
+        load_a<MA>(A, M, kend, m0, kbeg, ra);
+        load_b<MB>(B, N, kend, n0, kbeg, rb);
+    }
+    (void) K;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+        store_a<MA>(As, ra);
+        store_b<MB>(Bs, rb);
+        __syncthreads();
+        if (k0 + BK < kend) {
+            load_a<MA>(A, M, kend, m0, k0 + BK, ra);
+            load_b<MB>(B, N, kend, n0, k0 + BK, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), c = lane & 15;
+            const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+            const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    }
+}
+
+// D[i][j] layout of v_mfma_f64_16x16x4f64: lane l, element v -> row 4*(l/16)+v, col l%16
+template <class F>
+__device__ __forceinline__ void tile_epilogue(int m0, int n0, const v4d (&acc)[2][2], F &&store)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + v;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                store(m, n, acc[i][j][v]);
+            }
+}
+
+template <int MA, int MB>
+__global__ __launch_bounds__(NTHR) void k_gemm(Operand A, Operand B, int M, int N, int K, double alpha,
+                                               double beta, double *__restrict__ C, long ldc)
+{
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    v4d acc[2][2];
+    tile_mma<MA, MB>(A, B, M, N, K, m0, n0, 0, K, acc);
+    tile_epilogue(m0, n0, acc, [&](int m, int n, double v) {
+        if (m < M && n < N) {
+            double *c = C + (size_t) m * ldc + n;
+            *c = (beta == 0.0) ? alpha * v : alpha * v + beta * *c;
+        }
+    });
+}
+
+// X_i = Cocc[partners]^T * B[up rows], one z-slice per basis function i
+__global__ __launch_bounds__(NTHR) void k_xbuild(const int *__restrict__ xptr, const int *__restrict__ xup,
+                                                 const int *__restrict__ xpartner, const double *__restrict__ Cocc,
+                                                 const double *__restrict__ Bup, double *__restrict__ X, int n_occ,
+                                                 int naux_pad)
+{
+    const int i = blockIdx.z;
+    const int kb = xptr[i], klen = xptr[i + 1] - kb;
+    Operand A{Cocc, n_occ, xpartner + kb};
+    Operand B{Bup, naux_pad, xup + kb};
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    v4d acc[2][2];
+    tile_mma<MODE_GATHER_T, MODE_GATHER_N>(A, B, n_occ, naux_pad, klen, m0, n0, 0, klen, acc);
+    double *Xi = X + (size_t) i * n_occ * naux_pad;
+    tile_epilogue(m0, n0, acc, [&](int m, int n, double v) {
+        if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = v;
+    });
+}
+
+// upper tiles of X X^T; blockIdx.x = linear upper-tile id, blockIdx.z = k split
+__global__ __launch_bounds__(NTHR) void k_syrk_part(const double *__restrict__ X, int nbf, long kdim, int ntb,
+                                                    long kchunk, double *__restrict__ part)
+{
+    int t = blockIdx.x, bi = 0;
+    while (t >= ntb - bi) { t -= ntb - bi; ++bi; }
+    const int bj = bi + t;
+    const int m0 = bi * BM, n0 = bj * BN;
+    const long kb = (long) blockIdx.z * kchunk;
+    const long ke = std::min<long>(kdim, kb + kchunk);
+    // offset the operands so the k range starts at 0 (int k indices inside the tile engine)
+    Operand A{X + kb, kdim, nullptr};
+    Operand B{X + kb, kdim, nullptr};
+    v4d acc[2][2];
+    tile_mma<MODE_N, MODE_T>(A, B, nbf, nbf, (int)(ke - kb), m0, n0, 0, (int)(ke - kb), acc);
+    double *P = part + (size_t) blockIdx.z * nbf * nbf;
+    tile_epilogue(m0, n0, acc, [&](int m, int n, double v) {
+        if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = v;
+    });
+}
+
+// K[m][n] = K[n][m] = sum_z part[z][m][n] for m <= n
+__global__ void k_syrk_reduce(const double *__restrict__ part, int nsplit, int nbf, double *__restrict__ K)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) nbf * nbf) return;
+    const int m = (int)(idx / nbf), n = (int)(idx % nbf);
+    if (m > n) return;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += part[(size_t) z * nbf * nbf + idx];
+    K[(size_t) m * nbf + n] = s;
+    K[(size_t) n * nbf + m] = s;
+}
+
+}  // namespace
+
+// ===========================================================================
+int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+             const double *B, int ldb, double beta, double *C, int ldc)
+{
+    if (m <= 0 || n <= 0) return YGPU_OK;
+    dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
+    Operand a{A, lda, nullptr}, b{B, ldb, nullptr};
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<MODE_N, MODE_N>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<MODE_N, MODE_T>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<MODE_T, MODE_N>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
+    else hipLaunchKernelGGL((k_gemm<MODE_T, MODE_T>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup, const int *xpartner,
+              const double *Cocc, const double *B, double *X, int max_row_len)
+{
+    (void) max_row_len;
+    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
+    dim3 grid((naux_pad + BN - 1) / BN, (n_occ + BM - 1) / BM, nbf);
+    hipLaunchKernelGGL(k_xbuild, grid, dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X, n_occ, naux_pad);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part)
+{
+    const int ntb = (nbf + BM - 1) / BM;
+    const int ntiles = ntb * (ntb + 1) / 2;
+    // enough workgroups to fill the chip, chunks of at least 512 along k
+    int nsplit = (int) std::max<long>(1, std::min<long>((2048 + ntiles - 1) / ntiles, (kdim + 511) / 512));
+    long kchunk = (kdim + nsplit - 1) / nsplit;
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    nsplit = (int) ((kdim + kchunk - 1) / kchunk);
+    YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(double)));
+    hipLaunchKernelGGL(k_syrk_part, dim3(ntiles, 1, nsplit), dim3(NTHR), 0, s, X, nbf, kdim, ntb, kchunk,
+                       part.as<double>());
+    YG_HIP(hipGetLastError());
+    const long tot = (long) nbf * nbf;
+    hipLaunchKernelGGL(k_syrk_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<double>(), nsplit, nbf, K);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}

@@ -1,0 +1,109 @@
+// ygpu_internal.h -- device context and shared declarations of the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/yatscf_gpu.h"
+#include "md.h"
+
+// --------------------------------------------------------------------------
+// error reporting: every entry point returns a status and leaves a message
+void ygpu_set_error(int code, const char *fmt, ...);
+
+#define YG_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ygpu_set_error(YGPU_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                           hipGetErrorString(e_), __FILE__, __LINE__);            \
+            return YGPU_ERR_HIP;                                                  \
+        }                                                                         \
+    } while (0)
+
+#define YG_TRY(call)                 \
+    do {                             \
+        int s_ = (call);             \
+        if (s_ != YGPU_OK) return s_; \
+    } while (0)
+
+// growable device buffer
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    template <class T> T *as() const { return (T *) p; }
+};
+
+struct BasisDev {
+    int nshells = 0, nbf = 0, max_am = 0, max_nprim = 0;
+    std::vector<int> am, nprim, prim_off, bf_start;
+    std::vector<double> xyz, alpha, coef;
+    DBuf d_am, d_nprim, d_prim_off, d_bf_start, d_xyz, d_alpha, d_coef;
+    DevShells dev() const {
+        return DevShells{d_am.as<int>(), d_nprim.as<int>(), d_prim_off.as<int>(), d_bf_start.as<int>(),
+                         d_xyz.as<double>(), d_alpha.as<double>(), d_coef.as<double>()};
+    }
+};
+
+struct ygpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle rb = nullptr;
+    hipEvent_t ev[8] = {};
+    DBuf boys;                    // Boys table
+    BasisDev bas, aux;
+    int natoms = 0;
+    DBuf d_charge, d_atom_xyz;
+
+    // sparsity (set by ygpu_set_sparsity)
+    int have_sparsity = 0;
+    int nbf = 0, naux = 0, naux_pad = 0;
+    long n_up = 0, nnz = 0;
+    std::vector<int> h_upmask;    // nbf*nbf -> up row or -1
+    std::vector<int> h_mask;      // reference CSR row of (i,j) or -1
+    DBuf d_upmask, d_up_i, d_up_j;
+    DBuf d_xptr, d_xup, d_xpartner;  // per bf i: up rows and partners (full CSR order)
+    int max_row_len = 0;
+    // screened 3c work list: per (la,lb) class, list of (lid, rid) shell pairs
+    std::vector<int> h_sp_lid, h_sp_rid;
+    DBuf d_sp_scr, d_df_scr;
+    double scrtol2 = 1e-22;
+    int num_uniq_sp = 0;
+    std::vector<int> h_uniq_lid, h_uniq_rid;
+    std::vector<double> h_sp_scrval, h_df_sp_scrval;
+
+    // DF tensor
+    int have_df = 0;
+    DBuf d_pqA, d_B, d_Jpq;
+
+    // Fock / density work buffers
+    DBuf d_D, d_Cocc, d_H, d_J, d_K, d_F, d_tempJ, d_tempJpart, d_X, d_Kpart;
+    DBuf d_mat0, d_mat1, d_mat2, d_eig, d_info, d_evE;
+    DBuf d_gA, d_gB, d_gC;
+};
+
+// --------------------------------------------------------------------------
+// launchers implemented in the .hip files
+int yg_launch_ovlp_hcore(ygpu_ctx *c, double *dS, double *dH);
+int yg_launch_schwarz(ygpu_ctx *c, int quirk_a5, double *d_bf_scr, double *d_df_scr);
+int yg_launch_eri2(ygpu_ctx *c, double *dJpq, int ld);
+int yg_launch_eri3(ygpu_ctx *c, double *dpqA);
+
+// C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA
+int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+             const double *B, int ldb, double beta, double *C, int ldc);
+// X[i] = Cocc(partners of row i)^T * Bup(rows of i) for every basis function i
+int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+              const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
+// upper-triangle K = X X^T with split-K partials in Kpart, result in K (full, mirrored)
+int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part);
+int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, float *t_Jmat_ms);
+int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const double *K, double *F);
+int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);
+int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);
+int yg_transpose(hipStream_t s, int rows, int cols, const double *A, double *AT);

@@ -1,0 +1,263 @@
+/*
+ * TinySCF.c -- setup and SCF driver (drop-in for the reference's
+ * TinySCF.c:20-435).  Integrals and eigensolves go to the device engine;
+ * the screening logic, the pair layout, the SAD guess, the energy and the
+ * iteration loop stay on the host as in the reference.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "DIIS.h"
+#include "TinySCF.h"
+#include "build_Fock.h"
+#include "build_density.h"
+#include "utils.h"
+#include "yatscf_gpu.h"
+
+void TinySCF_compute_Hcore_Ovlp_mat(TinySCF_t T)
+{
+    double st = get_wtime_sec();
+    ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
+    YS_GPU(ygpu_ovlp_hcore(g, T->S_mat, T->Hcore_mat));
+    /* X = S^{-1/2} = U s^{-1/2} U^T (TinySCF.c:57-76) */
+    YS_GPU(ygpu_sym_inv_sqrt(g, T->nbasfuncs, T->S_mat, T->X_mat, NULL));
+    T->S_Hcore_time = get_wtime_sec() - st;
+    if (T->verbose)
+        printf("TinySCF precompute Hcore, S, and X matrices over,  elapsed time = %.3lf (s)\n", T->S_Hcore_time);
+}
+
+static int before(int M1, int N1, int M2, int N2) { return M1 == M2 ? N1 < N2 : M1 < M2; }
+
+/* quicksort of (M,N) pairs by M then N (TinySCF.c:91-110) */
+static void sort_pairs(int *M, int *N, int l, int r)
+{
+    int i = l, j = r;
+    const int pm = M[(l + r) / 2], pn = N[(l + r) / 2];
+    while (i <= j) {
+        while (before(M[i], N[i], pm, pn)) i++;
+        while (before(pm, pn, M[j], N[j])) j--;
+        if (i <= j) {
+            int t = M[i]; M[i] = M[j]; M[j] = t;
+            t = N[i]; N[i] = N[j]; N[j] = t;
+            i++;
+            j--;
+        }
+    }
+    if (l < j) sort_pairs(M, N, l, j);
+    if (i < r) sort_pairs(M, N, i, r);
+}
+
+void TinySCF_compute_sq_Schwarz_scrvals(TinySCF_t T)
+{
+    double st = get_wtime_sec();
+    const int ns = T->nshells;
+    YS_GPU(ygpu_schwarz((ygpu_ctx_t *) T->gpu, T->quirk_a5, T->sp_scrval, T->bf_pair_scrval, T->df_sp_scrval));
+    double gmax = 0.0, dfmax = 0.0;
+    for (int i = 0; i < ns * ns; i++) if (T->sp_scrval[i] > gmax) gmax = T->sp_scrval[i];
+    for (int i = 0; i < T->df_nshells; i++) if (T->df_sp_scrval[i] > dfmax) dfmax = T->df_sp_scrval[i];
+    T->max_scrval = gmax;
+    T->max_df_scrval = dfmax;
+    /* unique shell pairs surviving eta = tol^2 / max_P (P|P), AM(lid) >= AM(rid)
+     * (TinySCF.c:183-221); either orientation passing keeps the pair, so the list
+     * stays symmetric when the A5 index makes the two orientations differ */
+    const double eta = T->shell_scrtol2 / T->max_df_scrval;
+    int nnz = 0;
+    for (int M = 0; M < ns; M++)
+        for (int N = M; N < ns; N++) {
+            if (!(T->sp_scrval[M * ns + N] > eta || T->sp_scrval[N * ns + M] > eta)) continue;
+            const int amM = CMS_getShellDim(T->basis, M), amN = CMS_getShellDim(T->basis, N);
+            if (amM > amN) { T->uniq_sp_lid[nnz] = M; T->uniq_sp_rid[nnz] = N; }
+            else           { T->uniq_sp_lid[nnz] = N; T->uniq_sp_rid[nnz] = M; }
+            nnz++;
+        }
+    T->num_uniq_sp = nnz;
+    if (nnz > 0) sort_pairs(T->uniq_sp_lid, T->uniq_sp_rid, 0, nnz - 1);
+    T->shell_scr_time = get_wtime_sec() - st;
+    if (T->verbose)
+        printf("TinySCF precompute shell screening info over,      elapsed time = %.3lf (s)\n", T->shell_scr_time);
+}
+
+void TinySCF_prepare_sparsity(TinySCF_t T)
+{
+    double st = get_wtime_sec();
+    const int nbf = T->nbasfuncs;
+    const double eta = T->shell_scrtol2 / T->max_df_scrval;
+    int nnz = 0;
+    T->bf_mask_displs[0] = 0;
+    for (int i = 0; i < nbf; i++) {
+        for (int j = 0; j < nbf; j++) {
+            /* union of both orientations (see TinySCF_compute_sq_Schwarz_scrvals) */
+            if (T->bf_pair_scrval[(size_t) i * nbf + j] > eta || T->bf_pair_scrval[(size_t) j * nbf + i] > eta) {
+                T->bf_pair_mask[(size_t) i * nbf + j] = nnz;
+                T->bf_pair_j[nnz] = j;
+                nnz++;
+            } else {
+                T->bf_pair_mask[(size_t) i * nbf + j] = -1;
+            }
+        }
+        T->bf_pair_diag[i] = T->bf_pair_mask[(size_t) i * nbf + i];
+        T->bf_mask_displs[i + 1] = nnz;
+    }
+    T->num_bf_pair_scr = nnz;
+    YS_GPU(ygpu_set_sparsity((ygpu_ctx_t *) T->gpu, T->bf_pair_mask, T->num_uniq_sp, T->uniq_sp_lid, T->uniq_sp_rid,
+                             T->sp_scrval, T->df_sp_scrval, T->shell_scrtol2));
+    if (T->host_df_copy) {
+        size_t tsz = (size_t) nnz * T->df_nbf * DBL_SIZE;
+        T->pqA = (double *) ALIGN64B_MALLOC(tsz);
+        T->df_tensor = (double *) ALIGN64B_MALLOC(tsz);
+        if (!T->pqA || !T->df_tensor) {
+            fprintf(stderr, "TinySCF: host DF tensor copies (%zu bytes each) could not be allocated\n", tsz);
+            abort();
+        }
+        T->mem_size += 2.0 * (double) tsz;
+    }
+    if (T->verbose) {
+        printf("TinySCF handling shell pair sparsity over,         elapsed time = %.3lf (s)\n", get_wtime_sec() - st);
+        printf("TinySCF regular + density fitting memory usage = %.2lf MB \n", T->mem_size / 1048576.0);
+        printf("#### Sparsity of basis function pairs = %lf, %lf\n", (double) T->num_uniq_sp / T->nshellpairs,
+               (double) nnz / T->mat_size);
+    }
+}
+
+/*
+ * Pivoted Cholesky D = P L L^T P^T, lower, row-major, with LAPACK dpstf2
+ * semantics (first maximal diagonal is the pivot, stop when it is <= tol).
+ * On return A's lower triangle holds L in pivoted order, piv is 1-based.
+ */
+static int pivoted_cholesky(int n, double *A, int *piv, double tol)
+{
+    double *work = (double *) calloc(n, sizeof(double));
+    for (int i = 0; i < n; i++) piv[i] = i + 1;
+    int rank = n;
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        double best = -HUGE_VAL;
+        for (int i = j; i < n; i++) {
+            if (j > 0) work[i] += A[(size_t) i * n + j - 1] * A[(size_t) i * n + j - 1];
+            const double v = A[(size_t) i * n + i] - work[i];
+            if (v > best) { best = v; p = i; }
+        }
+        if (!(best > tol)) { A[(size_t) j * n + j] = best; rank = j; break; }
+        if (p != j) {
+            A[(size_t) p * n + p] = A[(size_t) j * n + j];
+            for (int c = 0; c < j; c++) {
+                double t = A[(size_t) j * n + c]; A[(size_t) j * n + c] = A[(size_t) p * n + c]; A[(size_t) p * n + c] = t;
+            }
+            for (int r = p + 1; r < n; r++) {
+                double t = A[(size_t) r * n + j]; A[(size_t) r * n + j] = A[(size_t) r * n + p]; A[(size_t) r * n + p] = t;
+            }
+            for (int c = j + 1; c < p; c++) {
+                double t = A[(size_t) c * n + j]; A[(size_t) c * n + j] = A[(size_t) p * n + c]; A[(size_t) p * n + c] = t;
+            }
+            double t = work[j]; work[j] = work[p]; work[p] = t;
+            int ip = piv[j]; piv[j] = piv[p]; piv[p] = ip;
+        }
+        const double ajj = sqrt(best);
+        A[(size_t) j * n + j] = ajj;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[(size_t) i * n + j];
+            for (int c = 0; c < j; c++) s -= A[(size_t) i * n + c] * A[(size_t) j * n + c];
+            A[(size_t) i * n + j] = s / ajj;
+        }
+    }
+    free(work);
+    return rank;
+}
+
+void TinySCF_D2Cocc(TinySCF_t T)
+{
+    const int nbf = T->nbasfuncs, n_occ = T->n_occ;
+    double *L = T->tmp_mat;
+    int *piv = (int *) malloc(sizeof(int) * nbf);
+    memcpy(L, T->D_mat, DBL_SIZE * T->mat_size);
+    const int rank = pivoted_cholesky(nbf, L, piv, 1e-12);            /* TinySCF.c:295 */
+    if (rank < n_occ)
+        for (int i = 0; i < nbf; i++)
+            for (int j = rank; j < n_occ; j++) L[(size_t) i * nbf + j] = 0.0;
+    if (T->quirk_a1) {
+        /* the reference ignores the pivot and drops the diagonal of rows < n_occ
+         * (TinySCF.c:304-318): this C_occ only feeds the first K build */
+        for (int i = 0; i < n_occ && i < nbf; i++) {
+            for (int j = 0; j < i; j++) T->Cocc_mat[(size_t) i * n_occ + j] = L[(size_t) i * nbf + j];
+            for (int j = i; j < n_occ; j++) T->Cocc_mat[(size_t) i * n_occ + j] = 0.0;
+        }
+        for (int i = n_occ; i < nbf; i++) memcpy(T->Cocc_mat + (size_t) i * n_occ, L + (size_t) i * nbf, DBL_SIZE * n_occ);
+    } else {
+        memset(T->Cocc_mat, 0, DBL_SIZE * (size_t) nbf * n_occ);
+        for (int i = 0; i < nbf; i++)
+            for (int j = 0; j < n_occ && j <= i && j < rank; j++)
+                T->Cocc_mat[(size_t) (piv[i] - 1) * n_occ + j] = L[(size_t) i * nbf + j];
+    }
+    free(piv);
+}
+
+void TinySCF_get_initial_guess(TinySCF_t T)
+{
+    const int nbf = T->nbasfuncs;
+    memset(T->D_mat, 0, DBL_SIZE * T->mat_size);
+    double t0 = get_wtime_sec();
+    for (int a = 0; a < T->natoms; a++) {
+        double *guess;
+        int spos, epos;
+        CMS_getInitialGuess(T->basis, a, &guess, &spos, &epos);
+        const int ldg = epos - spos + 1;
+        copy_matrix_block(T->D_mat + (size_t) spos * nbf + spos, nbf, guess, ldg, ldg, ldg);
+    }
+    double R = 1.0;
+    if (T->charge != 0 && T->electron != 0) R = (double) (T->electron - T->charge) / (double) T->electron;
+    R *= 0.5;
+    for (int i = 0; i < T->mat_size; i++) T->D_mat[i] *= R;
+    T->nuc_energy = CMS_getNucEnergy(T->basis);
+    double t1 = get_wtime_sec();
+    TinySCF_D2Cocc(T);
+    double t2 = get_wtime_sec();
+    if (T->verbose) printf("TinySCF SAD initial guess: D mat / Cocc mat used %lf, %lf (s)\n", t1 - t0, t2 - t1);
+}
+
+static void TinySCF_calc_energy(TinySCF_t T)
+{
+    double e = 0.0;
+    for (int i = 0; i < T->mat_size; i++) e += T->D_mat[i] * (T->F_mat[i] + T->Hcore_mat[i]);
+    T->HF_energy = e;
+}
+
+void TinySCF_do_SCF(TinySCF_t T)
+{
+    if (T->verbose) {
+        printf("TinySCF SCF iteration started...\n");
+        printf("Nuclear energy = %.10lf\n", T->nuc_energy);
+    }
+    T->iter = 0;
+    T->converged = 0;
+    double prev = 0.0, delta = 223.0;
+    while (T->iter < T->niters && delta >= T->ene_tol) {
+        double st0 = get_wtime_sec();
+        if (T->verbose) printf("--------------- Iteration %d ---------------\n", T->iter);
+        TinySCF_build_FockMat(T);
+        TinySCF_calc_energy(T);
+        delta = fabs(T->HF_energy - prev);
+        prev = T->HF_energy;
+        double st1 = get_wtime_sec();
+        TinySCF_DIIS(T);
+        double st2 = get_wtime_sec();
+        TinySCF_build_DenMat(T);
+        double st3 = get_wtime_sec();
+        T->t_diis = st2 - st1;
+        T->t_density = st3 - st2;
+        if (T->verbose) {
+            printf("* DIIS procedure        : %.3lf (s)\n", st2 - st1);
+            printf("* Build density matrix  : %.3lf (s)\n", st3 - st2);
+            printf("* Iteration runtime     = %.3lf (s)\n", st3 - st0);
+            printf("* Energy = %.10lf (%.10lf)", T->HF_energy + T->nuc_energy, T->HF_energy);
+            if (T->iter > 0) printf(", delta = %e\n", delta);
+            else printf("\n");
+        }
+        if (T->iter == 0) delta = 223.0;   /* iteration 0 never converges (TinySCF.c:429) */
+        T->iter++;
+    }
+    T->energy_delta = delta;
+    T->converged = delta < T->ene_tol;
+    if (T->verbose) printf("--------------- SCF iterations finished ---------------\n");
+}

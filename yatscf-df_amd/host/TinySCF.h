@@ -1,0 +1,131 @@
+/*
+ * TinySCF.h -- the TinySCF engine struct and lifecycle API (drop-in for the
+ * reference's TinySCF.h:12-138).
+ *
+ * Field names, meanings and layouts are the reference's.  Differences:
+ *  - `gpu` (the device engine, include/yatscf_gpu.h) replaces the Simint
+ *    handle; integrals, the DF tensor, J/K and the density build run there;
+ *  - temp_J / temp_K and the MKL batch-GEMM arrays stay unallocated on the
+ *    host (the device owns that work);
+ *  - pqA and df_tensor are host copies made only when `host_df_copy` is set
+ *    (the device keeps the packed tensor; the host copies are 2 x nnz x naux
+ *    doubles in the reference layout);
+ *  - quirk switches for the reference behaviours listed in DESIGN.md, and a
+ *    convergence flag.
+ */
+#ifndef _YATSCF_TFOCK_H_
+#define _YATSCF_TFOCK_H_
+
+#include "CMS.h"
+
+#define MAX_DIIS 10
+#define MIN_DIIS 2
+
+struct TinySCF_struct
+{
+    int    nthreads;
+    int    max_buf_size;
+    double *Accum_Fock_buf;
+
+    BasisSet_t basis;
+    BasisSet_t df_basis;
+    int natoms, nshells;
+    int nbasfuncs, n_occ;
+    int charge, electron;
+    int df_nbf, df_nshells;
+
+    int nshellpairs;
+    int num_uniq_sp;
+    int mat_size;
+    int max_dim;
+
+    int    niters, iter;
+    double nuc_energy;
+    double HF_energy;
+    double ene_tol;
+
+    double shell_scrtol2;
+    double max_scrval;
+    double max_df_scrval;
+    double *sp_scrval;
+    double *bf_pair_scrval;
+    int    *uniq_sp_lid;
+    int    *uniq_sp_rid;
+    double *df_sp_scrval;
+    int    *bf_pair_mask;
+    int    *bf_pair_j;
+    int    *bf_pair_diag;
+    int    *bf_mask_displs;
+    int    num_bf_pair_scr;
+
+    void   *gpu;            /* ygpu_ctx_t* (replaces Simint_t simint) */
+    int *shell_bf_sind;
+    int *shell_bf_num;
+    int *df_shell_bf_sind;
+    int *df_shell_bf_num;
+
+    double *Hcore_mat;
+    double *S_mat;
+    double *F_mat;
+    double *D_mat;
+    double *J_mat;
+    double *K_mat;
+    double *X_mat;
+    double *Cocc_mat;
+    double *eigval;
+    int    *ev_idx;
+    double *tmp_mat;
+
+    double *pqA, *Jpq, *df_tensor;
+    double *temp_J, *temp_K;
+    int    df_nbf_16;
+
+    double *F0_mat;
+    double *R_mat;
+    double *B_mat;
+    double *FDS_mat;
+    double *DIIS_rhs;
+    int    *DIIS_ipiv;
+    int    DIIS_len;
+    int    DIIS_bmax_id;
+    double DIIS_bmax;
+
+    int mat_K_BS, *mat_K_group_size, mat_K_ntiles;
+    int *mat_K_transa, *mat_K_transb;
+    int *mat_K_m,   *mat_K_n,   *mat_K_k;
+    int *mat_K_lda, *mat_K_ldb, *mat_K_ldc;
+    double *mat_K_alpha, *mat_K_beta;
+    double **mat_K_a, **mat_K_b, **mat_K_c;
+
+    double mem_size, init_time, S_Hcore_time, shell_scr_time;
+
+    /* additions */
+    int    quirk_a5;        /* reference Schwarz index (TinySCF.c:153), default 1 */
+    int    quirk_a1;        /* reference D2Cocc (TinySCF.c:304-318), default 1 */
+    int    host_df_copy;    /* keep host pqA / df_tensor copies, default 0 */
+    int    verbose;         /* reference-style printout, default 1 */
+    int    converged;
+    double energy_delta;
+    double t_df[4];         /* 3c, 2c, inv-sqrt, DF GEMM (s) */
+    double t_fock[6];       /* aux J, J, aux K, K, symm, total (s) of the last Fock build */
+    double t_density, t_diis;
+};
+
+typedef struct TinySCF_struct* TinySCF_t;
+
+void init_TinySCF(TinySCF_t TinySCF, char *bas_fname, char *df_bas_fname, char *xyz_fname, const int niters);
+void free_TinySCF(TinySCF_t TinySCF);
+void TinySCF_init_batch_dgemm_arrays(TinySCF_t TinySCF);
+void TinySCF_free_batch_dgemm_arrays(TinySCF_t TinySCF);
+void TinySCF_compute_Hcore_Ovlp_mat(TinySCF_t TinySCF);
+void TinySCF_compute_sq_Schwarz_scrvals(TinySCF_t TinySCF);
+void TinySCF_prepare_sparsity(TinySCF_t TinySCF);
+void TinySCF_get_initial_guess(TinySCF_t TinySCF);
+void TinySCF_do_SCF(TinySCF_t TinySCF);
+void TinySCF_D2Cocc(TinySCF_t TinySCF);
+
+/* Status-returning variant of init_TinySCF (0 = success); init_TinySCF
+ * aborts with the message instead, keeping the reference's void contract. */
+int  TinySCF_try_init(TinySCF_t TinySCF, char *bas_fname, char *df_bas_fname, char *xyz_fname, const int niters);
+
+#endif

@@ -1,0 +1,92 @@
+/*
+ * tinyscf_pyapi.c -- accessors that let Python (tests, bench.py) drive the
+ * TinySCF host engine through ctypes without mirroring the struct layout.
+ * Not part of the reference API.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "TinySCF.h"
+
+TinySCF_t tinyscf_alloc(void) { return (TinySCF_t) calloc(1, sizeof(struct TinySCF_struct)); }
+
+double *tinyscf_dptr(TinySCF_t T, const char *n, size_t *count)
+{
+    const size_t ms = (size_t) T->mat_size;
+#define RET(name, ptr, cnt) if (!strcmp(n, name)) { *count = (ptr) ? (cnt) : 0; return (ptr); }
+    RET("Hcore_mat", T->Hcore_mat, ms) RET("S_mat", T->S_mat, ms) RET("F_mat", T->F_mat, ms)
+    RET("D_mat", T->D_mat, ms) RET("J_mat", T->J_mat, ms) RET("K_mat", T->K_mat, ms)
+    RET("X_mat", T->X_mat, ms) RET("tmp_mat", T->tmp_mat, ms)
+    RET("Cocc_mat", T->Cocc_mat, (size_t) T->nbasfuncs * T->n_occ) RET("eigval", T->eigval, (size_t) T->nbasfuncs)
+    RET("sp_scrval", T->sp_scrval, (size_t) T->nshellpairs) RET("bf_pair_scrval", T->bf_pair_scrval, ms)
+    RET("df_sp_scrval", T->df_sp_scrval, (size_t) T->df_nshells)
+    RET("Jpq", T->Jpq, (size_t) T->df_nbf * T->df_nbf)
+    RET("pqA", T->pqA, (size_t) T->num_bf_pair_scr * T->df_nbf)
+    RET("df_tensor", T->df_tensor, (size_t) T->num_bf_pair_scr * T->df_nbf)
+    RET("B_mat", T->B_mat, (size_t) (MAX_DIIS + 1) * (MAX_DIIS + 1))
+    RET("t_df", T->t_df, 4) RET("t_fock", T->t_fock, 6)
+#undef RET
+    *count = 0;
+    return NULL;
+}
+
+int *tinyscf_iptr(TinySCF_t T, const char *n, size_t *count)
+{
+#define RET(name, ptr, cnt) if (!strcmp(n, name)) { *count = (ptr) ? (cnt) : 0; return (ptr); }
+    RET("bf_pair_mask", T->bf_pair_mask, (size_t) T->mat_size)
+    RET("bf_pair_j", T->bf_pair_j, (size_t) T->num_bf_pair_scr)
+    RET("bf_pair_diag", T->bf_pair_diag, (size_t) T->nbasfuncs)
+    RET("bf_mask_displs", T->bf_mask_displs, (size_t) T->nbasfuncs + 1)
+    RET("uniq_sp_lid", T->uniq_sp_lid, (size_t) T->num_uniq_sp)
+    RET("uniq_sp_rid", T->uniq_sp_rid, (size_t) T->num_uniq_sp)
+    RET("shell_bf_sind", T->shell_bf_sind, (size_t) T->nshells + 1)
+    RET("ev_idx", T->ev_idx, (size_t) T->nbasfuncs)
+#undef RET
+    *count = 0;
+    return NULL;
+}
+
+int tinyscf_geti(TinySCF_t T, const char *n)
+{
+    if (!strcmp(n, "nbasfuncs")) return T->nbasfuncs;
+    if (!strcmp(n, "df_nbf")) return T->df_nbf;
+    if (!strcmp(n, "nshells")) return T->nshells;
+    if (!strcmp(n, "df_nshells")) return T->df_nshells;
+    if (!strcmp(n, "n_occ")) return T->n_occ;
+    if (!strcmp(n, "natoms")) return T->natoms;
+    if (!strcmp(n, "iter")) return T->iter;
+    if (!strcmp(n, "niters")) return T->niters;
+    if (!strcmp(n, "num_uniq_sp")) return T->num_uniq_sp;
+    if (!strcmp(n, "num_bf_pair_scr")) return T->num_bf_pair_scr;
+    if (!strcmp(n, "converged")) return T->converged;
+    if (!strcmp(n, "DIIS_len")) return T->DIIS_len;
+    if (!strcmp(n, "charge")) return T->charge;
+    if (!strcmp(n, "electron")) return T->electron;
+    return -1;
+}
+
+void tinyscf_seti(TinySCF_t T, const char *n, int v)
+{
+    if (!strcmp(n, "iter")) T->iter = v;
+    else if (!strcmp(n, "niters")) T->niters = v;
+    else if (!strcmp(n, "verbose")) T->verbose = v;
+}
+
+double tinyscf_getd(TinySCF_t T, const char *n)
+{
+    if (!strcmp(n, "HF_energy")) return T->HF_energy;
+    if (!strcmp(n, "nuc_energy")) return T->nuc_energy;
+    if (!strcmp(n, "total_energy")) return T->HF_energy + T->nuc_energy;
+    if (!strcmp(n, "energy_delta")) return T->energy_delta;
+    if (!strcmp(n, "max_df_scrval")) return T->max_df_scrval;
+    if (!strcmp(n, "t_density")) return T->t_density;
+    if (!strcmp(n, "t_diis")) return T->t_diis;
+    return 0.0;
+}
+
+void tinyscf_setd(TinySCF_t T, const char *n, double v)
+{
+    if (!strcmp(n, "ene_tol")) T->ene_tol = v;
+}
+
+void *tinyscf_gpu(TinySCF_t T) { return T->gpu; }

@@ -1,0 +1,338 @@
+"""ctypes bindings of the MI355X DF-RHF path.
+
+* :class:`Engine` -- the C-ABI of ``libyatscf_gpu.so`` (include/yatscf_gpu.h),
+  one device engine per instance.
+* :class:`TinySCF` -- the TinySCF host engine ``libtinyscf.so`` (the drop-in
+  for the reference's TinySCF API) driven step by step.
+
+Both raise :class:`GpuError` on any non-zero status.  There is no fallback:
+if the shared libraries are missing, importing them fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB = os.path.join(HERE, "libyatscf_gpu.so")
+HOST_LIB = os.path.join(HERE, "libtinyscf.so")
+DATA = os.path.join(HERE, "data")
+BASIS = os.path.join(DATA, "cc-pvdz", "cc-pvdz.gbs")
+AUXBASIS = os.path.join(DATA, "cc-pVDZ-RI", "cc-pVDZ-RI.gbs")
+
+YGPU_OK = 0
+ERRORS = {1: "INVALID", 2: "HIP", 3: "NOMEM", 4: "SOLVER", 5: "NUMERIC", 6: "NODEVICE"}
+
+dp = ctypes.POINTER(ctypes.c_double)
+ip = ctypes.POINTER(ctypes.c_int)
+vp = ctypes.c_void_p
+
+
+class GpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(status, "?"), status, msg))
+        self.status = status
+
+
+class DfTiming(ctypes.Structure):
+    _fields_ = [("t_3c", ctypes.c_double), ("t_2c", ctypes.c_double),
+                ("t_invsqrt", ctypes.c_double), ("t_dfgemm", ctypes.c_double)]
+
+
+class FockTiming(ctypes.Structure):
+    _fields_ = [("t_tempJ", ctypes.c_double), ("t_Jmat", ctypes.c_double), ("t_tempK", ctypes.c_double),
+                ("t_Kmat", ctypes.c_double), ("t_symm", ctypes.c_double), ("t_total", ctypes.c_double)]
+
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "ygpu_device_count": [ip],
+    "ygpu_create": [ctypes.c_int, ctypes.POINTER(vp)],
+    "ygpu_destroy": [vp],
+    "ygpu_set_basis": [vp, ctypes.c_int, ctypes.c_int, ip, ip, ip, dp, dp, dp],
+    "ygpu_set_atoms": [vp, ctypes.c_int, dp, dp],
+    "ygpu_ovlp_hcore": [vp, dp, dp],
+    "ygpu_sym_inv_sqrt": [vp, ctypes.c_int, dp, dp, dp],
+    "ygpu_schwarz": [vp, ctypes.c_int, dp, dp, dp],
+    "ygpu_set_sparsity": [vp, ip, ctypes.c_int, ip, ip, dp, dp, ctypes.c_double],
+    "ygpu_build_df_tensor": [vp, dp, dp, ctypes.POINTER(DfTiming)],
+    "ygpu_build_df_tensor_ex": [vp, dp, dp, dp, ctypes.POINTER(DfTiming)],
+    "ygpu_set_df_tensor": [vp, dp],
+    "ygpu_build_fock": [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.POINTER(FockTiming)],
+    "ygpu_build_density": [vp, ctypes.c_int, dp, dp, dp, dp, dp, ip],
+    "ygpu_dgemm": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int],
+    "ygpu_layout_info": [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long), ip],
+}
+
+_glib = None
+_hlib = None
+
+
+def gpu_lib():
+    global _glib
+    if _glib is None:
+        if not os.path.exists(GPU_LIB):
+            raise RuntimeError("libyatscf_gpu.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(GPU_LIB, mode=ctypes.RTLD_GLOBAL)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.ygpu_last_error.restype = ctypes.c_char_p
+        L.ygpu_last_error.argtypes = []
+        _glib = L
+    return _glib
+
+
+def host_lib():
+    global _hlib
+    if _hlib is None:
+        gpu_lib()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("libtinyscf.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(HOST_LIB)
+        L.tinyscf_alloc.restype = vp
+        L.tinyscf_dptr.restype = dp
+        L.tinyscf_dptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.tinyscf_iptr.restype = ip
+        L.tinyscf_iptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.tinyscf_geti.argtypes = [vp, ctypes.c_char_p]
+        L.tinyscf_geti.restype = ctypes.c_int
+        L.tinyscf_seti.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.tinyscf_seti.restype = None
+        L.tinyscf_getd.argtypes = [vp, ctypes.c_char_p]
+        L.tinyscf_getd.restype = ctypes.c_double
+        L.tinyscf_setd.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.tinyscf_setd.restype = None
+        L.tinyscf_gpu.argtypes = [vp]
+        L.tinyscf_gpu.restype = vp
+        L.TinySCF_try_init.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.TinySCF_try_init.restype = ctypes.c_int
+        for fn in ("TinySCF_init_batch_dgemm_arrays", "TinySCF_compute_Hcore_Ovlp_mat",
+                   "TinySCF_compute_sq_Schwarz_scrvals", "TinySCF_prepare_sparsity", "TinySCF_get_initial_guess",
+                   "TinySCF_build_DF_tensor", "TinySCF_do_SCF", "TinySCF_build_FockMat", "TinySCF_build_DenMat",
+                   "TinySCF_DIIS", "TinySCF_D2Cocc", "TinySCF_free_batch_dgemm_arrays", "free_TinySCF"):
+            getattr(L, fn).argtypes = [vp]
+            getattr(L, fn).restype = None
+        L.quickSort_eigval.argtypes = [dp, ip, ctypes.c_int, ctypes.c_int]
+        L.quickSort_eigval.restype = None
+        _hlib = L
+    return _hlib
+
+
+def last_error():
+    return gpu_lib().ygpu_last_error().decode()
+
+
+def check(status):
+    if status != YGPU_OK:
+        raise GpuError(status, last_error())
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(ip)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    gpu_lib().ygpu_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Engine:
+    """One device engine (the C-ABI handle)."""
+
+    def __init__(self, device=0):
+        self.L = gpu_lib()
+        h = vp()
+        check(self.L.ygpu_create(device, ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ygpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_basis(self, which, b):
+        arrs = [np.ascontiguousarray(b[k], dtype=np.int32) for k in ("am", "nprim", "prim_off")]
+        fl = [np.ascontiguousarray(b[k], dtype=np.float64).reshape(-1) for k in ("xyz", "alpha", "coef")]
+        self._keep["basis%d" % which] = arrs + fl
+        check(self.L.ygpu_set_basis(self.h, which, int(b["nshells"]), _i(arrs[0]), _i(arrs[1]), _i(arrs[2]),
+                                    _d(fl[0]), _d(fl[1]), _d(fl[2])))
+
+    def set_atoms(self, Z, xyz):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        self._keep["atoms"] = (Z, xyz)
+        check(self.L.ygpu_set_atoms(self.h, len(Z), _d(Z), _d(xyz)))
+
+    def set_system(self, system):
+        """Upload an oracle.basis.System (both shell sets and the nuclei)."""
+        self.nbf, self.naux = system.nbf, system.naux
+        self.nshells, self.aux_nshells = system.basis["nshells"], system.auxbasis["nshells"]
+        self.set_basis(0, system.basis)
+        self.set_basis(1, system.auxbasis)
+        self.set_atoms(system.mol["Z"], system.mol["coords"])
+
+    def ovlp_hcore(self):
+        S = np.zeros((self.nbf, self.nbf))
+        H = np.zeros((self.nbf, self.nbf))
+        check(self.L.ygpu_ovlp_hcore(self.h, _d(S), _d(H)))
+        return S, H
+
+    def sym_inv_sqrt(self, A):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        X = np.zeros_like(A)
+        m = ctypes.c_double(0)
+        check(self.L.ygpu_sym_inv_sqrt(self.h, A.shape[0], _d(A), _d(X), ctypes.byref(m)))
+        return X, m.value
+
+    def schwarz(self, quirk_a5=True):
+        sp = np.zeros(self.nshells * self.nshells)
+        bf = np.zeros(self.nbf * self.nbf)
+        df = np.zeros(self.aux_nshells)
+        check(self.L.ygpu_schwarz(self.h, int(quirk_a5), _d(sp), _d(bf), _d(df)))
+        return sp, bf, df
+
+    def set_sparsity(self, mask, lid, rid, sp, df, scrtol2=1e-22):
+        mask = np.ascontiguousarray(mask, dtype=np.int32).reshape(-1)
+        lid = np.ascontiguousarray(lid, dtype=np.int32)
+        rid = np.ascontiguousarray(rid, dtype=np.int32)
+        sp = np.ascontiguousarray(sp, dtype=np.float64)
+        df = np.ascontiguousarray(df, dtype=np.float64)
+        self._keep["sparsity"] = (mask, lid, rid, sp, df)
+        self.nnz = int(np.sum(mask >= 0))
+        check(self.L.ygpu_set_sparsity(self.h, _i(mask), len(lid), _i(lid), _i(rid), _d(sp), _d(df), scrtol2))
+
+    def build_df_tensor(self, want_pqA=False, want_Jpq=False, want_B=False):
+        pqA = np.zeros((self.nnz, self.naux)) if want_pqA else None
+        Jpq = np.zeros((self.naux, self.naux)) if want_Jpq else None
+        B = np.zeros((self.nnz, self.naux)) if want_B else None
+        t = DfTiming()
+        check(self.L.ygpu_build_df_tensor_ex(self.h, _d(pqA), _d(Jpq), _d(B), ctypes.byref(t)))
+        return dict(pqA=pqA, Jpq=Jpq, B=B, timing=t)
+
+    def set_df_tensor(self, B):
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        check(self.L.ygpu_set_df_tensor(self.h, _d(B)))
+
+    def build_fock(self, D, Cocc, H, want_JK=True):
+        n_occ = Cocc.shape[1]
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        Cocc = np.ascontiguousarray(Cocc, dtype=np.float64)
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        F = np.zeros((self.nbf, self.nbf))
+        J = np.zeros((self.nbf, self.nbf)) if want_JK else None
+        K = np.zeros((self.nbf, self.nbf)) if want_JK else None
+        t = FockTiming()
+        check(self.L.ygpu_build_fock(self.h, n_occ, _d(D), _d(Cocc), _d(H), _d(J), _d(K), _d(F), ctypes.byref(t)))
+        return dict(F=F, J=J, K=K, timing=t)
+
+    def build_density(self, Fo, X, n_occ):
+        Fo = np.ascontiguousarray(Fo, dtype=np.float64)
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        n = Fo.shape[0]
+        D = np.zeros((n, n))
+        C = np.zeros((n, n_occ))
+        ev = np.zeros(n)
+        idx = np.zeros(n, dtype=np.int32)
+        check(self.L.ygpu_build_density(self.h, n_occ, _d(Fo), _d(X), _d(D), _d(C), _d(ev), _i(idx)))
+        return D, C, ev, idx
+
+    def dgemm(self, ta, tb, A, B, alpha=1.0, beta=0.0, C=None):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        m = A.shape[1] if ta else A.shape[0]
+        k = A.shape[0] if ta else A.shape[1]
+        n = B.shape[0] if tb else B.shape[1]
+        C = np.zeros((m, n)) if C is None else np.ascontiguousarray(C, dtype=np.float64).copy()
+        check(self.L.ygpu_dgemm(self.h, int(ta), int(tb), m, n, k, alpha, _d(A), A.shape[1], _d(B), B.shape[1],
+                                beta, _d(C), n))
+        return C
+
+    def layout_info(self):
+        a, b, c = ctypes.c_long(0), ctypes.c_long(0), ctypes.c_int(0)
+        check(self.L.ygpu_layout_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+
+class TinySCF:
+    """The TinySCF host engine (libtinyscf.so) with the reference's call sequence."""
+
+    def __init__(self, xyz, niters=50, basis=BASIS, auxbasis=AUXBASIS, verbose=False):
+        self.L = host_lib()
+        self.T = self.L.tinyscf_alloc()
+        os.environ.setdefault("YATSCF_VERBOSE", "1" if verbose else "0")
+        st = self.L.TinySCF_try_init(self.T, basis.encode(), auxbasis.encode(), xyz.encode(), niters)
+        if st != 0:
+            raise RuntimeError("TinySCF init failed (%d): %s" % (st, last_error()))
+        self.L.tinyscf_seti(self.T, b"verbose", int(verbose))
+
+    def step(self, name):
+        getattr(self.L, name)(self.T)
+
+    def setup(self):
+        for fn in ("TinySCF_init_batch_dgemm_arrays", "TinySCF_compute_Hcore_Ovlp_mat",
+                   "TinySCF_compute_sq_Schwarz_scrvals", "TinySCF_prepare_sparsity", "TinySCF_get_initial_guess"):
+            self.step(fn)
+
+    def build_df_tensor(self):
+        self.step("TinySCF_build_DF_tensor")
+
+    def do_scf(self):
+        self.step("TinySCF_do_SCF")
+
+    def close(self):
+        if getattr(self, "T", None):
+            self.L.TinySCF_free_batch_dgemm_arrays(self.T)
+            self.L.free_TinySCF(self.T)
+            self.T = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def d(self, name, shape=None):
+        n = ctypes.c_size_t(0)
+        p = self.L.tinyscf_dptr(self.T, name.encode(), ctypes.byref(n))
+        if not p or n.value == 0:
+            return np.zeros(0)
+        a = np.ctypeslib.as_array(p, shape=(n.value,))
+        return a.reshape(shape) if shape is not None else a
+
+    def i(self, name):
+        n = ctypes.c_size_t(0)
+        p = self.L.tinyscf_iptr(self.T, name.encode(), ctypes.byref(n))
+        if not p or n.value == 0:
+            return np.zeros(0, dtype=np.int32)
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+    def geti(self, name):
+        return self.L.tinyscf_geti(self.T, name.encode())
+
+    def seti(self, name, v):
+        self.L.tinyscf_seti(self.T, name.encode(), int(v))
+
+    def getd(self, name):
+        return self.L.tinyscf_getd(self.T, name.encode())
+
+    def setd(self, name, v):
+        self.L.tinyscf_setd(self.T, name.encode(), float(v))
+
+    def mat(self, name):
+        n = self.geti("nbasfuncs")
+        return self.d(name, (n, n))
