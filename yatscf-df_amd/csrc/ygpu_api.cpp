@@ -14,6 +14,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <vector>
 
 static thread_local std::string g_err;
@@ -94,12 +96,41 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Size the eigensolver workspace with a rocBLAS size query and hand the
+// handle a library-owned buffer: rocBLAS then never allocates or frees device
+// memory itself (its handle destructor has nothing to release).
+int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
+{
+    size_t need = 0;
+    if (rocblas_start_device_memory_size_query(c->rb) != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_start_device_memory_size_query failed");
+        return YGPU_ERR_SOLVER;
+    }
+    (void) rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, nullptr, ld, nullptr, nullptr,
+                            nullptr);
+    if (rocblas_stop_device_memory_size_query(c->rb, &need) != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_stop_device_memory_size_query failed");
+        return YGPU_ERR_SOLVER;
+    }
+    if (need > c->solver_ws_bytes) {
+        YG_HIP(hipStreamSynchronize(c->stream));
+        YG_TRY(c->d_solver_ws.ensure(need));
+        if (rocblas_set_workspace(c->rb, c->d_solver_ws.p, need) != rocblas_status_success) {
+            ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_set_workspace(%zu bytes) failed", need);
+            return YGPU_ERR_SOLVER;
+        }
+        c->solver_ws_bytes = need;
+    }
+    return YGPU_OK;
+}
+
 // In-place A (n x n, leading dimension ld) -> out = A^{-1/2}; A is destroyed.
 int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *dOut, double *min_eig)
 {
     YG_TRY(c->d_eig.ensure((size_t) n * sizeof(double)));
     YG_TRY(c->d_evE.ensure((size_t) n * sizeof(double)));
     YG_TRY(c->d_info.ensure(sizeof(int)));
+    YG_TRY(ensure_syevd_workspace(c, n, ld));
     rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, dA, ld,
                                          c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
     if (st != rocblas_status_success) {
@@ -129,6 +160,24 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
     return YGPU_OK;
 }
 
+// Converts an exception escaping a C-ABI entry point into an error status
+// (nothing may unwind through extern "C" frames into the caller).
+int exception_status()
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        ygpu_set_error(YGPU_ERR_NOMEM, "host allocation failed");
+        return YGPU_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        ygpu_set_error(YGPU_ERR_INVALID, "exception: %s", e.what());
+        return YGPU_ERR_INVALID;
+    } catch (...) {
+        ygpu_set_error(YGPU_ERR_INVALID, "unknown exception");
+        return YGPU_ERR_INVALID;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -136,16 +185,18 @@ extern "C" {
 const char *ygpu_last_error(void) { return g_err.c_str(); }
 
 int ygpu_device_count(int *count)
-{
+try {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess) n = 0;
     if (count) *count = n;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_create(int device, ygpu_ctx_t **out)
-{
+try {
     if (!out) {
         ygpu_set_error(YGPU_ERR_INVALID, "null output pointer");
         return YGPU_ERR_INVALID;
@@ -178,13 +229,24 @@ int ygpu_create(int device, ygpu_ctx_t **out)
     YG_HIP(hipStreamSynchronize(c->stream));
     *out = c;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_destroy(ygpu_ctx_t *c)
-{
+try {
     if (!c) return YGPU_OK;
     (void) hipSetDevice(c->device);
-    (void) hipStreamSynchronize(c->stream);
+    int status = YGPU_OK;
+    hipError_t pend = hipStreamSynchronize(c->stream);
+    if (pend != hipSuccess) {
+        ygpu_set_error(YGPU_ERR_HIP, "ygpu_destroy: pending device error %s", hipGetErrorString(pend));
+        status = YGPU_ERR_HIP;
+    }
+    // the handle first: it only references the library-owned workspace
+    if (c->rb) rocblas_destroy_handle(c->rb);
+    c->rb = nullptr;
+    c->d_solver_ws.release();
     DBuf *bufs[] = {&c->boys, &c->d_charge, &c->d_atom_xyz, &c->d_upmask, &c->d_up_i, &c->d_up_j, &c->d_xptr,
                     &c->d_xup, &c->d_xpartner, &c->d_sp_scr, &c->d_df_scr, &c->d_pqA, &c->d_B, &c->d_Jpq,
                     &c->d_D, &c->d_Cocc, &c->d_H, &c->d_J, &c->d_K, &c->d_F, &c->d_tempJ, &c->d_tempJpart,
@@ -195,16 +257,17 @@ int ygpu_destroy(ygpu_ctx_t *c)
         DBuf *bb[] = {&b->d_am, &b->d_nprim, &b->d_prim_off, &b->d_bf_start, &b->d_xyz, &b->d_alpha, &b->d_coef};
         for (DBuf *x : bb) x->release();
     }
-    if (c->rb) rocblas_destroy_handle(c->rb);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
     if (c->stream) (void) hipStreamDestroy(c->stream);
     delete c;
-    return YGPU_OK;
+    return status;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_set_basis(ygpu_ctx_t *c, int which, int nshells, const int *am, const int *nprim, const int *prim_off,
                    const double *xyz, const double *alpha, const double *coef)
-{
+try {
     YG_TRY(check_ctx(c));
     if ((which != 0 && which != 1) || nshells <= 0 || !am || !nprim || !prim_off || !xyz || !alpha || !coef) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_basis: bad arguments");
@@ -246,10 +309,12 @@ int ygpu_set_basis(ygpu_ctx_t *c, int which, int nshells, const int *am, const i
     if (which == 0) c->nbf = nbf;
     else c->naux = nbf;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_set_atoms(ygpu_ctx_t *c, int natoms, const double *charge, const double *xyz)
-{
+try {
     YG_TRY(check_ctx(c));
     if (natoms <= 0 || !charge || !xyz) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_atoms: bad arguments");
@@ -260,10 +325,12 @@ int ygpu_set_atoms(ygpu_ctx_t *c, int natoms, const double *charge, const double
     YG_TRY(upload(c->d_atom_xyz, xyz, 3 * (size_t) natoms * sizeof(double), c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_ovlp_hcore(ygpu_ctx_t *c, double *S, double *H)
-{
+try {
     YG_TRY(check_ctx(c));
     if (c->bas.nshells == 0 || c->natoms == 0 || !S || !H) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_ovlp_hcore: basis/atoms not set or null output");
@@ -277,10 +344,12 @@ int ygpu_ovlp_hcore(ygpu_ctx_t *c, double *S, double *H)
     YG_HIP(hipMemcpyAsync(H, c->d_mat1.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_sym_inv_sqrt(ygpu_ctx_t *c, int n, const double *A, double *X, double *min_eig)
-{
+try {
     YG_TRY(check_ctx(c));
     if (n <= 0 || !A || !X) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_sym_inv_sqrt: bad arguments");
@@ -295,10 +364,12 @@ int ygpu_sym_inv_sqrt(ygpu_ctx_t *c, int n, const double *A, double *X, double *
     YG_HIP(hipMemcpyAsync(X, c->d_mat2.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_schwarz(ygpu_ctx_t *c, int quirk_a5, double *sp_scrval, double *bf_pair_scrval, double *df_sp_scrval)
-{
+try {
     YG_TRY(check_ctx(c));
     if (c->bas.nshells == 0 || c->aux.nshells == 0) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_schwarz: basis sets not set");
@@ -325,11 +396,13 @@ int ygpu_schwarz(ygpu_ctx_t *c, int quirk_a5, double *sp_scrval, double *bf_pair
                 sp_scrval[(size_t) M * ns + N] = m;
             }
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_set_sparsity(ygpu_ctx_t *c, const int *mask, int num_uniq_sp, const int *lid, const int *rid,
                       const double *sp_scrval, const double *df_sp_scrval, double scrtol2)
-{
+try {
     YG_TRY(check_ctx(c));
     const int nbf = c->nbf, ns = c->bas.nshells;
     if (!mask || num_uniq_sp < 0 || (num_uniq_sp > 0 && (!lid || !rid)) || !sp_scrval || !df_sp_scrval || nbf == 0 ||
@@ -397,6 +470,8 @@ int ygpu_set_sparsity(ygpu_ctx_t *c, const int *mask, int num_uniq_sp, const int
     c->have_sparsity = 1;
     c->have_df = 0;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 static int unpack_to_reference(ygpu_ctx *c, const double *dsrc, double *host_out)
@@ -417,7 +492,7 @@ static int unpack_to_reference(ygpu_ctx *c, const double *dsrc, double *host_out
 }
 
 int ygpu_build_df_tensor_ex(ygpu_ctx_t *c, double *pqA_out, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
-{
+try {
     YG_TRY(check_ctx(c));
     if (!c->have_sparsity) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_df_tensor: call ygpu_set_sparsity first");
@@ -459,15 +534,19 @@ int ygpu_build_df_tensor_ex(ygpu_ctx_t *c, double *pqA_out, double *Jpq_out, dou
     if (df_out) YG_TRY(unpack_to_reference(c, c->d_B.as<double>(), df_out));
     c->have_df = 1;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_build_df_tensor(ygpu_ctx_t *c, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
-{
+try {
     return ygpu_build_df_tensor_ex(c, nullptr, Jpq_out, df_out, t);
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_set_df_tensor(ygpu_ctx_t *c, const double *df)
-{
+try {
     YG_TRY(check_ctx(c));
     if (!c->have_sparsity || !df) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_df_tensor: call ygpu_set_sparsity first / null tensor");
@@ -487,11 +566,13 @@ int ygpu_set_df_tensor(ygpu_ctx_t *c, const double *df)
     YG_HIP(hipStreamSynchronize(c->stream));
     c->have_df = 1;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Cocc, const double *H, double *J,
                     double *K, double *F, ygpu_fock_timing_t *t)
-{
+try {
     YG_TRY(check_ctx(c));
     if (!c->have_df || n_occ <= 0 || !D || !Cocc || !H || !F) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built or bad arguments");
@@ -538,11 +619,13 @@ int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Coc
         t->t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
     }
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double *X, double *D, double *Cocc,
                        double *eigval, int *ev_idx)
-{
+try {
     YG_TRY(check_ctx(c));
     const int nbf = c->nbf;
     if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || !Fo || !X || !D || !Cocc || !eigval) {
@@ -560,6 +643,7 @@ int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double 
     YG_TRY(c->d_info.ensure(sizeof(int)));
     YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
     YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
+    YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
     rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf, c->d_mat0.as<double>(),
                                          nbf, c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
     if (st != rocblas_status_success) {
@@ -585,11 +669,13 @@ int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double 
     if (ev_idx)
         for (int i = 0; i < nbf; ++i) ev_idx[i] = i;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_dgemm(ygpu_ctx_t *c, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
                const double *B, int ldb, double beta, double *C, int ldc)
-{
+try {
     YG_TRY(check_ctx(c));
     if (m < 0 || n < 0 || k < 0 || !A || !B || !C) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_dgemm: bad arguments");
@@ -609,15 +695,19 @@ int ygpu_dgemm(ygpu_ctx_t *c, int ta, int tb, int m, int n, int k, double alpha,
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 int ygpu_layout_info(ygpu_ctx_t *c, long *n_up, long *nnz, int *naux_pad)
-{
+try {
     YG_TRY(check_ctx(c));
     if (n_up) *n_up = c->n_up;
     if (nnz) *nnz = c->nnz;
     if (naux_pad) *naux_pad = c->naux_pad;
     return YGPU_OK;
+} catch (...) {
+    return exception_status();
 }
 
 }  // extern "C"

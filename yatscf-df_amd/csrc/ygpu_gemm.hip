@@ -139,7 +139,8 @@ __device__ __forceinline__ void tile_mma(const Operand &A, const Operand &B, int
     }
 }
 
-// D[i][j] layout of v_mfma_f64_16x16x4f64: lane l, element v -> row 4*(l/16)+v, col l%16
+// D[i][j] layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, element v ->
+// row l/16 + 4*v, col l%16.  Operands: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k.
 template <class F>
 __device__ __forceinline__ void tile_epilogue(int m0, int n0, const v4d (&acc)[2][2], F &&store)
 {
@@ -151,7 +152,7 @@ __device__ __forceinline__ void tile_epilogue(int m0, int n0, const v4d (&acc)[2
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + v;
+                const int m = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
                 const int n = n0 + wn + j * 16 + (lane & 15);
                 store(m, n, acc[i][j][v]);
             }

@@ -85,6 +85,8 @@ struct ygpu_ctx {
     DBuf d_D, d_Cocc, d_H, d_J, d_K, d_F, d_tempJ, d_tempJpart, d_X, d_Kpart;
     DBuf d_mat0, d_mat1, d_mat2, d_eig, d_info, d_evE;
     DBuf d_gA, d_gB, d_gC;
+    DBuf d_solver_ws;             // rocBLAS/rocSOLVER workspace owned by the library
+    size_t solver_ws_bytes = 0;
 };
 
 // --------------------------------------------------------------------------

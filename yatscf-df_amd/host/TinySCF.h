@@ -109,6 +109,9 @@ struct TinySCF_struct
     double t_df[4];         /* 3c, 2c, inv-sqrt, DF GEMM (s) */
     double t_fock[6];       /* aux J, J, aux K, K, symm, total (s) of the last Fock build */
     double t_density, t_diis;
+#define YS_MAX_HIST 512
+    int    n_hist;          /* per-iteration history of the phase timers (s) */
+    double t_hist[YS_MAX_HIST][6];
 };
 
 typedef struct TinySCF_struct* TinySCF_t;

@@ -24,6 +24,10 @@ void TinySCF_build_FockMat(TinySCF_t T)
     T->t_fock[3] = t.t_Kmat;
     T->t_fock[4] = t.t_symm;
     T->t_fock[5] = t.t_total;
+    if (T->n_hist < YS_MAX_HIST) {
+        for (int k = 0; k < 6; k++) T->t_hist[T->n_hist][k] = T->t_fock[k];
+        T->n_hist++;
+    }
     if (T->verbose) {
         printf("* Build Fock matrix     : %.3lf (s)\n", get_wtime_sec() - st);
         printf("|---- aux. J tensor     |---- %.3lf (s)\n", t.t_tempJ);

@@ -3,10 +3,49 @@
  * TinySCF host engine through ctypes without mirroring the struct layout.
  * Not part of the reference API.
  */
+#include <setjmp.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include "DIIS.h"
 #include "TinySCF.h"
+#include "build_DF_tensor.h"
+#include "build_Fock.h"
+#include "build_density.h"
+#include "utils.h"
+
+/* Runs one TinySCF API function by name; a failing device call returns its
+ * status here (message from tinyscf_error) instead of aborting the caller. */
+int tinyscf_call(TinySCF_t T, const char *name)
+{
+    static const struct { const char *n; void (*f)(TinySCF_t); } fns[] = {
+        {"TinySCF_init_batch_dgemm_arrays", TinySCF_init_batch_dgemm_arrays},
+        {"TinySCF_compute_Hcore_Ovlp_mat", TinySCF_compute_Hcore_Ovlp_mat},
+        {"TinySCF_compute_sq_Schwarz_scrvals", TinySCF_compute_sq_Schwarz_scrvals},
+        {"TinySCF_prepare_sparsity", TinySCF_prepare_sparsity},
+        {"TinySCF_get_initial_guess", TinySCF_get_initial_guess},
+        {"TinySCF_build_DF_tensor", TinySCF_build_DF_tensor},
+        {"TinySCF_do_SCF", TinySCF_do_SCF},
+        {"TinySCF_build_FockMat", TinySCF_build_FockMat},
+        {"TinySCF_build_DenMat", TinySCF_build_DenMat},
+        {"TinySCF_DIIS", TinySCF_DIIS},
+        {"TinySCF_D2Cocc", TinySCF_D2Cocc},
+    };
+    for (size_t k = 0; k < sizeof(fns) / sizeof(fns[0]); k++) {
+        if (strcmp(name, fns[k].n) != 0) continue;
+        jmp_buf jb;
+        int st = setjmp(jb);
+        if (st == 0) {
+            ys_set_error_jump(&jb);
+            fns[k].f(T);
+        }
+        ys_set_error_jump(NULL);
+        return st;
+    }
+    return -1;
+}
+
+const char *tinyscf_error(void) { return ys_error_message(); }
 
 TinySCF_t tinyscf_alloc(void) { return (TinySCF_t) calloc(1, sizeof(struct TinySCF_struct)); }
 
@@ -25,6 +64,7 @@ double *tinyscf_dptr(TinySCF_t T, const char *n, size_t *count)
     RET("df_tensor", T->df_tensor, (size_t) T->num_bf_pair_scr * T->df_nbf)
     RET("B_mat", T->B_mat, (size_t) (MAX_DIIS + 1) * (MAX_DIIS + 1))
     RET("t_df", T->t_df, 4) RET("t_fock", T->t_fock, 6)
+    RET("t_hist", &T->t_hist[0][0], (size_t) 6 * T->n_hist)
 #undef RET
     *count = 0;
     return NULL;

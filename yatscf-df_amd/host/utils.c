@@ -1,3 +1,4 @@
+#include <setjmp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -37,10 +38,22 @@ void print_mat(double *mat, const int ldm, const int nrows, const int ncols, con
     }
 }
 
+static jmp_buf *ys_jmp = NULL;
+static char ys_msg[1536];
+
+void ys_set_error_jump(void *jb) { ys_jmp = (jmp_buf *) jb; }
+const char *ys_error_message(void) { return ys_msg; }
+
+/* A failed device call ends the current TinySCF function: with a jump target
+ * installed (callers that handle errors, such as the Python bindings) control
+ * returns there with the status; otherwise the process aborts with the
+ * message, keeping the reference's void-function contract. */
 void ys_check_gpu(int status, const char *what)
 {
     if (status == YGPU_OK) return;
-    fprintf(stderr, "TinySCF: %s failed (status %d): %s\n", what, status, ygpu_last_error());
+    snprintf(ys_msg, sizeof(ys_msg), "%s failed (status %d): %s", what, status, ygpu_last_error());
+    if (ys_jmp) longjmp(*ys_jmp, status);
+    fprintf(stderr, "TinySCF: %s\n", ys_msg);
     fflush(stderr);
     abort();
 }

@@ -17,6 +17,8 @@ void print_mat(double *mat, const int ldm, const int nrows, const int ncols, con
 
 /* abort with the device library's message when a C-ABI call fails */
 void ys_check_gpu(int status, const char *what);
+void ys_set_error_jump(void *jmp_buf_ptr);
+const char *ys_error_message(void);
 #define YS_GPU(call) ys_check_gpu((call), #call)
 
 #endif

@@ -105,6 +105,10 @@ def host_lib():
         L.tinyscf_getd.restype = ctypes.c_double
         L.tinyscf_setd.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
         L.tinyscf_setd.restype = None
+        L.tinyscf_call.argtypes = [vp, ctypes.c_char_p]
+        L.tinyscf_call.restype = ctypes.c_int
+        L.tinyscf_error.argtypes = []
+        L.tinyscf_error.restype = ctypes.c_char_p
         L.tinyscf_gpu.argtypes = [vp]
         L.tinyscf_gpu.restype = vp
         L.TinySCF_try_init.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
@@ -156,8 +160,9 @@ class Engine:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.ygpu_destroy(self.h)
+            st = self.L.ygpu_destroy(self.h)
             self.h = None
+            check(st)
 
     def __del__(self):
         try:
@@ -281,7 +286,9 @@ class TinySCF:
         self.L.tinyscf_seti(self.T, b"verbose", int(verbose))
 
     def step(self, name):
-        getattr(self.L, name)(self.T)
+        st = self.L.tinyscf_call(self.T, name.encode())
+        if st != 0:
+            raise GpuError(st, "%s: %s" % (name, self.L.tinyscf_error().decode()))
 
     def setup(self):
         for fn in ("TinySCF_init_batch_dgemm_arrays", "TinySCF_compute_Hcore_Ovlp_mat",
