@@ -110,9 +110,9 @@ int ygpu_build_df_tensor_ex(ygpu_ctx_t *ctx, double *pqA, double *Jpq_invsqrt, d
 int ygpu_set_df_tensor(ygpu_ctx_t *ctx, const double *df_tensor);
 
 /* J, K and F = Hcore + 2J - K from D and C_occ (TinySCF_build_FockMat,
- * build_Fock.c:349-396).  D, Hcore: nbf x nbf; Cocc: nbf x n_occ.  J, K
- * (nullable) and F receive full symmetric matrices; J entries of screened
- * pairs are zero. */
+ * build_Fock.c:349-396).  D, Hcore: nbf x nbf (Hcore NULL: the copy cached by
+ * ygpu_ovlp_hcore); Cocc: nbf x n_occ.  J, K (nullable) and F receive full
+ * symmetric matrices; J entries of screened pairs are zero. */
 int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *Cocc,
                     const double *Hcore, double *J, double *K, double *F,
                     ygpu_fock_timing_t *timing);
@@ -120,10 +120,31 @@ int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *C
 /* Density from the orthogonal-basis Fock matrix F_orth = X^T F X
  * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
  * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.
- * ev_idx (nullable) receives the order used (identity for the ascending
- * eigenvalues of the eigensolver). */
+ * X NULL: the copy cached by ygpu_set_overlap.  ev_idx (nullable) receives
+ * the order used (identity for the ascending eigenvalues of the eigensolver). */
 int ygpu_build_density(ygpu_ctx_t *ctx, int n_occ, const double *F_orth, const double *X,
                        double *D, double *Cocc, double *eigval, int *ev_idx);
+
+/* Cache the overlap S and X = S^{-1/2} (nbf x nbf) on the device for the
+ * density build and DIIS.  ygpu_ovlp_hcore also caches Hcore; passing NULL
+ * for Hcore (ygpu_build_fock) or X (ygpu_build_density) uses the cached
+ * device copies. */
+int ygpu_set_overlap(ygpu_ctx_t *ctx, const double *S, const double *X);
+
+/* Commutator DIIS with the history on the device (DIIS.c:12-123); the caller
+ * keeps the reference's bookkeeping (DIIS_len, the replaced slot, B_mat,
+ * DIIS_bmax) and solves the small system.
+ *   ygpu_diis_transform: F_orth = X^T F X (iterations 0 and 1, DIIS.c:30-40).
+ *   ygpu_diis_update: residual R = X^T (F D S - S D F) X stored in history
+ *     slot `slot`, dots[j] = <R, R_j> for j < len, F_orth = X^T F X stored as
+ *     the slot's Fock matrix and returned (DIIS.c:51-110).
+ *   ygpu_diis_extrapolate: F_out = sum_{i<len} coef[i] F_orth_i (DIIS.c:119-122).
+ * len <= YGPU_MAX_DIIS. */
+#define YGPU_MAX_DIIS 10
+int ygpu_diis_transform(ygpu_ctx_t *ctx, const double *F, double *F_orth);
+int ygpu_diis_update(ygpu_ctx_t *ctx, int slot, int len, const double *F, const double *D, double *dots,
+                     double *F_orth);
+int ygpu_diis_extrapolate(ygpu_ctx_t *ctx, int len, const double *coef, double *F_out);
 
 /* Row-major FP64 GEMM on host operands, C = alpha op(A) op(B) + beta C
  * (the cblas_dgemm calls of DIIS.c:34-107). */

@@ -254,3 +254,35 @@ def test_full_scf_energy(ygpu, name):
         assert abs(e_gpu - e_ref) < 1e-8
     finally:
         T.close()
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_device_diis(ygpu, name):
+    """Device DIIS history (DIIS.c:42-122) against the same algebra in NumPy."""
+    log("case: device DIIS", name)
+    st = oracle_state(name)
+    o = st.o
+    S, X = o.mat("S_mat").copy(), o.mat("X_mat").copy()
+    rng = np.random.RandomState(11)
+    Fs = []
+    for k in range(3):
+        P = rng.randn(*st.F0.shape) * 1e-3
+        Fs.append(st.F0 + P + P.T)
+    D = st.D0
+    with make_engine(ygpu, st) as e:
+        e.set_overlap(S, X)
+        Fo0 = e.diis_transform(Fs[0])
+        assert np.max(np.abs(Fo0 - X.T @ Fs[0] @ X)) < 1e-11
+        Rs = []
+        for k, F in enumerate(Fs):
+            dots, Fo = e.diis_update(k, k + 1, F, D)
+            FDS = F @ D @ S
+            R = X.T @ (FDS - FDS.T) @ X
+            Rs.append(R)
+            ref = np.array([np.sum(R * Rj) for Rj in Rs])
+            assert np.allclose(dots, ref, rtol=1e-11, atol=1e-18), (k, dots, ref)
+            assert np.max(np.abs(Fo - X.T @ F @ X)) < 1e-11
+        coef = np.array([0.2, -0.5, 1.3])
+        Fx = e.diis_extrapolate(coef)
+        ref = sum(c * (X.T @ F @ X) for c, F in zip(coef, Fs))
+        assert np.max(np.abs(Fx - ref)) < 1e-11

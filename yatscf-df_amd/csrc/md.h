@@ -271,3 +271,106 @@ __device__ __forceinline__ void herm_R_rt(int L, double alpha, double X, double 
         for (int u = 0; u <= L - t; ++u)
             for (int v = 0; v <= L - t - u; ++v) R[(t * D + u) * D + v] = Rn[pidx(t, u, v)];
 }
+
+// ---------------------------------------------------------------------------
+// 1D Hermite coefficients E^{ij}_t for one run-time pair (i, j) <= (LA, LB),
+// built along the path (0,0) -> (i,0) -> (i,j).  Output E[t], t <= LA+LB (zero
+// beyond i+j).  All indexing is compile-time (no dynamically indexed arrays).
+template <int LA, int LB>
+__device__ __forceinline__ void herm_E1d(double a, double b, double XAB, int i, int j, double *E)
+{
+    constexpr int NT = LA + LB + 1;
+    const double p = a + b, oo2p = 0.5 / p;
+    const double XPA = -b / p * XAB, XPB = a / p * XAB;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) E[t] = 0.0;
+    E[0] = exp(-a * b / p * XAB * XAB);
+#pragma unroll
+    for (int s = 1; s <= LA + LB; ++s) {
+        const bool onA = s <= LA;                 // first LA steps raise i, then LB steps raise j
+        const bool active = onA ? (s <= i) : (s - LA <= j);
+        if (active) {
+            const double X = onA ? XPA : XPB;
+            double nx[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                double v = X * E[t];
+                if (t > 0) v += oo2p * E[t - 1];
+                if (t + 1 < NT) v += (double)(t + 1) * E[t + 1];
+                nx[t] = v;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) E[t] = nx[t];
+        }
+    }
+}
+
+// E^{i,j}_0 for j = 0..LB along (0,0) -> (i,0) -> (i,LB) (overlap factors)
+template <int LA, int LB>
+__device__ __forceinline__ void herm_E0_row(double a, double b, double XAB, int i, double *E0)
+{
+    constexpr int NT = LA + LB + 1;
+    const double p = a + b, oo2p = 0.5 / p;
+    const double XPA = -b / p * XAB, XPB = a / p * XAB;
+    double E[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) E[t] = 0.0;
+    E[0] = exp(-a * b / p * XAB * XAB);
+#pragma unroll
+    for (int s = 1; s <= LA; ++s) {
+        if (s <= i) {
+            double nx[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                double v = XPA * E[t];
+                if (t > 0) v += oo2p * E[t - 1];
+                if (t + 1 < NT) v += (double)(t + 1) * E[t + 1];
+                nx[t] = v;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) E[t] = nx[t];
+        }
+    }
+    E0[0] = E[0];
+#pragma unroll
+    for (int j = 1; j <= LB; ++j) {
+        double nx[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            double v = XPB * E[t];
+            if (t > 0) v += oo2p * E[t - 1];
+            if (t + 1 < NT) v += (double)(t + 1) * E[t + 1];
+            nx[t] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) E[t] = nx[t];
+        E0[j] = E[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// shell access helpers
+struct ShellRef {
+    int am, nprim, off, bf;
+    double x, y, z;
+};
+
+__device__ __forceinline__ ShellRef shell_ref(const DevShells &s, int i)
+{
+    ShellRef r;
+    r.am = s.am[i]; r.nprim = s.nprim[i]; r.off = s.prim_off[i]; r.bf = s.bf_start[i];
+    r.x = s.xyz[3 * i]; r.y = s.xyz[3 * i + 1]; r.z = s.xyz[3 * i + 2];
+    return r;
+}
+
+__device__ __forceinline__ void cart_pow_rt(int l, int idx, int &lx, int &ly, int &lz)
+{
+    int c = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (c == idx) { lx = i; ly = j; lz = l - i - j; return; }
+            ++c;
+        }
+    lx = ly = lz = 0;
+}
+

@@ -251,7 +251,8 @@ try {
                     &c->d_xup, &c->d_xpartner, &c->d_sp_scr, &c->d_df_scr, &c->d_pqA, &c->d_B, &c->d_Jpq,
                     &c->d_D, &c->d_Cocc, &c->d_H, &c->d_J, &c->d_K, &c->d_F, &c->d_tempJ, &c->d_tempJpart,
                     &c->d_X, &c->d_Kpart, &c->d_mat0, &c->d_mat1, &c->d_mat2, &c->d_eig, &c->d_info, &c->d_evE,
-                    &c->d_gA, &c->d_gB, &c->d_gC};
+                    &c->d_gA, &c->d_gB, &c->d_gC, &c->d_S, &c->d_Xo, &c->d_Fin, &c->d_Din, &c->d_T1,
+                    &c->d_T2, &c->d_diisR, &c->d_diisF, &c->d_dots};
     for (DBuf *b : bufs) b->release();
     for (BasisDev *b : {&c->bas, &c->aux}) {
         DBuf *bb[] = {&b->d_am, &b->d_nprim, &b->d_prim_off, &b->d_bf_start, &b->d_xyz, &b->d_alpha, &b->d_coef};
@@ -339,10 +340,12 @@ try {
     const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
     YG_TRY(c->d_mat0.ensure(ms));
     YG_TRY(c->d_mat1.ensure(ms));
-    YG_TRY(yg_launch_ovlp_hcore(c, c->d_mat0.as<double>(), c->d_mat1.as<double>()));
+    YG_TRY(c->d_H.ensure(ms));
+    YG_TRY(yg_launch_ovlp_hcore(c, c->d_mat0.as<double>(), c->d_H.as<double>()));
     YG_HIP(hipMemcpyAsync(S, c->d_mat0.p, ms, hipMemcpyDeviceToHost, c->stream));
-    YG_HIP(hipMemcpyAsync(H, c->d_mat1.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(H, c->d_H.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    c->have_H = 1;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -574,7 +577,7 @@ int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Coc
                     double *K, double *F, ygpu_fock_timing_t *t)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_df || n_occ <= 0 || !D || !Cocc || !H || !F) {
+    if (!c->have_df || n_occ <= 0 || !D || !Cocc || !F || (!H && !c->have_H)) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built or bad arguments");
         return YGPU_ERR_INVALID;
     }
@@ -588,7 +591,10 @@ try {
     YG_TRY(c->d_Cocc.ensure((size_t) nbf * n_occ * sizeof(double)));
     YG_TRY(c->d_X.ensure((size_t) nbf * n_occ * c->naux_pad * sizeof(double)));
     YG_HIP(hipMemcpyAsync(c->d_D.p, D, ms, hipMemcpyHostToDevice, c->stream));
-    YG_HIP(hipMemcpyAsync(c->d_H.p, H, ms, hipMemcpyHostToDevice, c->stream));
+    if (H) {
+        YG_HIP(hipMemcpyAsync(c->d_H.p, H, ms, hipMemcpyHostToDevice, c->stream));
+        c->have_H = 1;
+    }
     YG_HIP(hipMemcpyAsync(c->d_Cocc.p, Cocc, (size_t) nbf * n_occ * sizeof(double), hipMemcpyHostToDevice, c->stream));
     float tj1 = 0.f, tj2 = 0.f;
     YG_HIP(hipEventRecord(c->ev[5], c->stream));
@@ -628,7 +634,7 @@ int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double 
 try {
     YG_TRY(check_ctx(c));
     const int nbf = c->nbf;
-    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || !Fo || !X || !D || !Cocc || !eigval) {
+    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || !Fo || (!X && !c->have_S) || !D || !Cocc || !eigval) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_density: bad arguments");
         return YGPU_ERR_INVALID;
     }
@@ -642,7 +648,11 @@ try {
     YG_TRY(c->d_evE.ensure(nbf * sizeof(double)));
     YG_TRY(c->d_info.ensure(sizeof(int)));
     YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
-    YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
+    const double *dX = c->d_Xo.as<double>();
+    if (X) {
+        YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
+        dX = c->d_mat1.as<double>();
+    }
     YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
     rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf, c->d_mat0.as<double>(),
                                          nbf, c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
@@ -651,7 +661,7 @@ try {
         return YGPU_ERR_SOLVER;
     }
     // C = X V: with V column-major in d_mat0, V(k,n) = d_mat0[n*nbf+k] = op_T access
-    YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, c->d_mat1.as<double>(), nbf, c->d_mat0.as<double>(), nbf, 0.0,
+    YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, dX, nbf, c->d_mat0.as<double>(), nbf, 0.0,
                     c->d_mat2.as<double>(), nbf));
     YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
     YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, c->d_Cocc.as<double>(), n_occ,
@@ -668,6 +678,120 @@ try {
     }
     if (ev_idx)
         for (int i = 0; i < nbf; ++i) ev_idx[i] = i;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_overlap(ygpu_ctx_t *c, const double *S, const double *X)
+try {
+    YG_TRY(check_ctx(c));
+    if (!S || !X || c->nbf <= 0) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_overlap: bad arguments or basis not set");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
+    YG_TRY(upload(c->d_S, S, ms, c->stream));
+    YG_TRY(upload(c->d_Xo, X, ms, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    c->have_S = 1;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+// F_orth = X^T F X into dOut (device), F already on the device in dF
+static int orth_transform(ygpu_ctx *c, const double *dF, double *dOut)
+{
+    const int n = c->nbf;
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, dF, n, 0.0, c->d_T1.as<double>(), n));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_T1.as<double>(), n, c->d_Xo.as<double>(), n, 0.0, dOut, n));
+    return YGPU_OK;
+}
+
+static int diis_buffers(ygpu_ctx *c)
+{
+    const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
+    YG_TRY(c->d_Fin.ensure(ms));
+    YG_TRY(c->d_Din.ensure(ms));
+    YG_TRY(c->d_T1.ensure(ms));
+    YG_TRY(c->d_T2.ensure(ms));
+    YG_TRY(c->d_diisR.ensure(ms * YGPU_MAX_DIIS));
+    YG_TRY(c->d_diisF.ensure(ms * YGPU_MAX_DIIS));
+    return YGPU_OK;
+}
+
+int ygpu_diis_transform(ygpu_ctx_t *c, const double *F, double *F_orth)
+try {
+    YG_TRY(check_ctx(c));
+    if (!c->have_S || !F || !F_orth) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_transform: call ygpu_set_overlap first / null argument");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
+    YG_TRY(diis_buffers(c));
+    YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
+    YG_TRY(orth_transform(c, c->d_Fin.as<double>(), c->d_T2.as<double>()));
+    YG_HIP(hipMemcpyAsync(F_orth, c->d_T2.p, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_diis_update(ygpu_ctx_t *c, int slot, int len, const double *F, const double *D, double *dots,
+                     double *F_orth)
+try {
+    YG_TRY(check_ctx(c));
+    if (!c->have_S || !F || !D || !dots || !F_orth || slot < 0 || slot >= YGPU_MAX_DIIS || len <= slot ||
+        len > YGPU_MAX_DIIS) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_update: bad arguments (slot %d, len %d)", slot, len);
+        return YGPU_ERR_INVALID;
+    }
+    const int n = c->nbf;
+    const long nn = (long) n * n;
+    const size_t ms = (size_t) nn * sizeof(double);
+    YG_TRY(diis_buffers(c));
+    const int nchunk = 64;
+    YG_TRY(c->d_dots.ensure((size_t) YGPU_MAX_DIIS * nchunk * sizeof(double)));
+    YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_Din.p, D, ms, hipMemcpyHostToDevice, c->stream));
+    double *T1 = c->d_T1.as<double>(), *T2 = c->d_T2.as<double>();
+    double *R = c->d_diisR.as<double>() + (size_t) slot * nn;
+    double *F0 = c->d_diisF.as<double>() + (size_t) slot * nn;
+    // FDS = F D S (DIIS.c:51-54); R = X^T (FDS - FDS^T) X (DIIS.c:57-65)
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_Fin.as<double>(), n, c->d_Din.as<double>(), n, 0.0, T1, n));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T1, n, c->d_S.as<double>(), n, 0.0, T2, n));
+    YG_TRY(yg_antisym(c->stream, n, T2, T1));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, T1, n, 0.0, T2, n));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T2, n, c->d_Xo.as<double>(), n, 0.0, R, n));
+    YG_TRY(yg_dots(c->stream, nn, R, c->d_diisR.as<double>(), nn, len, c->d_dots.as<double>(), nchunk));
+    YG_TRY(orth_transform(c, c->d_Fin.as<double>(), F0));
+    std::vector<double> part((size_t) len * nchunk);
+    YG_HIP(hipMemcpyAsync(part.data(), c->d_dots.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipMemcpyAsync(F_orth, F0, ms, hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < len; ++j) {
+        double sacc = 0.0;
+        for (int k = 0; k < nchunk; ++k) sacc += part[(size_t) j * nchunk + k];
+        dots[j] = sacc;
+    }
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_diis_extrapolate(ygpu_ctx_t *c, int len, const double *coef, double *F_out)
+try {
+    YG_TRY(check_ctx(c));
+    if (len <= 0 || len > YGPU_MAX_DIIS || !coef || !F_out || !c->d_diisF.p) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_extrapolate: bad arguments (len %d)", len);
+        return YGPU_ERR_INVALID;
+    }
+    const long nn = (long) c->nbf * c->nbf;
+    YG_TRY(yg_lincomb(c->stream, nn, c->d_diisF.as<double>(), nn, len, coef, c->d_T1.as<double>()));
+    YG_HIP(hipMemcpyAsync(F_out, c->d_T1.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
 } catch (...) {
     return exception_status();

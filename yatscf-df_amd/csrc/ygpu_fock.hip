@@ -2,10 +2,11 @@
 //
 // J (build_Fock.c:37-130):
 //   pass 1 "aux. J tensor": temp_J[p] = sum_r w_r B[r][p] over the packed rows
-//     r = (i <= j), w_r = D_ii on the diagonal and 2 D_ij off it; each
-//     workgroup streams a contiguous band of rows once (coalesced row reads,
-//     one column per lane) into per-band partial sums, reduced in a fixed order
-//     (deterministic; replaces the per-thread buffers + tree of :15-33, :61-98);
+//     r = (i <= j), w_r = D_ii on the diagonal and 2 D_ij off it; a grid of
+//     (64-column tile) x (row group) workgroups streams each row segment once,
+//     reduces its four wavefronts in LDS and leaves one partial per row group,
+//     summed in a fixed order (deterministic; replaces the per-thread buffers
+//     and tree reduction of :15-33, :61-98);
 //   pass 2 "J matrix": J_ij = B[r] . temp_J with temp_J staged in LDS, one
 //     wavefront per row, cross-lane reduction with shuffles.
 // Screened pairs keep J = 0 (quirk A7 of the reference made explicit).
@@ -15,41 +16,67 @@
 
 namespace {
 
-constexpr int JCOLS = 256;     // columns per pass-1 workgroup (one per lane)
+constexpr int JTILE = 64;      // columns per pass-1 workgroup (one per lane)
+constexpr int JWAVES = 4;      // wavefronts per pass-1 workgroup, each on its own rows
 
-__global__ __launch_bounds__(JCOLS) void k_tempj_part(const double *__restrict__ B, long ld, long n_up,
-                                                      const int *__restrict__ up_i, const int *__restrict__ up_j,
-                                                      const double *__restrict__ D, int nbf, long rows_per_band,
-                                                      double *__restrict__ part)
+// pass 1: partial[g][c] = sum over the rows of row group g of w_r B[r][c]; grid
+// (column tiles, row groups); rows are interleaved over (group, wavefront) so
+// every wavefront streams whole 512-byte row segments, and the row index is
+// wave-uniform (scalar loads of the pair indices and of D).
+__global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const double *__restrict__ B, long ld, long n_up,
+                                                               const int *__restrict__ up_i, const int *__restrict__ up_j,
+                                                               const double *__restrict__ D, int nbf,
+                                                               double *__restrict__ part)
 {
-    const long col = (long) blockIdx.x * JCOLS + threadIdx.x;
-    const long r0 = (long) blockIdx.y * rows_per_band;
-    const long r1 = r0 + rows_per_band < n_up ? r0 + rows_per_band : n_up;
-    double acc0 = 0.0, acc1 = 0.0;
-    if (col < ld) {
-        long r = r0;
-        for (; r + 1 < r1; r += 2) {
-            const int i0 = up_i[r], j0 = up_j[r], i1 = up_i[r + 1], j1 = up_j[r + 1];
-            const double w0 = (i0 == j0 ? 1.0 : 2.0) * D[(size_t) i0 * nbf + j0];
-            const double w1 = (i1 == j1 ? 1.0 : 2.0) * D[(size_t) i1 * nbf + j1];
-            acc0 += w0 * B[(size_t) r * ld + col];
-            acc1 += w1 * B[(size_t)(r + 1) * ld + col];
+    __shared__ double red[JWAVES][JTILE];
+    const int lane = threadIdx.x & (JTILE - 1), w = threadIdx.x / JTILE;
+    const long col = (long) blockIdx.x * JTILE + lane;
+    const long stride = (long) gridDim.y * JWAVES;
+    const long r0 = (long) blockIdx.y * JWAVES + w;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const bool cin = col < ld;
+    long r = r0;
+    for (; r + 3 * stride < n_up; r += 4 * stride) {
+        double wt[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long rq = __builtin_amdgcn_readfirstlane((int) (r + q * stride));
+            const int i = up_i[rq], j = up_j[rq];
+            wt[q] = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
+            bv[q] = cin ? B[(size_t) rq * ld + col] : 0.0;
         }
-        if (r < r1) {
-            const int i0 = up_i[r], j0 = up_j[r];
-            acc0 += (i0 == j0 ? 1.0 : 2.0) * D[(size_t) i0 * nbf + j0] * B[(size_t) r * ld + col];
-        }
-        part[(size_t) blockIdx.y * ld + col] = acc0 + acc1;
+        a0 += wt[0] * bv[0];
+        a1 += wt[1] * bv[1];
+        a2 += wt[2] * bv[2];
+        a3 += wt[3] * bv[3];
+    }
+    for (; r < n_up; r += stride) {
+        const long rq = __builtin_amdgcn_readfirstlane((int) r);
+        const int i = up_i[rq], j = up_j[rq];
+        a0 += (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j] * (cin ? B[(size_t) rq * ld + col] : 0.0);
+    }
+    red[w][lane] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (w == 0 && cin) {
+        double s = red[0][lane];
+#pragma unroll
+        for (int q = 1; q < JWAVES; ++q) s += red[q][lane];
+        part[(size_t) blockIdx.y * ld + col] = s;
     }
 }
 
-__global__ void k_tempj_reduce(const double *__restrict__ part, int nband, long ld, double *__restrict__ tempJ)
+__global__ void k_tempj_reduce(const double *__restrict__ part, int ngroup, long ld, double *__restrict__ tempJ)
 {
     const long col = (long) blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ld) return;
-    double s = 0.0;
-    for (int b = 0; b < nband; ++b) s += part[(size_t) b * ld + col];
-    tempJ[col] = s;
+    double s0 = 0.0, s1 = 0.0;
+    int g = 0;
+    for (; g + 1 < ngroup; g += 2) {
+        s0 += part[(size_t) g * ld + col];
+        s1 += part[(size_t) (g + 1) * ld + col];
+    }
+    if (g < ngroup) s0 += part[(size_t) g * ld + col];
+    tempJ[col] = s0 + s1;
 }
 
 __global__ __launch_bounds__(256) void k_jmat(const double *__restrict__ B, long ld, long n_up,
@@ -123,25 +150,93 @@ __global__ void k_transpose(int rows, int cols, const double *__restrict__ A, do
     }
 }
 
+__global__ void k_antisym(int n, const double *__restrict__ A, double *__restrict__ out)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    out[idx] = A[idx] - A[(size_t) j * n + i];
+}
+
+// part[j][chunk] = sum over the chunk of x[k] * y_j[k]
+__global__ __launch_bounds__(256) void k_dots(long len, const double *__restrict__ x, const double *__restrict__ ys,
+                                              long ystride, double *__restrict__ part)
+{
+    __shared__ double red[4];
+    const int j = blockIdx.y, nchunk = gridDim.x;
+    const long per = (len + nchunk - 1) / nchunk;
+    const long b = (long) blockIdx.x * per, e = b + per < len ? b + per : len;
+    const double *y = ys + (size_t) j * ystride;
+    double s = 0.0;
+    for (long k = b + threadIdx.x; k < e; k += blockDim.x) s += x[k] * y[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t) j * nchunk + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+struct Coefs {
+    double c[YGPU_MAX_DIIS];
+};
+
+__global__ void k_lincomb(long len, const double *__restrict__ ys, long ystride, int ny, Coefs cf,
+                          double *__restrict__ out)
+{
+    const long k = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= len) return;
+    double s = 0.0;
+    for (int j = 0; j < ny; ++j) s += cf.c[j] * ys[(size_t) j * ystride + k];
+    out[k] = s;
+}
+
 }  // namespace
+
+int yg_antisym(hipStream_t s, int n, const double *A, double *out)
+{
+    const long tot = (long) n * n;
+    hipLaunchKernelGGL(k_antisym, dim3((tot + 255) / 256), dim3(256), 0, s, n, A, out);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part, int nchunk)
+{
+    if (ny <= 0) return YGPU_OK;
+    hipLaunchKernelGGL(k_dots, dim3(nchunk, ny), dim3(256), 0, s, len, x, ys, ystride, part);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, const double *coef, double *out)
+{
+    Coefs cf{};
+    if (ny > YGPU_MAX_DIIS) {
+        ygpu_set_error(YGPU_ERR_INVALID, "linear combination of %d > %d vectors", ny, YGPU_MAX_DIIS);
+        return YGPU_ERR_INVALID;
+    }
+    for (int j = 0; j < ny; ++j) cf.c[j] = coef[j];
+    hipLaunchKernelGGL(k_lincomb, dim3((len + 255) / 256), dim3(256), 0, s, len, ys, ystride, ny, cf, out);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
 
 int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, float *t_Jmat_ms)
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
-    const int ncolb = (int) ((ld + JCOLS - 1) / JCOLS);
-    // bands: enough workgroups to cover the chip several times, >= 16 rows each
-    long nband = std::max<long>(1, std::min<long>(4096 / ncolb, (n_up + 15) / 16));
-    const long rpb = (n_up + nband - 1) / nband;
-    nband = (n_up + rpb - 1) / rpb;
-    YG_TRY(c->d_tempJpart.ensure((size_t) nband * ld * sizeof(double)));
+    const int ncolt = (int) ((ld + JTILE - 1) / JTILE);
+    // row groups: about 2048 workgroups in total, at least 16 rows per wavefront
+    long ngroup = std::max<long>(1, std::min<long>(2048 / ncolt, n_up / (16L * JWAVES)));
+    ngroup = std::min<long>(ngroup, 65535);
+    YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
     YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_tempj_part, dim3(ncolb, nband), dim3(JCOLS), 0, c->stream, c->d_B.as<double>(), ld, n_up,
-                       c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, rpb, c->d_tempJpart.as<double>());
+    hipLaunchKernelGGL(k_tempj_part, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream, c->d_B.as<double>(), ld,
+                       n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, c->stream, c->d_tempJpart.as<double>(),
-                       (int) nband, ld, c->d_tempJ.as<double>());
+                       (int) ngroup, ld, c->d_tempJ.as<double>());
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[1], c->stream));
     YG_HIP(hipMemsetAsync(dJ, 0, (size_t) nbf * nbf * sizeof(double), c->stream));

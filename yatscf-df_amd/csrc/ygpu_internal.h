@@ -85,6 +85,9 @@ struct ygpu_ctx {
     DBuf d_D, d_Cocc, d_H, d_J, d_K, d_F, d_tempJ, d_tempJpart, d_X, d_Kpart;
     DBuf d_mat0, d_mat1, d_mat2, d_eig, d_info, d_evE;
     DBuf d_gA, d_gB, d_gC;
+    DBuf d_S, d_Xo;               // cached overlap and X = S^{-1/2}
+    int have_S = 0, have_H = 0;
+    DBuf d_Fin, d_Din, d_T1, d_T2, d_diisR, d_diisF, d_dots;   // DIIS work and history
     DBuf d_solver_ws;             // rocBLAS/rocSOLVER workspace owned by the library
     size_t solver_ws_bytes = 0;
 };
@@ -109,3 +112,7 @@ int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const d
 int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);
 int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);
 int yg_transpose(hipStream_t s, int rows, int cols, const double *A, double *AT);
+int yg_antisym(hipStream_t s, int n, const double *A, double *out);                  // out = A - A^T
+int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part,
+            int nchunk);                                                               // partial <x, y_j>
+int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, const double *coef, double *out);

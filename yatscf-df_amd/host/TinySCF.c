@@ -23,6 +23,7 @@ void TinySCF_compute_Hcore_Ovlp_mat(TinySCF_t T)
     YS_GPU(ygpu_ovlp_hcore(g, T->S_mat, T->Hcore_mat));
     /* X = S^{-1/2} = U s^{-1/2} U^T (TinySCF.c:57-76) */
     YS_GPU(ygpu_sym_inv_sqrt(g, T->nbasfuncs, T->S_mat, T->X_mat, NULL));
+    YS_GPU(ygpu_set_overlap(g, T->S_mat, T->X_mat));
     T->S_Hcore_time = get_wtime_sec() - st;
     if (T->verbose)
         printf("TinySCF precompute Hcore, S, and X matrices over,  elapsed time = %.3lf (s)\n", T->S_Hcore_time);

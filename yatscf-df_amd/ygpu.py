@@ -63,6 +63,10 @@ SIGNATURES = {
     "ygpu_dgemm": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                    dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int],
     "ygpu_layout_info": [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long), ip],
+    "ygpu_set_overlap": [vp, dp, dp],
+    "ygpu_diis_transform": [vp, dp, dp],
+    "ygpu_diis_update": [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp],
+    "ygpu_diis_extrapolate": [vp, ctypes.c_int, dp, dp],
 }
 
 _glib = None
@@ -266,6 +270,32 @@ class Engine:
         check(self.L.ygpu_dgemm(self.h, int(ta), int(tb), m, n, k, alpha, _d(A), A.shape[1], _d(B), B.shape[1],
                                 beta, _d(C), n))
         return C
+
+    def set_overlap(self, S, X):
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        self._keep["overlap"] = (S, X)
+        check(self.L.ygpu_set_overlap(self.h, _d(S), _d(X)))
+
+    def diis_transform(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        out = np.zeros_like(F)
+        check(self.L.ygpu_diis_transform(self.h, _d(F), _d(out)))
+        return out
+
+    def diis_update(self, slot, length, F, D):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        dots = np.zeros(length)
+        out = np.zeros_like(F)
+        check(self.L.ygpu_diis_update(self.h, slot, length, _d(F), _d(D), _d(dots), _d(out)))
+        return dots, out
+
+    def diis_extrapolate(self, coef):
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        out = np.zeros((self.nbf, self.nbf))
+        check(self.L.ygpu_diis_extrapolate(self.h, len(coef), _d(coef), _d(out)))
+        return out
 
     def layout_info(self):
         a, b, c = ctypes.c_long(0), ctypes.c_long(0), ctypes.c_int(0)
