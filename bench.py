@@ -15,7 +15,16 @@ dominant per-iteration kernel (the X = C_occ^T B build on FP64 MFMA, timed
 with HIP events around its single launch) and the CPU baseline (the oracle
 restatement of the reference algorithm on the host cores, same workload).
 
+With --precision mixed the timed solve starts in the mixed-precision mode
+(K from BF16-split MFMA products with FP32 accumulation, incremental J from
+the FP32 tensor) and switches to FP64 once the DIIS error is below 1e-5.
+Unless --no-mixed-compare is given, one mixed solve of the same workload is
+also run after the timed FP64 steps and reported under "mixed_vs_fp64"
+(time, iterations, energy difference, max |dF| and max |d eps| at
+convergence).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c10h22]
+                       [--precision fp64|mixed] [--no-cpu-baseline] [--no-mixed-compare]
 """
 import argparse
 import json
@@ -39,7 +48,18 @@ WORKLOADS = {
     "c30h62": "C30H62 alkane, torsions +/-30 deg, 0.05 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 760, naux 2910)",
     "c96h24": "C96H24 graphene flake, seeded 0.01 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 1560, naux 6696)",
 }
+# reference defaults (DIIS rule of the reference, |dE| < 1e-11) for the small
+# cases; the large ones converge to the tolerance their configuration names with
+# the DIIS slot of the largest stored residual replaced
+SETTINGS = {
+    "water": dict(ene_tol=1e-11, quirk_a2=1),
+    "c10h22": dict(ene_tol=1e-11, quirk_a2=1),
+    "c54h18": dict(ene_tol=1e-9, quirk_a2=0),
+    "c30h62": dict(ene_tol=1e-9, quirk_a2=0),
+    "c96h24": dict(ene_tol=1e-8, quirk_a2=0),
+}
 FP64_MFMA_PEAK_TFLOPS = 78.6
+BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0   # dense BF16 MFMA peak, three products per FP32 product
 HBM_PEAK_GBS = 8000.0
 
 
@@ -47,9 +67,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_scf(xyz, niters):
+def run_gpu_scf(xyz, niters, settings, mixed=False):
     import ygpu
     T = ygpu.TinySCF(xyz, niters=niters)        # parse + upload shells (untimed)
+    T.setd("ene_tol", settings["ene_tol"])
+    T.seti("quirk_a2", settings["quirk_a2"])
+    T.seti("mixed", int(mixed))
     t0 = time.perf_counter()
     T.setup()
     T.build_df_tensor()
@@ -59,7 +82,8 @@ def run_gpu_scf(xyz, niters):
     hist = T.d("t_hist").reshape(-1, 6).copy()
     res = dict(seconds=dt, iters=n, energy=T.getd("total_energy"), converged=T.geti("converged"),
                hist=hist, t_df=T.d("t_df").copy(), nbf=T.geti("nbasfuncs"), naux=T.geti("df_nbf"),
-               n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"))
+               n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"), switch=T.geti("mixed_switch_iter"),
+               F=T.mat("F_mat").copy(), eigval=T.d("eigval").copy())
     import numpy as np
     mask = T.i("bf_pair_mask").reshape(res["nbf"], res["nbf"])
     iu = np.triu_indices(res["nbf"])
@@ -68,12 +92,12 @@ def run_gpu_scf(xyz, niters):
     return res
 
 
-def run_cpu_baseline(xyz, niters):
+def run_cpu_baseline(xyz, niters, settings):
     import basis
     import oracle
     s = basis.System(os.path.join(PKG, "data", "cc-pvdz", "cc-pvdz.gbs"),
                      os.path.join(PKG, "data", "cc-pVDZ-RI", "cc-pVDZ-RI.gbs"), xyz)
-    o = oracle.OracleSCF(s, niters=niters)
+    o = oracle.OracleSCF(s, niters=niters, ene_tol=settings["ene_tol"], quirk_a2=bool(settings["quirk_a2"]))
     t0 = time.perf_counter()
     o.setup()
     o.build_df_tensor()
@@ -90,8 +114,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c10h22", choices=sorted(WORKLOADS))
     ap.add_argument("--niters", type=int, default=60)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed-compare", action="store_true")
     args = ap.parse_args()
+    settings = SETTINGS[args.workload]
+    mixed = args.precision == "mixed"
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the path is not "
               "sharded across devices (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -103,16 +131,27 @@ def main():
         sys.exit(1)
     xyz = os.path.join(XYZ, args.workload + ".xyz")
     for w in range(args.warmup):
-        r = run_gpu_scf(xyz, args.niters)
+        r = run_gpu_scf(xyz, args.niters, settings, mixed)
         log("warmup %d: %.3f s, %d iterations, E = %.10f" % (w, r["seconds"], r["iters"], r["energy"]))
     runs = []
     for k in range(args.steps):
-        r = run_gpu_scf(xyz, args.niters)
+        r = run_gpu_scf(xyz, args.niters, settings, mixed)
         runs.append(r)
-        log("step %d: %.3f s, %d iterations, E = %.10f" % (k, r["seconds"], r["iters"], r["energy"]))
+        log("step %d: %.3f s, %d iterations, E = %.10f%s" % (k, r["seconds"], r["iters"], r["energy"],
+            (", FP64 from iteration %d" % r["switch"]) if mixed else ""))
     secs = [r["seconds"] for r in runs]
     last = runs[-1]
-    hist = np.vstack([r["hist"][2:] for r in runs if len(r["hist"]) > 2])
+
+    def rows(r, mixed_run, want_mixed):
+        # per-iteration phase times; iterations >= 2 of the requested arithmetic
+        h = r["hist"]
+        sw = (r["switch"] if r["switch"] >= 0 else len(h)) if mixed_run else 0
+        idx = [i for i in range(2, len(h)) if (i < sw) == want_mixed]
+        return h[idx]
+
+    hist = np.vstack([rows(r, mixed, mixed) for r in runs])
+    if len(hist) == 0:
+        hist = np.vstack([r["hist"][2:] for r in runs])
     jk_ms = float(np.median(hist[:, 0] + hist[:, 1] + hist[:, 2] + hist[:, 3]) * 1e3)
     j_ms = float(np.median(hist[:, 0] + hist[:, 1]) * 1e3)
     k_ms = float(np.median(hist[:, 2] + hist[:, 3]) * 1e3)
@@ -122,7 +161,9 @@ def main():
     # X = C_occ^T B over both triangles of the pair list: 2 nnz n_occ naux flops per launch
     xb_flops = 2.0 * nnz * n_occ * naux
     achieved = xb_flops / (xb_ms * 1e-3) / 1e12
-    j_bytes = 2.0 * n_up * naux * 8 + 16.0 * nbf * nbf
+    peak = BF16_SPLIT_PEAK_TFLOPS if mixed else FP64_MFMA_PEAK_TFLOPS
+    # J reads the packed tensor twice (FP32 copy in the mixed phase) and D, J once
+    j_bytes = 2.0 * n_up * naux * (4 if mixed else 8) + 16.0 * nbf * nbf
     out = {
         "metric": "SCF time-to-converged energy (s) + J/K build ms/iter; mixed vs FP64 at 1 uEh",
         "value": statistics.mean(secs),
@@ -134,9 +175,13 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3-split/f32-acc K, f32-tensor incremental J, then f64" if mixed else "f64",
         "data": "synthetic (seeded generated geometry; reference basis-set files)",
-        "config": {"workload": WORKLOADS[args.workload], "precision": "FP64 (plain path)",
+        "config": {"workload": WORKLOADS[args.workload],
+                   "precision": ("mixed until DIIS error < 1e-5, then FP64" if mixed else "FP64 (plain path)"),
+                   "ene_tol": settings["ene_tol"],
+                   "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
+                                   "largest stored residual",
                    "basis": "cc-pVDZ", "aux_basis": "cc-pVDZ-RI", "nbf": nbf, "naux": naux, "n_occ": n_occ,
                    "pairs_both_triangles": nnz, "pairs_packed": n_up, "schwarz_tol": 1e-11},
         "jk_ms_per_iter": jk_ms,
@@ -147,16 +192,36 @@ def main():
         "converged": bool(last["converged"]),
         "df_build_s": {"3c": last["t_df"][0], "2c": last["t_df"][1], "inv_sqrt": last["t_df"][2],
                        "df_gemm": last["t_df"][3]},
-        "roofline": {"kernel": "k_xbuild (X_i = C_occ^T B rows of i, FP64 MFMA 16x16x4)",
-                     "bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+        "roofline": {"kernel": ("k_xbuild_bf3 (X_i = C_occ^T B rows of i, BF16-split MFMA 32x32x16)" if mixed
+                               else "k_xbuild (X_i = C_occ^T B rows of i, FP64 MFMA 16x16x4)"),
+                     "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
                      "launch_ms": xb_ms, "flops_per_launch": xb_flops},
         "j_build": {"bound": "hbm", "ms": j_ms, "bytes_per_call": j_bytes,
                     "achieved_GBs": j_bytes / (j_ms * 1e-3) / 1e9, "peak_GBs": HBM_PEAK_GBS},
         "syrk_ms": syrk_ms,
     }
+    if mixed:
+        out["mixed_switch_iteration"] = last["switch"]
+    if not args.no_mixed_compare:
+        # the other arithmetic on the same workload, for the mixed-vs-FP64 comparison
+        other = run_gpu_scf(xyz, args.niters, settings, not mixed)
+        fp, mx = (other, last) if mixed else (last, other)
+        mh = rows(mx, True, True)
+        out["mixed_vs_fp64"] = {
+            "fp64_s": fp["seconds"], "mixed_s": mx["seconds"], "speedup": fp["seconds"] / mx["seconds"],
+            "fp64_iters": fp["iters"], "mixed_iters": mx["iters"], "mixed_switch_iteration": mx["switch"],
+            "dE_Eh": mx["energy"] - fp["energy"],
+            "max_abs_dF": float(np.max(np.abs(mx["F"] - fp["F"]))),
+            "max_abs_d_orbital_energy": float(np.max(np.abs(mx["eigval"] - fp["eigval"]))),
+            "mixed_k_ms_per_iter": float(np.median(mh[:, 2] + mh[:, 3]) * 1e3) if len(mh) else None,
+            "mixed_j_ms_per_iter": float(np.median(mh[:, 0] + mh[:, 1]) * 1e3) if len(mh) else None,
+            "mixed_xbuild_tflops": (xb_flops / (float(np.median(mh[:, 2])) * 1e12)) if len(mh) else None,
+            "converged": bool(mx["converged"] and fp["converged"]),
+        }
+        log("mixed vs FP64: %s" % json.dumps(out["mixed_vs_fp64"]))
     if not args.no_cpu_baseline:
-        cb = run_cpu_baseline(xyz, args.niters)
+        cb = run_cpu_baseline(xyz, args.niters, settings)
         out["cpu_baseline"] = {"value": cb["seconds"], "unit": "s", "cores": cb["cores"], "kind": "port",
                                "sample": "one full SCF of the same workload (setup, DF build, %d iterations) by the "
                                          "oracle (C/OpenMP + %s)" % (cb["iters"], "MKL" if cb["mkl"] else "built-in BLAS")}

@@ -117,6 +117,22 @@ int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *C
                     const double *Hcore, double *J, double *K, double *F,
                     ygpu_fock_timing_t *timing);
 
+/* Arithmetic of the following ygpu_build_fock calls.  The reference builds
+ * J and K in FP64 only (build_Fock.c:37-347); YGPU_FOCK_MIXED is this
+ * library's reduced-precision mode for the early SCF iterations:
+ *   K: X = C_occ^T B and K = X X^T as BF16-split products (hi*hi + hi*lo +
+ *      lo*hi of bf16 hi/lo parts) on the matrix cores with FP32
+ *      accumulation, FP32 X, FP64 sums across k chunks;
+ *   J: incremental, J = J_prev + J(D - D_prev), from an FP32 copy of B with
+ *      FP64 arithmetic; the first build after entering the mode is a full
+ *      FP64 J that starts the increments.
+ * YGPU_FOCK_FP64 (the default) is the reference arithmetic; switching modes
+ * restarts the incremental J.  The FP32 copy of B costs n_up x naux_pad x 4
+ * bytes of device memory while the mode is in use. */
+#define YGPU_FOCK_FP64 0
+#define YGPU_FOCK_MIXED 1
+int ygpu_set_fock_precision(ygpu_ctx_t *ctx, int mode);
+
 /* Density from the orthogonal-basis Fock matrix F_orth = X^T F X
  * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
  * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.
@@ -145,6 +161,9 @@ int ygpu_diis_transform(ygpu_ctx_t *ctx, const double *F, double *F_orth);
 int ygpu_diis_update(ygpu_ctx_t *ctx, int slot, int len, const double *F, const double *D, double *dots,
                      double *F_orth);
 int ygpu_diis_extrapolate(ygpu_ctx_t *ctx, int len, const double *coef, double *F_out);
+/* max |R| of the residual stored in history slot `slot` (the DIIS error that
+ * DIIS.c:57-65 forms; used to leave the mixed-precision phase). */
+int ygpu_diis_error(ygpu_ctx_t *ctx, int slot, double *max_abs);
 
 /* Row-major FP64 GEMM on host operands, C = alpha op(A) op(B) + beta C
  * (the cblas_dgemm calls of DIIS.c:34-107). */

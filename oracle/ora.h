@@ -82,6 +82,7 @@ ora_scf_t *ora_create(const ora_basis_t *bas, const ora_basis_t *aux,
 void   ora_destroy(ora_scf_t *s);
 void   ora_set_flags(ora_scf_t *s, int quirk_a5_schwarz, int quirk_a1_d2cocc,
                      double ene_tol, double scrtol);
+void   ora_set_quirk_a2(ora_scf_t *s, int quirk_a2_diis_bmax);
 int    ora_setup(ora_scf_t *s);            /* S,H,X; Schwarz; sparsity; SAD; D2Cocc */
 int    ora_build_df_tensor(ora_scf_t *s);
 int    ora_build_fock(ora_scf_t *s);

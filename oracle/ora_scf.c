@@ -34,7 +34,7 @@
 #define MAX_DIIS 10
 
 struct ora_scf {
-    int quirk_a5, quirk_a1;
+    int quirk_a5, quirk_a1, quirk_a2;
     double ene_tol, shell_scrtol2;
     int nthreads;
 
@@ -111,7 +111,7 @@ ora_scf_t *ora_create(const ora_basis_t *bas, const ora_basis_t *aux,
 {
     ora_lapack_init();
     ora_scf_t *s = (ora_scf_t *) xcalloc(1, sizeof(ora_scf_t));
-    s->quirk_a5 = 1; s->quirk_a1 = 1;
+    s->quirk_a5 = 1; s->quirk_a1 = 1; s->quirk_a2 = 1;
     s->ene_tol = 1e-11;                       /* TinySCF_init_free.c:75 */
     s->shell_scrtol2 = 1e-11 * 1e-11;         /* TinySCF_init_free.c:78 */
     s->nthreads = ora_get_threads();
@@ -218,6 +218,12 @@ void ora_set_flags(ora_scf_t *s, int qa5, int qa1, double ene_tol, double scrtol
     s->quirk_a5 = qa5; s->quirk_a1 = qa1;
     if (ene_tol > 0) s->ene_tol = ene_tol;
     if (scrtol > 0) s->shell_scrtol2 = scrtol * scrtol;
+}
+
+/* quirk A2 off: the replaced slot is the largest residual among the stored ones */
+void ora_set_quirk_a2(ora_scf_t *s, int qa2)
+{
+    s->quirk_a2 = qa2;
 }
 
 /* ---------------- setup ---------------- */
@@ -635,6 +641,7 @@ int ora_diis(ora_scf_t *s)
         dot[j] = d;
     }
     for (int i = 0; i < s->DIIS_len; i++) { s->B[idx * ldB + i] = dot[i]; s->B[i * ldB + idx] = dot[i]; }
+    if (!s->quirk_a2) s->DIIS_bmax = -DBL_MAX;                      /* A2 off: recomputed */
     for (int i = 0; i < s->DIIS_len; i++)                            /* A2: bmax never reset */
         if (s->B[i * ldB + i] > s->DIIS_bmax) { s->DIIS_bmax = s->B[i * ldB + i]; s->DIIS_bmax_id = i; }
     ora_dgemm(1, 0, nbf, nbf, nbf, 1.0, s->X, nbf, s->F, nbf, 0.0, tmp, nbf);
@@ -759,6 +766,7 @@ int ora_get_int(ora_scf_t *s, const char *n)
     if (!strcmp(n, "num_uniq_sp")) return s->num_uniq_sp;
     if (!strcmp(n, "num_bf_pair_scr")) return s->num_bf_pair_scr;
     if (!strcmp(n, "DIIS_len")) return s->DIIS_len;
+    if (!strcmp(n, "DIIS_bmax_id")) return s->DIIS_bmax_id;
     if (!strcmp(n, "converged")) return s->converged;
     if (!strcmp(n, "nthreads")) return s->nthreads;
     if (!strcmp(n, "mkl")) return ora_lapack_is_mkl();
@@ -776,7 +784,7 @@ double ora_get_double(ora_scf_t *s, const char *n)
 #define RET(name, v) if (!strcmp(n, name)) return (v);
     RET("HF_energy", s->HF_energy) RET("nuc_energy", s->nuc_energy)
     RET("total_energy", s->HF_energy + s->nuc_energy) RET("energy_delta", s->energy_delta)
-    RET("max_scrval", s->max_scrval) RET("max_df_scrval", s->max_df_scrval)
+    RET("max_scrval", s->max_scrval) RET("max_df_scrval", s->max_df_scrval) RET("DIIS_bmax", s->DIIS_bmax)
     RET("t_setup", s->t_setup) RET("t_3c", s->t_3c) RET("t_2c", s->t_2c)
     RET("t_invsqrt", s->t_invsqrt) RET("t_dfgemm", s->t_dfgemm)
     RET("t_tempJ", s->t_tempJ) RET("t_Jmat", s->t_Jmat) RET("t_tempK", s->t_tempK)

@@ -60,6 +60,8 @@ def lib():
         L.ora_destroy.restype = None
         L.ora_set_flags.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.ora_set_flags.restype = None
+        L.ora_set_quirk_a2.argtypes = [vp, ctypes.c_int]
+        L.ora_set_quirk_a2.restype = None
         L.ora_get_dptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
         L.ora_get_dptr.restype = dp
         L.ora_get_iptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
@@ -167,7 +169,8 @@ def is_mkl():
 class OracleSCF:
     """The reference SCF engine restated on the CPU, driven from Python."""
 
-    def __init__(self, system, niters=50, quirk_a5=True, quirk_a1=True, ene_tol=1e-11, scrtol=1e-11):
+    def __init__(self, system, niters=50, quirk_a5=True, quirk_a1=True, ene_tol=1e-11, scrtol=1e-11,
+                 quirk_a2=True):
         L = lib()
         self.sys = system
         self._keep = []
@@ -181,6 +184,7 @@ class OracleSCF:
                               system.mol["charge"], system.mol["nelectrons"], system.mol["enuc"],
                               _dp(sad), niters)
         L.ora_set_flags(self.h, int(quirk_a5), int(quirk_a1), ene_tol, scrtol)
+        L.ora_set_quirk_a2(self.h, int(quirk_a2))
 
     def _mk_basis(self, b):
         arrs = [np.ascontiguousarray(b[k]) for k in ("am", "nprim", "prim_off")]

@@ -1,0 +1,167 @@
+"""Mixed-precision Fock builds on the GPU (ygpu_set_fock_precision).
+
+K: BF16-split MFMA products with FP32 accumulation, checked against the
+bit-exact CPU emulation of the same split (oracle/mixed.py; the only
+difference left is the GPU's FP32 accumulation) and against the FP64 K.
+J: incremental from the FP32 tensor; the first build is full FP64.
+SCF: the mixed phase followed by FP64 iterations converges to the FP64
+oracle energy within 1e-8 Eh."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import system, xyz_path
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ygpu():
+    import ygpu as m
+    assert m.device_count() >= 1, "no GPU visible"
+    return m
+
+
+class Ref:
+    """Two consecutive (D, C_occ) states of the oracle SCF and their FP64 J, K, F."""
+
+    def __init__(self, name):
+        import oracle
+        self.sys = s = system(name)
+        o = oracle.OracleSCF(s, niters=60)
+        o.setup()
+        o.build_df_tensor()
+        n, nocc = s.nbf, s.n_occ
+        self.o = o
+        self.H = o.mat("Hcore_mat").copy()
+        self.states = []
+        for _ in range(2):
+            D = o.mat("D_mat").copy()
+            C = o.d("Cocc_mat", (n, nocc)).copy()
+            o.build_fock()
+            self.states.append(dict(D=D, C=C, J=o.mat("J_mat").copy(), K=o.mat("K_mat").copy(),
+                                    F=o.mat("F_mat").copy()))
+            o.diis()
+            o.build_density()
+        nnz = o.geti("num_bf_pair_scr")
+        self.B = o.d("df_tensor").reshape(nnz, s.naux).copy()
+        self.mask = o.i("bf_pair_mask").reshape(n, n).copy()
+
+
+_refs = {}
+
+
+def ref(name):
+    if name not in _refs:
+        _refs[name] = Ref(name)
+    return _refs[name]
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_mixed_fock_against_emulation(ygpu, name):
+    import mixed
+    log("case: mixed Fock", name)
+    r = ref(name)
+    o = r.o
+    e = ygpu.Engine(0)
+    try:
+        e.set_system(r.sys)
+        e.set_sparsity(o.i("bf_pair_mask"), o.i("uniq_sp_lid"), o.i("uniq_sp_rid"), o.d("sp_scrval"),
+                       o.d("df_sp_scrval"))
+        e.set_df_tensor(o.d("df_tensor"))
+        e.set_fock_precision(ygpu.FOCK_MIXED)
+        out = []
+        for st in r.states:
+            out.append(e.build_fock(st["D"], st["C"], r.H))
+        e.set_fock_precision(ygpu.FOCK_FP64)
+        back = e.build_fock(r.states[1]["D"], r.states[1]["C"], r.H)
+    finally:
+        e.close()
+    for k, (st, got) in enumerate(zip(r.states, out)):
+        Kem, X32 = mixed.k_build(r.B, r.mask, st["C"])
+        # S_ij = sum_k |X_ik| |X_jk| >= |K_ij|: the scale of rounding errors made in the K product;
+        # T_ij = sum_k (A_ik |X_jk| + |X_ik| A_jk) with A = |C|^T |B| summed like X: the scale of
+        # the FP32 accumulation errors of X carried into K
+        Xa = np.abs(X32.reshape(X32.shape[0], -1).astype(np.float64))
+        A = mixed.abs_x(r.B, r.mask, st["C"]).reshape(Xa.shape)
+        S = Xa @ Xa.T + 1e-300
+        T = A @ Xa.T + Xa @ A.T + 1e-300
+        kmax = np.abs(st["K"]).max()
+        e_emul = np.abs(got["K"] - Kem) / T
+        e_fp64 = np.abs(got["K"] - st["K"]) / S
+        d_J = np.abs(got["J"] - st["J"]).max()
+        log("  build %d: max|K| %.3e  max|dK| vs emulation %.3e (max |dK|/T %.2e), vs FP64 %.3e (max |dK|/S %.2e),"
+            " emulation vs FP64 %.3e; max|dJ| %.3e (max|J| %.3e)"
+            % (k, kmax, np.abs(got["K"] - Kem).max(), e_emul.max(), np.abs(got["K"] - st["K"]).max(), e_fp64.max(),
+               np.abs(Kem - st["K"]).max(), d_J, np.abs(st["J"]).max()))
+        # the GPU differs from the exact-split emulation only by its FP32 accumulation: one
+        # rounding per MFMA (three per 16-deep step) in X and in each 1024-long chunk of K,
+        # ~1.5e-6 T when emulated step by step for c10h22; a dropped split term shows at ~1e-4 T
+        assert e_emul.max() < 2.0 ** -17
+        # the method's own error: <= 3 2^-17 per product plus the accumulation
+        assert e_fp64.max() < 2.0 ** -14
+        assert np.allclose(got["K"], got["K"].T, rtol=0, atol=0)
+        if k == 0:
+            assert d_J < 1e-10            # first mixed build: full FP64 J
+        else:
+            # J_prev + J(D - D_prev) from the FP32 tensor: error ~ 2^-24 |dD| |B|^2
+            assert d_J < 1e-7 * np.abs(st["J"]).max()
+        F = r.H + 2 * got["J"] - got["K"]
+        assert np.abs(got["F"] - F).max() < 1e-12 * max(1.0, np.abs(F).max())
+    # back to FP64: the reference arithmetic again
+    assert np.abs(back["F"] - r.states[1]["F"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_diis_error_is_max_abs_residual(ygpu, name):
+    log("case: DIIS error", name)
+    r = ref(name)
+    o = r.o
+    S, X = o.mat("S_mat").copy(), o.mat("X_mat").copy()
+    st = r.states[1]
+    e = ygpu.Engine(0)
+    try:
+        e.set_system(r.sys)
+        e.set_overlap(S, X)
+        e.diis_update(0, 1, st["F"], st["D"])
+        err = e.diis_error(0)
+    finally:
+        e.close()
+    FDS = st["F"] @ st["D"] @ S
+    R = X.T @ (FDS - FDS.T) @ X
+    log("  max|R| gpu %.6e numpy %.6e" % (err, np.abs(R).max()))
+    assert abs(err - np.abs(R).max()) <= 1e-12 * max(1.0, np.abs(R).max())
+
+
+def test_mixed_scf_reaches_fp64_energy(ygpu):
+    """Mixed phase until the DIIS error < 1e-5 (or stalls), then FP64 to convergence."""
+    import oracle
+    name = "c10h22"
+    log("case: mixed SCF", name)
+    s = system(name)
+    o = oracle.OracleSCF(s, niters=60)
+    o.setup()
+    o.build_df_tensor()
+    o.do_scf()
+    e_ref = o.getd("total_energy")
+    T = ygpu.TinySCF(xyz_path(name), niters=60)
+    try:
+        T.seti("mixed", 1)
+        T.setup()
+        T.build_df_tensor()
+        T.do_scf()
+        e = T.getd("total_energy")
+        sw = T.geti("mixed_switch_iter")
+        log("  E mixed %.10f oracle %.10f dE %.3e iters %d (oracle %d), FP64 from iteration %d"
+            % (e, e_ref, e - e_ref, T.geti("iter"), o.geti("iter"), sw))
+        assert T.geti("converged") == 1
+        assert sw >= 3
+        assert abs(e - e_ref) < 1e-8
+    finally:
+        T.close()

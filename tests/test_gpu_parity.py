@@ -231,13 +231,14 @@ def test_density(ygpu, name):
         assert np.all(idx == np.arange(n))
 
 
-@pytest.mark.parametrize("name", ["water", "c10h22"])
-def test_full_scf_energy(ygpu, name):
-    """Drop-in TinySCF engine (host C + device) against the oracle SCF."""
+@pytest.mark.parametrize("name,a2", [("water", True), ("c10h22", True), ("c10h22", False)])
+def test_full_scf_energy(ygpu, name, a2):
+    """Drop-in TinySCF engine (host C + device) against the oracle SCF; a2=False takes the
+    DIIS slot to replace as the largest stored residual instead of the largest ever."""
     import oracle
-    log("case: full SCF", name)
+    log("case: full SCF", name, "quirk_a2", a2)
     s = system(name)
-    o = oracle.OracleSCF(s, niters=60)
+    o = oracle.OracleSCF(s, niters=60, quirk_a2=a2)
     o.setup()
     o.build_df_tensor()
     o.do_scf()
@@ -245,6 +246,7 @@ def test_full_scf_energy(ygpu, name):
     it_ref = o.geti("iter")
     T = ygpu.TinySCF(xyz_path(name), niters=60)
     try:
+        T.seti("quirk_a2", int(a2))
         T.setup()
         T.build_df_tensor()
         T.do_scf()

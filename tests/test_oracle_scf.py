@@ -143,3 +143,28 @@ def test_setup_quirk_switches():
     # the quirk permutes values only inside shell blocks of dimension > 1
     assert np.allclose(np.diag(va)[:1], np.diag(vb)[:1])
     assert np.allclose(a.d("sp_scrval").reshape(-1).max(), b.d("sp_scrval").max())
+
+
+@pytest.mark.parametrize("a2", [True, False])
+def test_diis_replacement_rule(a2):
+    """DIIS.c:93-100: once MAX_DIIS (10) vectors are stored, the slot of the largest residual
+    norm is replaced; the reference never resets that maximum (quirk A2), so the slot stays
+    the one of the largest residual ever seen.  With the switch off the maximum is taken over
+    the stored residuals."""
+    s = system("water")
+    o = oracle.OracleSCF(s, niters=16, ene_tol=1e-16, quirk_a2=a2)
+    o.setup()
+    o.build_df_tensor()
+    o.do_scf()
+    assert o.geti("iter") == 16 and o.geti("DIIS_len") == 10
+    B = o.d("B_mat").reshape(11, 11)
+    diag = np.diag(B)[:10]
+    R = o.d("R_mat").reshape(10, -1)
+    # the B diagonal is the squared Frobenius norm of each stored residual
+    assert np.allclose(diag, np.sum(R * R, axis=1), rtol=1e-12, atol=0)
+    if a2:
+        # replacements happened, so the stored maxima are below the first-ever maximum
+        assert o.getd("DIIS_bmax") > diag.max()
+    else:
+        assert o.getd("DIIS_bmax") == diag.max()
+        assert o.geti("DIIS_bmax_id") == int(np.argmax(diag))

@@ -535,7 +535,10 @@ try {
                            c->naux, hipMemcpyDeviceToHost));
     if (pqA_out) YG_TRY(unpack_to_reference(c, c->d_pqA.as<double>(), pqA_out));
     if (df_out) YG_TRY(unpack_to_reference(c, c->d_B.as<double>(), df_out));
+    c->d_pqA.release();   // (mn|P) is not needed once B is formed
     c->have_df = 1;
+    c->have_B32 = 0;
+    c->jinc_valid = 0;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -568,6 +571,8 @@ try {
     YG_TRY(upload(c->d_B, up.data(), up.size() * sizeof(double), c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     c->have_df = 1;
+    c->have_B32 = 0;
+    c->jinc_valid = 0;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -597,16 +602,51 @@ try {
     }
     YG_HIP(hipMemcpyAsync(c->d_Cocc.p, Cocc, (size_t) nbf * n_occ * sizeof(double), hipMemcpyHostToDevice, c->stream));
     float tj1 = 0.f, tj2 = 0.f;
-    YG_HIP(hipEventRecord(c->ev[5], c->stream));
-    YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr));
-    YG_HIP(hipEventRecord(c->ev[3], c->stream));
-    YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
-                     c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B.as<double>(), c->d_X.as<double>(),
-                     c->max_row_len));
-    YG_HIP(hipEventRecord(c->ev[4], c->stream));
-    YG_TRY(yg_syrk_upper(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<double>(), c->d_K.as<double>(),
-                         c->d_Kpart));
-    YG_HIP(hipEventRecord(c->ev[6], c->stream));
+    const long nn = (long) nbf * nbf;
+    if (c->fock_mode == YGPU_FOCK_MIXED) {
+        if (!c->have_B32) {
+            YG_TRY(c->d_B32.ensure((size_t) c->n_up * c->naux_pad * sizeof(float)));
+            YG_TRY(yg_to_f32(c->stream, c->d_B.as<double>(), c->d_B32.as<float>(), c->n_up * c->naux_pad));
+            c->have_B32 = 1;
+        }
+        YG_TRY(c->d_Dprev.ensure(ms));
+        YG_TRY(c->d_Jprev.ensure(ms));
+        YG_TRY(c->d_dD.ensure(ms));
+        YG_HIP(hipEventRecord(c->ev[5], c->stream));
+        if (!c->jinc_valid) {
+            // first build of a mixed run: full FP64 J, kept with its D as the base of the increments
+            YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+            YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->stream));
+            YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->stream));
+            c->jinc_valid = 1;
+        } else {
+            // J = J_prev + J(D - D_prev) from the FP32 tensor with FP64 arithmetic: the
+            // rounding of the tensor enters only through the small difference density
+            YG_TRY(yg_delta(c->stream, nn, c->d_D.as<double>(), c->d_Dprev.as<double>(), c->d_dD.as<double>()));
+            YG_TRY(yg_jbuild(c, c->d_dD.as<double>(), c->d_J.as<double>(), c->d_B32.as<float>(), nullptr, nullptr));
+            YG_TRY(yg_accum(c->stream, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
+        }
+        YG_HIP(hipEventRecord(c->ev[3], c->stream));
+        // K from BF16-split products; X in FP32 occupies the front half of the FP64 X buffer
+        YG_TRY(yg_xbuild_bf3(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
+                             c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B32.as<float>(),
+                             c->d_X.as<float>()));
+        YG_HIP(hipEventRecord(c->ev[4], c->stream));
+        YG_TRY(yg_syrk_bf3(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(), c->d_K.as<double>(),
+                           c->d_Kpart32));
+        YG_HIP(hipEventRecord(c->ev[6], c->stream));
+    } else {
+        YG_HIP(hipEventRecord(c->ev[5], c->stream));
+        YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+        YG_HIP(hipEventRecord(c->ev[3], c->stream));
+        YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
+                         c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B.as<double>(), c->d_X.as<double>(),
+                         c->max_row_len));
+        YG_HIP(hipEventRecord(c->ev[4], c->stream));
+        YG_TRY(yg_syrk_upper(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<double>(), c->d_K.as<double>(),
+                             c->d_Kpart));
+        YG_HIP(hipEventRecord(c->ev[6], c->stream));
+    }
     YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
                             c->d_F.as<double>()));
     YG_HIP(hipEventRecord(c->ev[7], c->stream));
@@ -623,6 +663,26 @@ try {
         t->t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
         t->t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
         t->t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
+    }
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_fock_precision(ygpu_ctx_t *c, int mode)
+try {
+    YG_TRY(check_ctx(c));
+    if (mode != YGPU_FOCK_FP64 && mode != YGPU_FOCK_MIXED) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_fock_precision: unknown mode %d", mode);
+        return YGPU_ERR_INVALID;
+    }
+    if (mode != c->fock_mode) c->jinc_valid = 0;
+    c->fock_mode = mode;
+    if (mode == YGPU_FOCK_FP64) {
+        // the FP32 tensor copy and the incremental state are only used by the mixed mode
+        c->d_B32.release();
+        c->have_B32 = 0;
+        c->d_Kpart32.release();
     }
     return YGPU_OK;
 } catch (...) {
@@ -776,6 +836,23 @@ try {
         for (int k = 0; k < nchunk; ++k) sacc += part[(size_t) j * nchunk + k];
         dots[j] = sacc;
     }
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_diis_error(ygpu_ctx_t *c, int slot, double *max_abs)
+try {
+    YG_TRY(check_ctx(c));
+    if (slot < 0 || slot >= YGPU_MAX_DIIS || !max_abs || !c->d_diisR.p) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_error: bad arguments (slot %d)", slot);
+        return YGPU_ERR_INVALID;
+    }
+    const long nn = (long) c->nbf * c->nbf;
+    YG_TRY(c->d_dots.ensure(sizeof(double)));
+    YG_TRY(yg_maxabs(c->stream, nn, c->d_diisR.as<double>() + (size_t) slot * nn, c->d_dots.as<double>()));
+    YG_HIP(hipMemcpyAsync(max_abs, c->d_dots.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
 } catch (...) {
     return exception_status();

@@ -10,6 +10,8 @@
 //   pass 2 "J matrix": J_ij = B[r] . temp_J with temp_J staged in LDS, one
 //     wavefront per row, cross-lane reduction with shuffles.
 // Screened pairs keep J = 0 (quirk A7 of the reference made explicit).
+// Both passes read the tensor as FP64 or, in the mixed-precision phase, as its
+// FP32 copy (half the bytes); the arithmetic is FP64 either way.
 #include "ygpu_internal.h"
 
 #include <algorithm>
@@ -23,7 +25,8 @@ constexpr int JWAVES = 4;      // wavefronts per pass-1 workgroup, each on its o
 // (column tiles, row groups); rows are interleaved over (group, wavefront) so
 // every wavefront streams whole 512-byte row segments, and the row index is
 // wave-uniform (scalar loads of the pair indices and of D).
-__global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const double *__restrict__ B, long ld, long n_up,
+template <class TB>
+__global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const TB *__restrict__ B, long ld, long n_up,
                                                                const int *__restrict__ up_i, const int *__restrict__ up_j,
                                                                const double *__restrict__ D, int nbf,
                                                                double *__restrict__ part)
@@ -43,7 +46,7 @@ __global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const double *__r
             const long rq = __builtin_amdgcn_readfirstlane((int) (r + q * stride));
             const int i = up_i[rq], j = up_j[rq];
             wt[q] = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
-            bv[q] = cin ? B[(size_t) rq * ld + col] : 0.0;
+            bv[q] = cin ? (double) B[(size_t) rq * ld + col] : 0.0;
         }
         a0 += wt[0] * bv[0];
         a1 += wt[1] * bv[1];
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const double *__r
     for (; r < n_up; r += stride) {
         const long rq = __builtin_amdgcn_readfirstlane((int) r);
         const int i = up_i[rq], j = up_j[rq];
-        a0 += (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j] * (cin ? B[(size_t) rq * ld + col] : 0.0);
+        a0 += (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j] * (cin ? (double) B[(size_t) rq * ld + col] : 0.0);
     }
     red[w][lane] = (a0 + a1) + (a2 + a3);
     __syncthreads();
@@ -79,7 +82,8 @@ __global__ void k_tempj_reduce(const double *__restrict__ part, int ngroup, long
     tempJ[col] = s0 + s1;
 }
 
-__global__ __launch_bounds__(256) void k_jmat(const double *__restrict__ B, long ld, long n_up,
+template <class TB>
+__global__ __launch_bounds__(256) void k_jmat(const TB *__restrict__ B, long ld, long n_up,
                                               const int *__restrict__ up_i, const int *__restrict__ up_j,
                                               const double *__restrict__ tempJ, int nbf, int rows_per_wave,
                                               double *__restrict__ J)
@@ -92,9 +96,9 @@ __global__ __launch_bounds__(256) void k_jmat(const double *__restrict__ B, long
     for (int q = 0; q < rows_per_wave; ++q) {
         const long r = rbase + q;
         if (r >= n_up) break;
-        const double *row = B + (size_t) r * ld;
+        const TB *row = B + (size_t) r * ld;
         double s = 0.0;
-        for (long c = lane; c < ld; c += 64) s += row[c] * tj[c];
+        for (long c = lane; c < ld; c += 64) s += (double) row[c] * tj[c];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) J[(size_t) up_i[r] * nbf + up_j[r]] = s;
@@ -158,6 +162,42 @@ __global__ void k_antisym(int n, const double *__restrict__ A, double *__restric
     out[idx] = A[idx] - A[(size_t) j * n + i];
 }
 
+// difference density of the incremental J: dD = D - Dprev, then Dprev = D
+__global__ void k_delta(long n, const double *__restrict__ D, double *__restrict__ Dprev, double *__restrict__ dD)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double d = D[idx];
+    dD[idx] = d - Dprev[idx];
+    Dprev[idx] = d;
+}
+
+// J += Jprev, then Jprev = J
+__global__ void k_accum(long n, double *__restrict__ J, double *__restrict__ Jprev)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double v = J[idx] + Jprev[idx];
+    J[idx] = v;
+    Jprev[idx] = v;
+}
+
+// out[0] = max |x|, one workgroup
+__global__ __launch_bounds__(1024) void k_maxabs(long n, const double *__restrict__ x, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double m = 0.0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(x[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int) (blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+        out[0] = m;
+    }
+}
+
 // part[j][chunk] = sum over the chunk of x[k] * y_j[k]
 __global__ __launch_bounds__(256) void k_dots(long len, const double *__restrict__ x, const double *__restrict__ ys,
                                               long ystride, double *__restrict__ part)
@@ -200,6 +240,27 @@ int yg_antisym(hipStream_t s, int n, const double *A, double *out)
     return YGPU_OK;
 }
 
+int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD)
+{
+    hipLaunchKernelGGL(k_delta, dim3((n + 255) / 256), dim3(256), 0, s, n, D, Dprev, dD);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_accum(hipStream_t s, long n, double *J, double *Jprev)
+{
+    hipLaunchKernelGGL(k_accum, dim3((n + 255) / 256), dim3(256), 0, s, n, J, Jprev);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_maxabs(hipStream_t s, long n, const double *x, double *out)
+{
+    hipLaunchKernelGGL(k_maxabs, dim3(1), dim3(1024), 0, s, n, x, out);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
 int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part, int nchunk)
 {
     if (ny <= 0) return YGPU_OK;
@@ -221,7 +282,7 @@ int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, 
     return YGPU_OK;
 }
 
-int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, float *t_Jmat_ms)
+int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms)
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
@@ -232,8 +293,13 @@ int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, floa
     YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
     YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_tempj_part, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream, c->d_B.as<double>(), ld,
-                       n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
+    if (B32)
+        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream, B32, ld, n_up,
+                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
+    else
+        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream,
+                           c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf,
+                           c->d_tempJpart.as<double>());
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, c->stream, c->d_tempJpart.as<double>(),
                        (int) ngroup, ld, c->d_tempJ.as<double>());
@@ -242,8 +308,12 @@ int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, floa
     YG_HIP(hipMemsetAsync(dJ, 0, (size_t) nbf * nbf * sizeof(double), c->stream));
     const int rpw = 4;
     const long nblk = (n_up + 4L * rpw - 1) / (4L * rpw);
-    hipLaunchKernelGGL(k_jmat, dim3(nblk), dim3(256), ld * sizeof(double), c->stream, c->d_B.as<double>(), ld, n_up,
-                       c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
+    if (B32)
+        hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(256), ld * sizeof(double), c->stream, B32, ld, n_up,
+                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
+    else
+        hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(256), ld * sizeof(double), c->stream, c->d_B.as<double>(),
+                           ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[2], c->stream));
     if (t_tempJ_ms || t_Jmat_ms) {

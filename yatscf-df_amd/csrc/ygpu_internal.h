@@ -90,6 +90,12 @@ struct ygpu_ctx {
     DBuf d_Fin, d_Din, d_T1, d_T2, d_diisR, d_diisF, d_dots;   // DIIS work and history
     DBuf d_solver_ws;             // rocBLAS/rocSOLVER workspace owned by the library
     size_t solver_ws_bytes = 0;
+
+    // mixed-precision Fock builds (ygpu_set_fock_precision)
+    int fock_mode = YGPU_FOCK_FP64;
+    int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
+    int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
+    DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32;
 };
 
 // --------------------------------------------------------------------------
@@ -107,7 +113,16 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
 // upper-triangle K = X X^T with split-K partials in Kpart, result in K (full, mirrored)
 int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part);
-int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, float *t_tempJ_ms, float *t_Jmat_ms);
+// J from density dD (upper triangle of dJ; the rest zero); B32 != nullptr reads the FP32 copy
+int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms);
+int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
+int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
+int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
+// mixed precision (BF16-split MFMA, FP32 accumulation)
+int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
+int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                  const int *xpartner, const double *Cocc, const float *B32, float *X);
+int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part);
 int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const double *K, double *F);
 int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);
 int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);

@@ -1,0 +1,306 @@
+// ygpu_mixed.hip -- reduced-precision K build for the mixed-precision SCF phase.
+//
+// The two K products of build_Fock.c:201-347,
+//   X_i = C_occ(partners of i)^T B(rows of i)   ("aux. K tensor", :249-285)
+//   K   = X X^T                                  ("K matrix", :317-341)
+// run as BF16-split products on the gfx950 matrix cores: every FP32 operand
+// x is split into hi = bf16(x) and lo = bf16(x - hi), and a*b is formed as
+// hi_a*hi_b + hi_a*lo_b + lo_a*hi_b (the lo*lo term, ~2^-18 relative, is
+// dropped) with v_mfma_f32_32x32x16_bf16 and FP32 accumulation.  The split
+// keeps ~16 significant bits per operand at three BF16 MFMAs per product,
+// against one FP32 MFMA at a sixteenth of the BF16 rate.
+//
+// Tile engine: 256 threads = 4 wavefronts, each owning a 64x64 output
+// sub-tile as 2x2 MFMA blocks of 32x32; the workgroup tile is (64*WM) x
+// (64*WN).  Operands are staged global -> registers (FP32) -> split -> LDS as
+// k-contiguous rows of 8 BF16 values, so every MFMA operand is one 16-byte
+// LDS read; the next k-stage's global loads are issued before the current
+// stage's MFMAs.
+//
+// Layouts of v_mfma_f32_32x32x16_bf16 on gfx950 (measured): lane l holds
+// A[l%32][k] and B[k][l%32] for the eight k of half l/32 of the 16-deep step;
+// output element v of lane l is D[8*(v/4) + 4*(l/32) + v%4][l%32].
+#include "ygpu_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int NTHR = 256;
+constexpr int BK = 32;          // k per LDS stage: two 16-deep MFMA steps
+constexpr int KV = BK / 8;      // 8-value groups per row per stage
+constexpr int KVP = KV + 1;     // padded LDS row stride (16-byte units)
+
+__device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16) x[e];
+        hi[e] = h;
+        lo[e] = (__bf16) (x[e] - (float) h);
+    }
+}
+
+// acc[i][j] += A(64 rows of this wave) * B(64 cols of this wave) over k in [kbeg, kend).
+// Loader contract: coords(idx, row, kv) names the tile row and 8-group of the
+// idx-th (row, kv) slot (idx < tile rows * KV); load(row, k8, kend, v) fills the
+// 8 values k8 .. k8+7 of that row (zero beyond kend or outside the matrix).
+template <int WM, int WN, class LA, class LB>
+__device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2])
+{
+    constexpr int BMt = 64 * WM, BNt = 64 * WN;
+    constexpr int GA = BMt * KV / NTHR, GB = BNt * KV / NTHR;
+    static_assert(GA * NTHR == BMt * KV && GB * NTHR == BNt * KV, "tile shape");
+    __shared__ v8bf Ah[BMt][KVP], Al[BMt][KVP], Bh[BNt][KVP], Bl[BNt][KVP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w / WN) * 64, wn = (w % WN) * 64;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    int rowa[GA], kva[GA], rowb[GB], kvb[GB];
+#pragma unroll
+    for (int g = 0; g < GA; ++g) la.coords(threadIdx.x + NTHR * g, rowa[g], kva[g]);
+#pragma unroll
+    for (int g = 0; g < GB; ++g) lb.coords(threadIdx.x + NTHR * g, rowb[g], kvb[g]);
+    float ra[GA][8], rb[GB][8];
+    if (kbeg < kend) {
+#pragma unroll
+        for (int g = 0; g < GA; ++g) la.load(rowa[g], kbeg + 8 * kva[g], kend, ra[g]);
+#pragma unroll
+        for (int g = 0; g < GB; ++g) lb.load(rowb[g], kbeg + 8 * kvb[g], kend, rb[g]);
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < GA; ++g) split8(ra[g], Ah[rowa[g]][kva[g]], Al[rowa[g]][kva[g]]);
+#pragma unroll
+        for (int g = 0; g < GB; ++g) split8(rb[g], Bh[rowb[g]][kvb[g]], Bl[rowb[g]][kvb[g]]);
+        __syncthreads();
+        if (k0 + BK < kend) {
+#pragma unroll
+            for (int g = 0; g < GA; ++g) la.load(rowa[g], k0 + BK + 8 * kva[g], kend, ra[g]);
+#pragma unroll
+            for (int g = 0; g < GB; ++g) lb.load(rowb[g], k0 + BK + 8 * kvb[g], kend, rb[g]);
+        }
+#pragma unroll
+        for (int s = 0; s < BK / 16; ++s) {
+            const int kv = 2 * s + (lane >> 5), r = lane & 31;
+            v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ah[i] = Ah[wm + 32 * i + r][kv];
+                al[i] = Al[wm + 32 * i + r][kv];
+                bh[i] = Bh[wn + 32 * i + r][kv];
+                bl[i] = Bl[wn + 32 * i + r][kv];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+}
+
+template <int WN, class F>
+__device__ __forceinline__ void bf3_epilogue(const v16f (&acc)[2][2], F &&store)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w / WN) * 64, wn = (w % WN) * 64;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                store(wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), wn + 32 * j + (lane & 31), acc[i][j][v]);
+}
+
+// workgroup id -> logical tile id such that consecutive logical ids run on one
+// XCD (the hardware deals workgroups round-robin over the 8 XCDs), so tiles
+// that share an operand panel share that XCD's L2
+__device__ __forceinline__ long xcd_remap(long b, long total)
+{
+    const long per = total / 8, rem = total % 8, x = b % 8, slot = b / 8;
+    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// ---- X build: A(m,k) = Cocc[partner[k]][m], B(k,n) = B32[up[k]][n]
+struct XLoadA {
+    const double *cocc; const int *partner; int n_occ, m0;
+    __device__ void coords(int idx, int &row, int &kv) const { row = idx % 64; kv = idx / 64; }
+    __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
+        const int m = m0 + row;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = k8 + e;
+            v[e] = (k < kend && m < n_occ) ? (float) cocc[(size_t) partner[k] * n_occ + m] : 0.f;
+        }
+    }
+};
+
+template <int BNt>
+struct XLoadB {
+    const float *b32; const int *up; long ld; int n0;
+    __device__ void coords(int idx, int &row, int &kv) const { row = idx % BNt; kv = idx / BNt; }
+    __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
+        const long n = n0 + row;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = k8 + e;
+            v[e] = (k < kend && n < ld) ? b32[(size_t) up[k] * ld + n] : 0.f;
+        }
+    }
+};
+
+constexpr int XWN = 4;   // X build: 64 occupied rows x 256 aux columns per workgroup
+
+__global__ __launch_bounds__(NTHR) void k_xbuild_bf3(const int *__restrict__ xptr, const int *__restrict__ xup,
+                                                     const int *__restrict__ xpartner, const double *__restrict__ Cocc,
+                                                     const float *__restrict__ B32, float *__restrict__ X, int n_occ,
+                                                     int naux_pad, int ncolt, int nrowt, long total)
+{
+    const long t = xcd_remap(blockIdx.x, total);
+    const int rt = (int) (t % nrowt), ct = (int) ((t / nrowt) % ncolt), i = (int) (t / ((long) nrowt * ncolt));
+    const int kb = xptr[i], klen = xptr[i + 1] - kb;
+    const int m0 = rt * 64, n0 = ct * 64 * XWN;
+    XLoadA la{Cocc, xpartner + kb, n_occ, m0};
+    XLoadB<64 * XWN> lb{B32, xup + kb, naux_pad, n0};
+    v16f acc[2][2];
+    bf3_tile<1, XWN>(la, lb, 0, klen, acc);
+    float *Xi = X + (size_t) i * n_occ * naux_pad;
+    bf3_epilogue<XWN>(acc, [&](int r, int c, float v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = v;
+    });
+}
+
+// ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous
+struct SLoad {
+    const float *x; long kdim; int nbf, r0;
+    __device__ void coords(int idx, int &row, int &kv) const { kv = idx % KV; row = idx / KV; }
+    __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
+        const int r = r0 + row;
+        if (r < nbf && k8 < kend) {
+            const float4 *p = (const float4 *) (x + (size_t) r * kdim + k8);
+            const float4 a = p[0], b = p[1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = 0.f;
+        }
+    }
+};
+
+constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
+
+// upper 128x128 tiles of X X^T over k chunks; part[z] is FP32 per chunk
+__global__ __launch_bounds__(NTHR) void k_syrk_bf3(const float *__restrict__ X, int nbf, long kdim, int ntb,
+                                                   int ntiles, long kchunk, long total, float *__restrict__ part)
+{
+    const long t = xcd_remap(blockIdx.x, total);
+    int tile = (int) (t % ntiles);
+    const long z = t / ntiles;
+    int bi = 0;
+    while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
+    const int bj = bi + tile;
+    const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
+    SLoad la{X + kb, kdim, nbf, bi * ST};
+    SLoad lb{X + kb, kdim, nbf, bj * ST};
+    v16f acc[2][2];
+    bf3_tile<2, 2>(la, lb, 0, (int) (ke - kb), acc);
+    float *P = part + (size_t) z * nbf * nbf;
+    const int m0 = bi * ST, n0 = bj * ST;
+    bf3_epilogue<2>(acc, [&](int r, int c, float v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = v;
+    });
+}
+
+// K[m][n] = K[n][m] = sum_z part[z][m][n] (in FP64) for m <= n inside upper tiles
+__global__ void k_syrk_bf3_reduce(const float *__restrict__ part, int nsplit, int nbf, double *__restrict__ K)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) nbf * nbf) return;
+    const int m = (int) (idx / nbf), n = (int) (idx % nbf);
+    if (m > n) return;
+    double s0 = 0.0, s1 = 0.0;
+    int z = 0;
+    for (; z + 1 < nsplit; z += 2) {
+        s0 += (double) part[(size_t) z * nbf * nbf + idx];
+        s1 += (double) part[(size_t) (z + 1) * nbf * nbf + idx];
+    }
+    if (z < nsplit) s0 += (double) part[(size_t) z * nbf * nbf + idx];
+    K[(size_t) m * nbf + n] = s0 + s1;
+    K[(size_t) n * nbf + m] = s0 + s1;
+}
+
+__global__ void k_to_f32(const double *__restrict__ in, float *__restrict__ out, long n)
+{
+    for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x)
+        out[i] = (float) in[i];
+}
+
+}  // namespace
+
+// ===========================================================================
+int yg_to_f32(hipStream_t s, const double *in, float *out, long n)
+{
+    if (n <= 0) return YGPU_OK;
+    const long nb = std::min<long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, in, out, n);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                  const int *xpartner, const double *Cocc, const float *B32, float *X)
+{
+    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
+    const int ncolt = (naux_pad + 64 * XWN - 1) / (64 * XWN), nrowt = (n_occ + 63) / 64;
+    const long total = (long) ncolt * nrowt * nbf;
+    if (total > 0x7fffffffL) {
+        ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_xbuild_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B32, X,
+                       n_occ, naux_pad, ncolt, nrowt, total);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part)
+{
+    if (nbf <= 0) return YGPU_OK;
+    if (kdim % 8) {
+        ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
+        return YGPU_ERR_INVALID;
+    }
+    const int ntb = (nbf + ST - 1) / ST;
+    const int ntiles = ntb * (ntb + 1) / 2;
+    // chunks of 1024..8192 along k (FP32 accumulation inside a chunk, FP64 across
+    // chunks), enough of them for about 1024 workgroups
+    long kchunk = kdim * ntiles / 1024;
+    kchunk = std::max<long>(1024, std::min<long>(8192, kchunk));
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
+    YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
+    const long total = (long) ntiles * nsplit;
+    hipLaunchKernelGGL(k_syrk_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, X, nbf, kdim, ntb, ntiles, kchunk, total,
+                       part.as<float>());
+    YG_HIP(hipGetLastError());
+    const long tot = (long) nbf * nbf;
+    hipLaunchKernelGGL(k_syrk_bf3_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, K);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}

@@ -8,8 +8,10 @@
  * DIIS.c:117).  Iterations 0
  * and 1 only transform F (:30-40); the slot of the largest stored residual is
  * replaced once MAX_DIIS vectors exist, and that maximum is never reset
- * (:93-100), as in the reference.
+ * (:93-100), as in the reference, unless quirk_a2 is 0: then the maximum is
+ * taken over the stored residuals each iteration.
  */
+#include <float.h>
 #include <math.h>
 #include <string.h>
 
@@ -63,11 +65,13 @@ void TinySCF_DIIS(TinySCF_t T)
      * the stored residuals, and X^T F X stored as the slot's Fock matrix */
     double dot[MAX_DIIS + 1];
     YS_GPU(ygpu_diis_update(g, idx, T->DIIS_len, T->F_mat, T->D_mat, dot, T->tmp_mat));
+    YS_GPU(ygpu_diis_error(g, idx, &T->diis_error));
     memcpy(T->F_mat, T->tmp_mat, DBL_SIZE * ms);
     for (int i = 0; i < T->DIIS_len; i++) {
         T->B_mat[idx * ldB + i] = dot[i];
         T->B_mat[i * ldB + idx] = dot[i];
     }
+    if (!T->quirk_a2) T->DIIS_bmax = -DBL_MAX;   /* recomputed over the stored residuals */
     for (int i = 0; i < T->DIIS_len; i++)
         if (T->B_mat[i * ldB + i] > T->DIIS_bmax) {
             T->DIIS_bmax = T->B_mat[i * ldB + i];

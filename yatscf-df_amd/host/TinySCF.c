@@ -232,10 +232,16 @@ void TinySCF_do_SCF(TinySCF_t T)
     }
     T->iter = 0;
     T->converged = 0;
-    double prev = 0.0, delta = 223.0;
+    T->diis_error = -1.0;
+    T->mixed_phase = T->mixed;
+    T->mixed_switch_iter = -1;
+    double prev = 0.0, delta = 223.0, best_err = -1.0;
+    int stall = 0;
     while (T->iter < T->niters && delta >= T->ene_tol) {
         double st0 = get_wtime_sec();
         if (T->verbose) printf("--------------- Iteration %d ---------------\n", T->iter);
+        if (T->mixed)
+            YS_GPU(ygpu_set_fock_precision((ygpu_ctx_t *) T->gpu, T->mixed_phase ? YGPU_FOCK_MIXED : YGPU_FOCK_FP64));
         TinySCF_build_FockMat(T);
         TinySCF_calc_energy(T);
         delta = fabs(T->HF_energy - prev);
@@ -252,12 +258,35 @@ void TinySCF_do_SCF(TinySCF_t T)
             printf("* Build density matrix  : %.3lf (s)\n", st3 - st2);
             printf("* Iteration runtime     = %.3lf (s)\n", st3 - st0);
             printf("* Energy = %.10lf (%.10lf)", T->HF_energy + T->nuc_energy, T->HF_energy);
-            if (T->iter > 0) printf(", delta = %e\n", delta);
-            else printf("\n");
+            if (T->iter > 0) printf(", delta = %e", delta);
+            if (T->iter >= 2) printf(", DIIS error = %.3e", T->diis_error);
+            printf("%s\n", T->mixed_phase ? " [mixed]" : "");
         }
         if (T->iter == 0) delta = 223.0;   /* iteration 0 never converges (TinySCF.c:429) */
+        if (T->mixed_phase) {
+            /* a reduced-precision energy never ends the SCF; the phase ends once
+             * the DIIS error is below mixed_tol */
+            delta = 223.0;
+            if (T->iter >= 2 && T->diis_error >= 0.0) {
+                /* or when the error has stopped falling (the reduced-precision noise
+                 * floor lies above mixed_tol): no new best by 30 % in 3 iterations */
+                if (best_err < 0.0 || T->diis_error < 0.7 * best_err) {
+                    best_err = T->diis_error;
+                    stall = 0;
+                } else {
+                    stall++;
+                }
+                if (T->diis_error < T->mixed_tol || stall >= 3) {
+                    T->mixed_phase = 0;
+                    T->mixed_switch_iter = T->iter + 1;
+                }
+            }
+        } else if (T->iter == T->mixed_switch_iter) {
+            delta = 223.0;   /* the first FP64 energy is compared with a mixed one */
+        }
         T->iter++;
     }
+    if (T->mixed) YS_GPU(ygpu_set_fock_precision((ygpu_ctx_t *) T->gpu, YGPU_FOCK_FP64));
     T->energy_delta = delta;
     T->converged = delta < T->ene_tol;
     if (T->verbose) printf("--------------- SCF iterations finished ---------------\n");

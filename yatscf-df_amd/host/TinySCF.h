@@ -10,8 +10,9 @@
  *  - pqA and df_tensor are host copies made only when `host_df_copy` is set
  *    (the device keeps the packed tensor; the host copies are 2 x nnz x naux
  *    doubles in the reference layout);
- *  - quirk switches for the reference behaviours listed in DESIGN.md, and a
- *    convergence flag.
+ *  - switches for reference behaviours that a user may want to change (the
+ *    Schwarz element read, D2Cocc, the DIIS replacement rule), a convergence
+ *    flag and per-phase timers.
  */
 #ifndef _YATSCF_TFOCK_H_
 #define _YATSCF_TFOCK_H_
@@ -102,7 +103,14 @@ struct TinySCF_struct
     /* additions */
     int    quirk_a5;        /* reference Schwarz index (TinySCF.c:153), default 1 */
     int    quirk_a1;        /* reference D2Cocc (TinySCF.c:304-318), default 1 */
+    int    quirk_a2;        /* DIIS_bmax never reset (DIIS.c:93-100), default 1; 0 recomputes it
+                               over the stored residuals each iteration */
     int    host_df_copy;    /* keep host pqA / df_tensor copies, default 0 */
+    int    mixed;           /* 1: reduced-precision Fock builds until the DIIS error < mixed_tol */
+    double mixed_tol;       /* DIIS error (max |R|) that ends the mixed phase, default 1e-5 */
+    int    mixed_phase;     /* 1 while the Fock builds run in reduced precision */
+    int    mixed_switch_iter; /* first FP64 iteration of a mixed run (-1: never switched) */
+    double diis_error;      /* max |R| of the newest DIIS residual (iterations >= 2) */
     int    verbose;         /* reference-style printout, default 1 */
     int    converged;
     double energy_delta;

@@ -43,6 +43,16 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     memset(T, 0, sizeof(*T));
     T->quirk_a5 = env_int("YATSCF_QUIRK_A5", 1);
     T->quirk_a1 = env_int("YATSCF_QUIRK_A1", 1);
+    T->quirk_a2 = env_int("YATSCF_QUIRK_A2", 1);
+    T->mixed = env_int("YATSCF_MIXED", 0);
+    T->mixed_tol = 1e-5;
+    {
+        const char *e = getenv("YATSCF_MIXED_TOL");
+        if (e && *e && atof(e) > 0.0) T->mixed_tol = atof(e);
+    }
+    T->mixed_phase = 0;
+    T->mixed_switch_iter = -1;
+    T->diis_error = -1.0;
     T->host_df_copy = env_int("YATSCF_HOST_DF", 0);
     T->verbose = env_int("YATSCF_VERBOSE", 1);
 
@@ -94,6 +104,10 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->iter = 0;
     T->niters = niters;
     T->ene_tol = 1e-11;
+    {
+        const char *e = getenv("YATSCF_ENE_TOL");
+        if (e && *e && atof(e) > 0.0) T->ene_tol = atof(e);
+    }
     T->shell_scrtol2 = 1e-11 * 1e-11;
 
     const size_t ms = (size_t) T->mat_size;

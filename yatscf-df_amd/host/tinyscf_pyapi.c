@@ -101,6 +101,11 @@ int tinyscf_geti(TinySCF_t T, const char *n)
     if (!strcmp(n, "converged")) return T->converged;
     if (!strcmp(n, "DIIS_len")) return T->DIIS_len;
     if (!strcmp(n, "charge")) return T->charge;
+    if (!strcmp(n, "quirk_a1")) return T->quirk_a1;
+    if (!strcmp(n, "quirk_a2")) return T->quirk_a2;
+    if (!strcmp(n, "quirk_a5")) return T->quirk_a5;
+    if (!strcmp(n, "mixed")) return T->mixed;
+    if (!strcmp(n, "mixed_switch_iter")) return T->mixed_switch_iter;
     if (!strcmp(n, "electron")) return T->electron;
     return -1;
 }
@@ -110,6 +115,10 @@ void tinyscf_seti(TinySCF_t T, const char *n, int v)
     if (!strcmp(n, "iter")) T->iter = v;
     else if (!strcmp(n, "niters")) T->niters = v;
     else if (!strcmp(n, "verbose")) T->verbose = v;
+    else if (!strcmp(n, "quirk_a1")) T->quirk_a1 = v;
+    else if (!strcmp(n, "quirk_a2")) T->quirk_a2 = v;
+    else if (!strcmp(n, "quirk_a5")) T->quirk_a5 = v;
+    else if (!strcmp(n, "mixed")) T->mixed = v;
 }
 
 double tinyscf_getd(TinySCF_t T, const char *n)
@@ -121,12 +130,16 @@ double tinyscf_getd(TinySCF_t T, const char *n)
     if (!strcmp(n, "max_df_scrval")) return T->max_df_scrval;
     if (!strcmp(n, "t_density")) return T->t_density;
     if (!strcmp(n, "t_diis")) return T->t_diis;
+    if (!strcmp(n, "ene_tol")) return T->ene_tol;
+    if (!strcmp(n, "diis_error")) return T->diis_error;
+    if (!strcmp(n, "mixed_tol")) return T->mixed_tol;
     return 0.0;
 }
 
 void tinyscf_setd(TinySCF_t T, const char *n, double v)
 {
     if (!strcmp(n, "ene_tol")) T->ene_tol = v;
+    else if (!strcmp(n, "mixed_tol")) T->mixed_tol = v;
 }
 
 void *tinyscf_gpu(TinySCF_t T) { return T->gpu; }

@@ -67,7 +67,11 @@ SIGNATURES = {
     "ygpu_diis_transform": [vp, dp, dp],
     "ygpu_diis_update": [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp],
     "ygpu_diis_extrapolate": [vp, ctypes.c_int, dp, dp],
+    "ygpu_diis_error": [vp, ctypes.c_int, dp],
+    "ygpu_set_fock_precision": [vp, ctypes.c_int],
 }
+
+FOCK_FP64, FOCK_MIXED = 0, 1
 
 _glib = None
 _hlib = None
@@ -249,6 +253,10 @@ class Engine:
         check(self.L.ygpu_build_fock(self.h, n_occ, _d(D), _d(Cocc), _d(H), _d(J), _d(K), _d(F), ctypes.byref(t)))
         return dict(F=F, J=J, K=K, timing=t)
 
+    def set_fock_precision(self, mode):
+        """FOCK_FP64 (reference arithmetic) or FOCK_MIXED (BF16-split K, incremental FP32-tensor J)."""
+        check(self.L.ygpu_set_fock_precision(self.h, int(mode)))
+
     def build_density(self, Fo, X, n_occ):
         Fo = np.ascontiguousarray(Fo, dtype=np.float64)
         X = np.ascontiguousarray(X, dtype=np.float64)
@@ -290,6 +298,11 @@ class Engine:
         out = np.zeros_like(F)
         check(self.L.ygpu_diis_update(self.h, slot, length, _d(F), _d(D), _d(dots), _d(out)))
         return dots, out
+
+    def diis_error(self, slot):
+        v = ctypes.c_double(0.0)
+        check(self.L.ygpu_diis_error(self.h, slot, ctypes.byref(v)))
+        return v.value
 
     def diis_extrapolate(self, coef):
         coef = np.ascontiguousarray(coef, dtype=np.float64)
