@@ -67,7 +67,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_scf(xyz, niters, settings, mixed=False):
+def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False):
     import ygpu
     T = ygpu.TinySCF(xyz, niters=niters)        # parse + upload shells (untimed)
     T.setd("ene_tol", settings["ene_tol"])
@@ -88,6 +88,18 @@ def run_gpu_scf(xyz, niters, settings, mixed=False):
     mask = T.i("bf_pair_mask").reshape(res["nbf"], res["nbf"])
     iu = np.triu_indices(res["nbf"])
     res["n_up"] = int(np.sum(mask[iu] >= 0))
+    if fock_check:
+        # one FP64 and one mixed Fock build on the converged density (untimed): the
+        # arithmetic's own error in F and in the orbital energies of X^T F X
+        X = T.mat("X_mat").copy()
+        F64 = T.build_fock(ygpu.FOCK_FP64)
+        Fmx = T.build_fock(ygpu.FOCK_MIXED)
+        e64 = np.linalg.eigvalsh(X.T @ F64 @ X)
+        emx = np.linalg.eigvalsh(X.T @ Fmx @ X)
+        no = res["n_occ"]
+        res["fock_check"] = {"max_abs_dF": float(np.max(np.abs(Fmx - F64))),
+                             "max_abs_d_orbital_energy": float(np.max(np.abs(emx - e64))),
+                             "max_abs_d_occupied_orbital_energy": float(np.max(np.abs(emx[:no] - e64[:no])))}
     T.close()
     return res
 
@@ -135,7 +147,9 @@ def main():
         log("warmup %d: %.3f s, %d iterations, E = %.10f" % (w, r["seconds"], r["iters"], r["energy"]))
     runs = []
     for k in range(args.steps):
-        r = run_gpu_scf(xyz, args.niters, settings, mixed)
+        # the FP64 Fock check runs after the timer of the last step stops
+        check = (not mixed) and (not args.no_mixed_compare) and k == args.steps - 1
+        r = run_gpu_scf(xyz, args.niters, settings, mixed, fock_check=check)
         runs.append(r)
         log("step %d: %.3f s, %d iterations, E = %.10f%s" % (k, r["seconds"], r["iters"], r["energy"],
             (", FP64 from iteration %d" % r["switch"]) if mixed else ""))
@@ -205,15 +219,16 @@ def main():
         out["mixed_switch_iteration"] = last["switch"]
     if not args.no_mixed_compare:
         # the other arithmetic on the same workload, for the mixed-vs-FP64 comparison
-        other = run_gpu_scf(xyz, args.niters, settings, not mixed)
+        other = run_gpu_scf(xyz, args.niters, settings, not mixed, fock_check=mixed)
         fp, mx = (other, last) if mixed else (last, other)
+        if "fock_check" not in fp:
+            fp = run_gpu_scf(xyz, args.niters, settings, False, fock_check=True)
         mh = rows(mx, True, True)
         out["mixed_vs_fp64"] = {
             "fp64_s": fp["seconds"], "mixed_s": mx["seconds"], "speedup": fp["seconds"] / mx["seconds"],
             "fp64_iters": fp["iters"], "mixed_iters": mx["iters"], "mixed_switch_iteration": mx["switch"],
             "dE_Eh": mx["energy"] - fp["energy"],
-            "max_abs_dF": float(np.max(np.abs(mx["F"] - fp["F"]))),
-            "max_abs_d_orbital_energy": float(np.max(np.abs(mx["eigval"] - fp["eigval"]))),
+            "fock_at_converged_density": fp["fock_check"],
             "mixed_k_ms_per_iter": float(np.median(mh[:, 2] + mh[:, 3]) * 1e3) if len(mh) else None,
             "mixed_j_ms_per_iter": float(np.median(mh[:, 0] + mh[:, 1]) * 1e3) if len(mh) else None,
             "mixed_xbuild_tflops": (xb_flops / (float(np.median(mh[:, 2])) * 1e12)) if len(mh) else None,

@@ -174,6 +174,30 @@ int ygpu_dgemm(ygpu_ctx_t *ctx, int transa, int transb, int m, int n, int k, dou
  * rows of the reference layout (both triangles) and the padded aux width. */
 int ygpu_layout_info(ygpu_ctx_t *ctx, long *n_up, long *nnz, int *naux_pad);
 
+/* ---- learned initial guess: the pair-correction network (no counterpart in
+ * the reference, whose guess is SAD only, TinySCF.c:324-364).  FP32 MLP
+ * n_in -> nh -> nh -> nh -> n_out with SiLU after the hidden layers; params
+ * packed as W1 [n_in][nh], b1 [nh], W2 [nh][nh], b2, W3 [nh][nh], b3,
+ * W4 [nh][n_out], b4 (row-major, y = x W + b).  Host pointers throughout. */
+
+/* Y [n][n_out] = network(X [n][n_in]) */
+int ygpu_mlp_forward(ygpu_ctx_t *ctx, int n_in, int nh, int n_out, const float *params, int n,
+                     const float *X, float *Y);
+
+/* Device-resident trainer: the samples X [n][n_in], targets Y [n][n_out] and
+ * 0/1 masks M [n][n_out] are uploaded once; the loss of a batch of B samples
+ * is sum M (net(X) - Y)^2 / (B n_out); Adam (0.9, 0.999, 1e-8). */
+typedef struct ygpu_mlp_trainer ygpu_mlp_trainer_t;
+int ygpu_mlp_trainer_create(ygpu_ctx_t *ctx, int n_in, int nh, int n_out, int n_samples, const float *X,
+                            const float *Y, const float *M, const float *params_init,
+                            ygpu_mlp_trainer_t **trainer);
+/* one pass over the samples in batches, in the order of a shuffle seeded by seed */
+int ygpu_mlp_trainer_epoch(ygpu_mlp_trainer_t *trainer, int batch, float lr, unsigned seed, double *mean_loss);
+/* loss and gradient (packed like params) on the samples idx[0..n) */
+int ygpu_mlp_trainer_grad(ygpu_mlp_trainer_t *trainer, int n, const int *idx, float *grad, double *loss);
+int ygpu_mlp_trainer_params(ygpu_mlp_trainer_t *trainer, float *params);
+int ygpu_mlp_trainer_destroy(ygpu_mlp_trainer_t *trainer);
+
 #ifdef __cplusplus
 }
 #endif

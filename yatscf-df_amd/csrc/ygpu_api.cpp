@@ -628,8 +628,10 @@ try {
         }
         YG_HIP(hipEventRecord(c->ev[3], c->stream));
         // K from BF16-split products; X in FP32 occupies the front half of the FP64 X buffer
+        YG_TRY(c->d_Cocc32.ensure((size_t) nbf * n_occ * sizeof(float)));
+        YG_TRY(yg_to_f32(c->stream, c->d_Cocc.as<double>(), c->d_Cocc32.as<float>(), (long) nbf * n_occ));
         YG_TRY(yg_xbuild_bf3(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
-                             c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B32.as<float>(),
+                             c->d_xpartner.as<int>(), c->d_Cocc32.as<float>(), c->d_B32.as<float>(),
                              c->d_X.as<float>()));
         YG_HIP(hipEventRecord(c->ev[4], c->stream));
         YG_TRY(yg_syrk_bf3(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(), c->d_K.as<double>(),

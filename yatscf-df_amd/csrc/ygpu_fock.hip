@@ -18,53 +18,87 @@
 
 namespace {
 
-constexpr int JTILE = 64;      // columns per pass-1 workgroup (one per lane)
 constexpr int JWAVES = 4;      // wavefronts per pass-1 workgroup, each on its own rows
 
-// pass 1: partial[g][c] = sum over the rows of row group g of w_r B[r][c]; grid
-// (column tiles, row groups); rows are interleaved over (group, wavefront) so
-// every wavefront streams whole 512-byte row segments, and the row index is
-// wave-uniform (scalar loads of the pair indices and of D).
+// columns per lane: every lane moves 8 bytes per row (one FP64 or two FP32 values)
+template <class TB> constexpr int cpl() { return (int) (8 / sizeof(TB)); }
+
 template <class TB>
-__global__ __launch_bounds__(JTILE * JWAVES) void k_tempj_part(const TB *__restrict__ B, long ld, long n_up,
-                                                               const int *__restrict__ up_i, const int *__restrict__ up_j,
-                                                               const double *__restrict__ D, int nbf,
-                                                               double *__restrict__ part)
+__device__ __forceinline__ void load_cols(const TB *p, double (&v)[cpl<TB>()])
 {
-    __shared__ double red[JWAVES][JTILE];
-    const int lane = threadIdx.x & (JTILE - 1), w = threadIdx.x / JTILE;
-    const long col = (long) blockIdx.x * JTILE + lane;
+    if constexpr (cpl<TB>() == 1) {
+        v[0] = (double) p[0];
+    } else {
+        const float2 f = *(const float2 *) p;
+        v[0] = f.x;
+        v[1] = f.y;
+    }
+}
+
+// pass 1: partial[g][c] = sum over the rows of row group g of w_r B[r][c]; grid
+// (column tiles of 64 lanes x cpl columns, row groups); rows are interleaved over
+// (group, wavefront) so every wavefront streams whole 512-byte row segments, and
+// the row index is wave-uniform (scalar loads of the pair indices and of D).
+// ld is a multiple of 64, so a lane's cpl columns are all inside or all outside.
+template <class TB>
+__global__ __launch_bounds__(64 * JWAVES) void k_tempj_part(const TB *__restrict__ B, long ld, long n_up,
+                                                            const int *__restrict__ up_i, const int *__restrict__ up_j,
+                                                            const double *__restrict__ D, int nbf,
+                                                            double *__restrict__ part)
+{
+    constexpr int C = cpl<TB>();
+    __shared__ double red[JWAVES][64 * C];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long col = ((long) blockIdx.x * 64 + lane) * C;
     const long stride = (long) gridDim.y * JWAVES;
     const long r0 = (long) blockIdx.y * JWAVES + w;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double a[4][C];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[q][c] = 0.0;
     const bool cin = col < ld;
     long r = r0;
     for (; r + 3 * stride < n_up; r += 4 * stride) {
-        double wt[4], bv[4];
+        double wt[4], bv[4][C];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const long rq = __builtin_amdgcn_readfirstlane((int) (r + q * stride));
             const int i = up_i[rq], j = up_j[rq];
             wt[q] = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
-            bv[q] = cin ? (double) B[(size_t) rq * ld + col] : 0.0;
+            if (cin) load_cols<TB>(B + (size_t) rq * ld + col, bv[q]);
+            else
+#pragma unroll
+                for (int c = 0; c < C; ++c) bv[q][c] = 0.0;
         }
-        a0 += wt[0] * bv[0];
-        a1 += wt[1] * bv[1];
-        a2 += wt[2] * bv[2];
-        a3 += wt[3] * bv[3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < C; ++c) a[q][c] += wt[q] * bv[q][c];
     }
     for (; r < n_up; r += stride) {
         const long rq = __builtin_amdgcn_readfirstlane((int) r);
         const int i = up_i[rq], j = up_j[rq];
-        a0 += (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j] * (cin ? (double) B[(size_t) rq * ld + col] : 0.0);
+        const double wt = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
+        double bv[C];
+        if (cin) load_cols<TB>(B + (size_t) rq * ld + col, bv);
+        else
+#pragma unroll
+            for (int c = 0; c < C; ++c) bv[c] = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[0][c] += wt * bv[c];
     }
-    red[w][lane] = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[w][lane * C + c] = (a[0][c] + a[1][c]) + (a[2][c] + a[3][c]);
     __syncthreads();
     if (w == 0 && cin) {
-        double s = red[0][lane];
 #pragma unroll
-        for (int q = 1; q < JWAVES; ++q) s += red[q][lane];
-        part[(size_t) blockIdx.y * ld + col] = s;
+        for (int c = 0; c < C; ++c) {
+            double s = red[0][lane * C + c];
+#pragma unroll
+            for (int q = 1; q < JWAVES; ++q) s += red[q][lane * C + c];
+            part[(size_t) blockIdx.y * ld + col + c] = s;
+        }
     }
 }
 
@@ -96,9 +130,15 @@ __global__ __launch_bounds__(256) void k_jmat(const TB *__restrict__ B, long ld,
     for (int q = 0; q < rows_per_wave; ++q) {
         const long r = rbase + q;
         if (r >= n_up) break;
+        constexpr int C = cpl<TB>();
         const TB *row = B + (size_t) r * ld;
         double s = 0.0;
-        for (long c = lane; c < ld; c += 64) s += (double) row[c] * tj[c];
+        for (long c = lane * C; c < ld; c += 64 * C) {
+            double v[C];
+            load_cols<TB>(row + c, v);
+#pragma unroll
+            for (int e = 0; e < C; ++e) s += v[e] * tj[c + e];
+        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) J[(size_t) up_i[r] * nbf + up_j[r]] = s;
@@ -286,7 +326,8 @@ int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
-    const int ncolt = (int) ((ld + JTILE - 1) / JTILE);
+    const int C = B32 ? cpl<float>() : cpl<double>();
+    const int ncolt = (int) ((ld + 64 * C - 1) / (64 * C));
     // row groups: about 2048 workgroups in total, at least 16 rows per wavefront
     long ngroup = std::max<long>(1, std::min<long>(2048 / ncolt, n_up / (16L * JWAVES)));
     ngroup = std::min<long>(ngroup, 65535);
@@ -294,10 +335,10 @@ int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float
     YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
     if (B32)
-        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream, B32, ld, n_up,
+        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, c->stream, B32, ld, n_up,
                            c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
     else
-        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(JTILE * JWAVES), 0, c->stream,
+        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, c->stream,
                            c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf,
                            c->d_tempJpart.as<double>());
     YG_HIP(hipGetLastError());

@@ -30,10 +30,14 @@ void ygpu_set_error(int code, const char *fmt, ...);
         if (s_ != YGPU_OK) return s_; \
     } while (0)
 
-// growable device buffer
+// growable device buffer, freed when released or destroyed (not copyable)
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
     int ensure(size_t n);
     void release();
     template <class T> T *as() const { return (T *) p; }
@@ -95,7 +99,7 @@ struct ygpu_ctx {
     int fock_mode = YGPU_FOCK_FP64;
     int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
     int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
-    DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32;
+    DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
 };
 
 // --------------------------------------------------------------------------
@@ -121,7 +125,7 @@ int yg_maxabs(hipStream_t s, long n, const double *x, double *out);             
 // mixed precision (BF16-split MFMA, FP32 accumulation)
 int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
 int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                  const int *xpartner, const double *Cocc, const float *B32, float *X);
+                  const int *xpartner, const float *Cocc32, const float *B32, float *X);
 int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part);
 int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const double *K, double *F);
 int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);

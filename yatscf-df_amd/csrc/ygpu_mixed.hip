@@ -10,12 +10,13 @@
 // keeps ~16 significant bits per operand at three BF16 MFMAs per product,
 // against one FP32 MFMA at a sixteenth of the BF16 rate.
 //
-// Tile engine: 256 threads = 4 wavefronts, each owning a 64x64 output
+// Tile engine: WM x WN wavefronts (up to 16), each owning a 64x64 output
 // sub-tile as 2x2 MFMA blocks of 32x32; the workgroup tile is (64*WM) x
 // (64*WN).  Operands are staged global -> registers (FP32) -> split -> LDS as
 // k-contiguous rows of 8 BF16 values, so every MFMA operand is one 16-byte
 // LDS read; the next k-stage's global loads are issued before the current
-// stage's MFMAs.
+// stage's MFMAs.  The X build takes all occupied rows (up to 256) in one
+// workgroup so every B tile is read once per basis function.
 //
 // Layouts of v_mfma_f32_32x32x16_bf16 on gfx950 (measured): lane l holds
 // A[l%32][k] and B[k][l%32] for the eight k of half l/32 of the 16-deep step;
@@ -29,7 +30,7 @@ namespace {
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int NTHR = 256;
+constexpr int NTHR = 256;     // SYRK workgroup (2 x 2 wavefronts)
 constexpr int BK = 32;          // k per LDS stage: two 16-deep MFMA steps
 constexpr int KV = BK / 8;      // 8-value groups per row per stage
 constexpr int KVP = KV + 1;     // padded LDS row stride (16-byte units)
@@ -51,9 +52,10 @@ __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 template <int WM, int WN, class LA, class LB>
 __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2])
 {
+    constexpr int NT = 64 * WM * WN;
     constexpr int BMt = 64 * WM, BNt = 64 * WN;
-    constexpr int GA = BMt * KV / NTHR, GB = BNt * KV / NTHR;
-    static_assert(GA * NTHR == BMt * KV && GB * NTHR == BNt * KV, "tile shape");
+    constexpr int SA = BMt * KV, SB = BNt * KV;                    // (row, 8-group) slots per stage
+    constexpr int GA = (SA + NT - 1) / NT, GB = (SB + NT - 1) / NT;
     __shared__ v8bf Ah[BMt][KVP], Al[BMt][KVP], Bh[BNt][KVP], Bl[BNt][KVP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w / WN) * 64, wn = (w % WN) * 64;
@@ -63,30 +65,46 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    // slot g of this thread is idx = threadIdx.x + NT g; slots beyond SA / SB are idle
     int rowa[GA], kva[GA], rowb[GB], kvb[GB];
+    bool ina[GA], inb[GB];
 #pragma unroll
-    for (int g = 0; g < GA; ++g) la.coords(threadIdx.x + NTHR * g, rowa[g], kva[g]);
+    for (int g = 0; g < GA; ++g) {
+        const int idx = threadIdx.x + NT * g;
+        ina[g] = idx < SA;
+        la.coords(ina[g] ? idx : 0, rowa[g], kva[g]);
+    }
 #pragma unroll
-    for (int g = 0; g < GB; ++g) lb.coords(threadIdx.x + NTHR * g, rowb[g], kvb[g]);
+    for (int g = 0; g < GB; ++g) {
+        const int idx = threadIdx.x + NT * g;
+        inb[g] = idx < SB;
+        lb.coords(inb[g] ? idx : 0, rowb[g], kvb[g]);
+    }
     float ra[GA][8], rb[GB][8];
     if (kbeg < kend) {
 #pragma unroll
-        for (int g = 0; g < GA; ++g) la.load(rowa[g], kbeg + 8 * kva[g], kend, ra[g]);
+        for (int g = 0; g < GA; ++g)
+            if (ina[g]) la.load(rowa[g], kbeg + 8 * kva[g], kend, ra[g]);
 #pragma unroll
-        for (int g = 0; g < GB; ++g) lb.load(rowb[g], kbeg + 8 * kvb[g], kend, rb[g]);
+        for (int g = 0; g < GB; ++g)
+            if (inb[g]) lb.load(rowb[g], kbeg + 8 * kvb[g], kend, rb[g]);
     }
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         __syncthreads();
 #pragma unroll
-        for (int g = 0; g < GA; ++g) split8(ra[g], Ah[rowa[g]][kva[g]], Al[rowa[g]][kva[g]]);
+        for (int g = 0; g < GA; ++g)
+            if (ina[g]) split8(ra[g], Ah[rowa[g]][kva[g]], Al[rowa[g]][kva[g]]);
 #pragma unroll
-        for (int g = 0; g < GB; ++g) split8(rb[g], Bh[rowb[g]][kvb[g]], Bl[rowb[g]][kvb[g]]);
+        for (int g = 0; g < GB; ++g)
+            if (inb[g]) split8(rb[g], Bh[rowb[g]][kvb[g]], Bl[rowb[g]][kvb[g]]);
         __syncthreads();
         if (k0 + BK < kend) {
 #pragma unroll
-            for (int g = 0; g < GA; ++g) la.load(rowa[g], k0 + BK + 8 * kva[g], kend, ra[g]);
+            for (int g = 0; g < GA; ++g)
+                if (ina[g]) la.load(rowa[g], k0 + BK + 8 * kva[g], kend, ra[g]);
 #pragma unroll
-            for (int g = 0; g < GB; ++g) lb.load(rowb[g], k0 + BK + 8 * kvb[g], kend, rb[g]);
+            for (int g = 0; g < GB; ++g)
+                if (inb[g]) lb.load(rowb[g], k0 + BK + 8 * kvb[g], kend, rb[g]);
         }
 #pragma unroll
         for (int s = 0; s < BK / 16; ++s) {
@@ -134,16 +152,17 @@ __device__ __forceinline__ long xcd_remap(long b, long total)
     return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
-// ---- X build: A(m,k) = Cocc[partner[k]][m], B(k,n) = B32[up[k]][n]
+// ---- X build: A(m,k) = Cocc32[partner[k]][m], B(k,n) = B32[up[k]][n]
+template <int BMt>
 struct XLoadA {
-    const double *cocc; const int *partner; int n_occ, m0;
-    __device__ void coords(int idx, int &row, int &kv) const { row = idx % 64; kv = idx / 64; }
+    const float *cocc; const int *partner; int n_occ, m0;
+    __device__ void coords(int idx, int &row, int &kv) const { row = idx % BMt; kv = idx / BMt; }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
         const int m = m0 + row;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int k = k8 + e;
-            v[e] = (k < kend && m < n_occ) ? (float) cocc[(size_t) partner[k] * n_occ + m] : 0.f;
+            v[e] = (k < kend && m < n_occ) ? cocc[(size_t) partner[k] * n_occ + m] : 0.f;
         }
     }
 };
@@ -162,23 +181,25 @@ struct XLoadB {
     }
 };
 
-constexpr int XWN = 4;   // X build: 64 occupied rows x 256 aux columns per workgroup
-
-__global__ __launch_bounds__(NTHR) void k_xbuild_bf3(const int *__restrict__ xptr, const int *__restrict__ xup,
-                                                     const int *__restrict__ xpartner, const double *__restrict__ Cocc,
-                                                     const float *__restrict__ B32, float *__restrict__ X, int n_occ,
-                                                     int naux_pad, int ncolt, int nrowt, long total)
+// X build: (64 WM occupied rows) x (64 WN aux columns) per workgroup of WM x WN wavefronts
+template <int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restrict__ xptr,
+                                                             const int *__restrict__ xup,
+                                                             const int *__restrict__ xpartner,
+                                                             const float *__restrict__ Cocc32,
+                                                             const float *__restrict__ B32, float *__restrict__ X,
+                                                             int n_occ, int naux_pad, int ncolt, int nrowt, long total)
 {
     const long t = xcd_remap(blockIdx.x, total);
     const int rt = (int) (t % nrowt), ct = (int) ((t / nrowt) % ncolt), i = (int) (t / ((long) nrowt * ncolt));
     const int kb = xptr[i], klen = xptr[i + 1] - kb;
-    const int m0 = rt * 64, n0 = ct * 64 * XWN;
-    XLoadA la{Cocc, xpartner + kb, n_occ, m0};
-    XLoadB<64 * XWN> lb{B32, xup + kb, naux_pad, n0};
+    const int m0 = rt * 64 * WM, n0 = ct * 64 * WN;
+    XLoadA<64 * WM> la{Cocc32, xpartner + kb, n_occ, m0};
+    XLoadB<64 * WN> lb{B32, xup + kb, naux_pad, n0};
     v16f acc[2][2];
-    bf3_tile<1, XWN>(la, lb, 0, klen, acc);
+    bf3_tile<WM, WN>(la, lb, 0, klen, acc);
     float *Xi = X + (size_t) i * n_occ * naux_pad;
-    bf3_epilogue<XWN>(acc, [&](int r, int c, float v) {
+    bf3_epilogue<WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
         if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = v;
     });
@@ -263,20 +284,31 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n)
     return YGPU_OK;
 }
 
-int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                  const int *xpartner, const double *Cocc, const float *B32, float *X)
+template <int WM, int WN>
+static int launch_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                             const int *xpartner, const float *Cocc32, const float *B32, float *X)
 {
-    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
-    const int ncolt = (naux_pad + 64 * XWN - 1) / (64 * XWN), nrowt = (n_occ + 63) / 64;
+    const int ncolt = (naux_pad + 64 * WN - 1) / (64 * WN), nrowt = (n_occ + 64 * WM - 1) / (64 * WM);
     const long total = (long) ncolt * nrowt * nbf;
     if (total > 0x7fffffffL) {
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_xbuild_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B32, X,
-                       n_occ, naux_pad, ncolt, nrowt, total);
+    hipLaunchKernelGGL((k_xbuild_bf3<WM, WN>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, xup, xpartner,
+                       Cocc32, B32, X, n_occ, naux_pad, ncolt, nrowt, total);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
+}
+
+int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                  const int *xpartner, const float *Cocc32, const float *B32, float *X)
+{
+    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
+    // all occupied rows in one workgroup up to 256 (row tiles of 256 beyond)
+    if (n_occ <= 64) return launch_xbuild_bf3<1, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (n_occ <= 128) return launch_xbuild_bf3<2, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (n_occ <= 192) return launch_xbuild_bf3<3, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    return launch_xbuild_bf3<4, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
 }
 
 int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part)

@@ -4,6 +4,7 @@
  * the screening logic, the pair layout, the SAD guess, the energy and the
  * iteration loop stay on the host as in the reference.
  */
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -13,6 +14,7 @@
 #include "TinySCF.h"
 #include "build_Fock.h"
 #include "build_density.h"
+#include "guess_ml.h"
 #include "utils.h"
 #include "yatscf_gpu.h"
 
@@ -215,6 +217,15 @@ void TinySCF_get_initial_guess(TinySCF_t T)
     TinySCF_D2Cocc(T);
     double t2 = get_wtime_sec();
     if (T->verbose) printf("TinySCF SAD initial guess: D mat / Cocc mat used %lf, %lf (s)\n", t1 - t0, t2 - t1);
+    T->guess_ml = 0;
+    if (T->guess_weights[0]) {
+        /* learned pair correction; on any failure the SAD guess above stays */
+        const int st = TinySCF_learned_guess(T, T->guess_weights);
+        T->guess_ml = (st == 0);
+        if (T->verbose)
+            printf("TinySCF learned guess correction: %s (%lf s)\n", st == 0 ? "applied" : "not applied, SAD kept",
+                   get_wtime_sec() - t2);
+    }
 }
 
 static void TinySCF_calc_energy(TinySCF_t T)
@@ -279,6 +290,12 @@ void TinySCF_do_SCF(TinySCF_t T)
                 if (T->diis_error < T->mixed_tol || stall >= 3) {
                     T->mixed_phase = 0;
                     T->mixed_switch_iter = T->iter + 1;
+                    /* the stored Fock matrices carry the reduced-precision error: the FP64
+                     * phase restarts the DIIS history from its own Fock matrices */
+                    T->DIIS_len = 0;
+                    T->DIIS_bmax_id = 0;
+                    T->DIIS_bmax = -DBL_MAX;
+                    memset(T->B_mat, 0, DBL_SIZE * (MAX_DIIS + 1) * (MAX_DIIS + 1));
                 }
             }
         } else if (T->iter == T->mixed_switch_iter) {

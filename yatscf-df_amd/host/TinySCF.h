@@ -111,6 +111,8 @@ struct TinySCF_struct
     int    mixed_phase;     /* 1 while the Fock builds run in reduced precision */
     int    mixed_switch_iter; /* first FP64 iteration of a mixed run (-1: never switched) */
     double diis_error;      /* max |R| of the newest DIIS residual (iterations >= 2) */
+    char   guess_weights[512]; /* learned-guess network file ("" = SAD guess only) */
+    int    guess_ml;        /* 1 if the last initial guess used the learned correction */
     int    verbose;         /* reference-style printout, default 1 */
     int    converged;
     double energy_delta;

@@ -53,6 +53,12 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->mixed_phase = 0;
     T->mixed_switch_iter = -1;
     T->diis_error = -1.0;
+    T->guess_ml = 0;
+    T->guess_weights[0] = 0;
+    {
+        const char *w = getenv("YATSCF_GUESS_WEIGHTS");
+        if (w && *w) snprintf(T->guess_weights, sizeof(T->guess_weights), "%s", w);
+    }
     T->host_df_copy = env_int("YATSCF_HOST_DF", 0);
     T->verbose = env_int("YATSCF_VERBOSE", 1);
 

@@ -4,6 +4,7 @@
  * Not part of the reference API.
  */
 #include <setjmp.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -12,6 +13,7 @@
 #include "build_DF_tensor.h"
 #include "build_Fock.h"
 #include "build_density.h"
+#include "guess_ml.h"
 #include "utils.h"
 
 /* Runs one TinySCF API function by name; a failing device call returns its
@@ -106,6 +108,7 @@ int tinyscf_geti(TinySCF_t T, const char *n)
     if (!strcmp(n, "quirk_a5")) return T->quirk_a5;
     if (!strcmp(n, "mixed")) return T->mixed;
     if (!strcmp(n, "mixed_switch_iter")) return T->mixed_switch_iter;
+    if (!strcmp(n, "guess_ml")) return T->guess_ml;
     if (!strcmp(n, "electron")) return T->electron;
     return -1;
 }
@@ -143,3 +146,17 @@ void tinyscf_setd(TinySCF_t T, const char *n, double v)
 }
 
 void *tinyscf_gpu(TinySCF_t T) { return T->gpu; }
+
+/* the engine's molecule for the pair model; ss receives natoms + 1 shell starts */
+void tinyscf_fill_mol(TinySCF_t T, ysml_mol_t *M, int *ss)
+{
+    ysml_mol_of(T, M);
+    for (int a = 0; a < T->natoms; a++) ss[a] = CMS_getAtomStartInd(T->basis, a);
+    ss[T->natoms] = T->nshells;
+    M->atom_shell_start = ss;
+}
+
+void tinyscf_set_guess_weights(TinySCF_t T, const char *path)
+{
+    snprintf(T->guess_weights, sizeof(T->guess_weights), "%s", path ? path : "");
+}

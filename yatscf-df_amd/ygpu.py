@@ -69,9 +69,31 @@ SIGNATURES = {
     "ygpu_diis_extrapolate": [vp, ctypes.c_int, dp, dp],
     "ygpu_diis_error": [vp, ctypes.c_int, dp],
     "ygpu_set_fock_precision": [vp, ctypes.c_int],
+    "ygpu_mlp_forward": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "ygpu_mlp_trainer_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(vp)],
+    "ygpu_mlp_trainer_epoch": [vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint, dp],
+    "ygpu_mlp_trainer_grad": [vp, ctypes.c_int, ip, ctypes.POINTER(ctypes.c_float), dp],
+    "ygpu_mlp_trainer_params": [vp, ctypes.POINTER(ctypes.c_float)],
+    "ygpu_mlp_trainer_destroy": [vp],
 }
 
 FOCK_FP64, FOCK_MIXED = 0, 1
+
+fp = ctypes.POINTER(ctypes.c_float)
+
+
+class YsmlMol(ctypes.Structure):
+    """ysml_mol_t of host/guess_ml.h: the molecule as the pair model sees it."""
+    _fields_ = [("natoms", ctypes.c_int), ("nbf", ctypes.c_int), ("Z", dp), ("xyz", dp),
+                ("atom_shell_start", ip), ("am", ip), ("shell_bf_start", ip)]
+
+
+# the pair model's dimensions (host/guess_ml.h)
+YSML_NCEN, YSML_NEL, YSML_NTYPES, YSML_NOUT = 32, 2, 5, 29
+YSML_NIN = 2 * YSML_NEL * YSML_NCEN + YSML_NCEN + YSML_NTYPES
 
 _glib = None
 _hlib = None
@@ -118,6 +140,17 @@ def host_lib():
         L.tinyscf_error.argtypes = []
         L.tinyscf_error.restype = ctypes.c_char_p
         L.tinyscf_gpu.argtypes = [vp]
+        L.tinyscf_fill_mol.argtypes = [vp, ctypes.POINTER(YsmlMol), ip]
+        L.tinyscf_fill_mol.restype = None
+        L.tinyscf_set_guess_weights.argtypes = [vp, ctypes.c_char_p]
+        L.tinyscf_set_guess_weights.restype = None
+        L.ysml_supported.argtypes = [ctypes.POINTER(YsmlMol)]
+        L.ysml_pairs.argtypes = [ctypes.POINTER(YsmlMol), ip, ctypes.c_int]
+        L.ysml_features.argtypes = [ctypes.POINTER(YsmlMol), ctypes.c_int, ip, fp]
+        L.ysml_targets.argtypes = [ctypes.POINTER(YsmlMol), ctypes.c_int, ip, dp, fp, fp]
+        L.ysml_assemble.argtypes = [ctypes.POINTER(YsmlMol), ctypes.c_int, ip, fp, dp]
+        for fn in ("ysml_supported", "ysml_pairs", "ysml_features", "ysml_targets", "ysml_assemble"):
+            getattr(L, fn).restype = ctypes.c_int
         L.tinyscf_gpu.restype = vp
         L.TinySCF_try_init.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.TinySCF_try_init.restype = ctypes.c_int
@@ -344,6 +377,18 @@ class TinySCF:
     def do_scf(self):
         self.step("TinySCF_do_SCF")
 
+    def build_fock(self, mode=FOCK_FP64):
+        """One Fock build from the engine's current D and C_occ in the given arithmetic
+        (FOCK_FP64 or FOCK_MIXED); the engine is left in FP64 mode.  Returns F (AO basis)."""
+        g = self.L.tinyscf_gpu(self.T)
+        L = gpu_lib()
+        check(L.ygpu_set_fock_precision(g, int(mode)))
+        try:
+            self.step("TinySCF_build_FockMat")
+        finally:
+            check(L.ygpu_set_fock_precision(g, FOCK_FP64))
+        return self.mat("F_mat").copy()
+
     def close(self):
         if getattr(self, "T", None):
             self.L.TinySCF_free_batch_dgemm_arrays(self.T)
@@ -386,3 +431,122 @@ class TinySCF:
     def mat(self, name):
         n = self.geti("nbasfuncs")
         return self.d(name, (n, n))
+
+
+# ---------------------------------------------------------------------------
+# learned-guess pair model (host/guess_ml.c) and its network (C-ABI)
+
+def _f(a):
+    return a.ctypes.data_as(fp) if a is not None else None
+
+
+class PairModel:
+    """Features, targets and the rebuild of the pair model for one molecule.
+
+    Built from plain arrays (Z, xyz in Bohr, per-atom shell starts, shell angular
+    momenta, shell first basis functions) or from a TinySCF engine."""
+
+    def __init__(self, Z, xyz, atom_shell_start, am, shell_bf_start, nbf):
+        self._keep = [np.ascontiguousarray(Z, dtype=np.float64), np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1),
+                      np.ascontiguousarray(atom_shell_start, dtype=np.int32), np.ascontiguousarray(am, dtype=np.int32),
+                      np.ascontiguousarray(shell_bf_start, dtype=np.int32)]
+        z, x, ss, a, sb = self._keep
+        self.mol = YsmlMol(len(z), int(nbf), _d(z), _d(x), _i(ss), _i(a), _i(sb))
+        self.L = host_lib()
+        n = self.L.ysml_pairs(ctypes.byref(self.mol), None, 0)
+        self.pairs = np.zeros((n, 2), dtype=np.int32)
+        self.L.ysml_pairs(ctypes.byref(self.mol), _i(self.pairs), n)
+
+    @classmethod
+    def of_engine(cls, T):
+        m = YsmlMol()
+        ss = np.zeros(T.geti("natoms") + 1, dtype=np.int32)
+        T.L.tinyscf_fill_mol(T.T, ctypes.byref(m), _i(ss))
+        na, ns = m.natoms, int(ss[-1])
+        Z = np.ctypeslib.as_array(m.Z, shape=(na,)).copy()
+        xyz = np.ctypeslib.as_array(m.xyz, shape=(3 * na,)).copy()
+        am = np.ctypeslib.as_array(m.am, shape=(ns,)).copy()
+        sb = np.ctypeslib.as_array(m.shell_bf_start, shape=(ns,)).copy()
+        return cls(Z, xyz, ss, am, sb, m.nbf)
+
+    def supported(self):
+        return bool(self.L.ysml_supported(ctypes.byref(self.mol)))
+
+    def types(self):
+        X = self.features()
+        return np.argmax(X[:, YSML_NIN - YSML_NTYPES:], axis=1)
+
+    def features(self):
+        X = np.zeros((len(self.pairs), YSML_NIN), dtype=np.float32)
+        if self.L.ysml_features(ctypes.byref(self.mol), len(self.pairs), _i(self.pairs), _f(X)) != 0:
+            raise RuntimeError("ysml_features failed")
+        return X
+
+    def targets(self, dD):
+        dD = np.ascontiguousarray(dD, dtype=np.float64)
+        Y = np.zeros((len(self.pairs), YSML_NOUT), dtype=np.float32)
+        M = np.zeros_like(Y)
+        if self.L.ysml_targets(ctypes.byref(self.mol), len(self.pairs), _i(self.pairs), _d(dD), _f(Y), _f(M)) != 0:
+            raise RuntimeError("ysml_targets failed (unsupported shells)")
+        return Y, M
+
+    def assemble(self, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        n = self.mol.nbf
+        dD = np.zeros((n, n))
+        if self.L.ysml_assemble(ctypes.byref(self.mol), len(self.pairs), _i(self.pairs), _f(Y), _d(dD)) != 0:
+            raise RuntimeError("ysml_assemble failed (unsupported shells)")
+        return dD
+
+
+def mlp_param_count(n_in, nh, n_out):
+    return n_in * nh + nh + 2 * (nh * nh + nh) + nh * n_out + n_out
+
+
+def mlp_forward(engine, n_in, nh, n_out, params, X):
+    """Y = network(X) on the device of ``engine`` (an Engine)."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], n_out), dtype=np.float32)
+    check(engine.L.ygpu_mlp_forward(engine.h, n_in, nh, n_out, _f(params), X.shape[0], _f(X), _f(Y)))
+    return Y
+
+
+class MLPTrainer:
+    """Device-resident training of the pair network (ygpu_mlp_trainer_*)."""
+
+    def __init__(self, engine, nh, X, Y, M, params):
+        self.engine = engine
+        self.L = engine.L
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        M = np.ascontiguousarray(M, dtype=np.float32)
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self.n_in, self.nh, self.n_out, self.n = X.shape[1], nh, Y.shape[1], X.shape[0]
+        self.np = mlp_param_count(self.n_in, nh, self.n_out)
+        assert params.size == self.np and M.shape == Y.shape and Y.shape[0] == self.n
+        self.h = vp()
+        check(self.L.ygpu_mlp_trainer_create(engine.h, self.n_in, nh, self.n_out, self.n, _f(X), _f(Y), _f(M),
+                                             _f(params), ctypes.byref(self.h)))
+
+    def epoch(self, batch, lr, seed):
+        loss = ctypes.c_double(0.0)
+        check(self.L.ygpu_mlp_trainer_epoch(self.h, int(batch), float(lr), int(seed) & 0xFFFFFFFF, ctypes.byref(loss)))
+        return loss.value
+
+    def grad(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.zeros(self.np, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        check(self.L.ygpu_mlp_trainer_grad(self.h, len(idx), _i(idx), _f(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def params(self):
+        p = np.zeros(self.np, dtype=np.float32)
+        check(self.L.ygpu_mlp_trainer_params(self.h, _f(p)))
+        return p
+
+    def close(self):
+        if self.h:
+            check(self.L.ygpu_mlp_trainer_destroy(self.h))
+            self.h = vp()
