@@ -52,11 +52,11 @@ WORKLOADS = {
 # cases; the large ones converge to the tolerance their configuration names with
 # the DIIS slot of the largest stored residual replaced
 SETTINGS = {
-    "water": dict(ene_tol=1e-11, quirk_a2=1),
-    "c10h22": dict(ene_tol=1e-11, quirk_a2=1),
-    "c54h18": dict(ene_tol=1e-9, quirk_a2=0),
-    "c30h62": dict(ene_tol=1e-9, quirk_a2=0),
-    "c96h24": dict(ene_tol=1e-8, quirk_a2=0),
+    "water": dict(ene_tol=1e-11, quirk_a2=1, family=None),
+    "c10h22": dict(ene_tol=1e-11, quirk_a2=1, family="alkane"),
+    "c54h18": dict(ene_tol=1e-9, quirk_a2=0, family="flake"),
+    "c30h62": dict(ene_tol=1e-9, quirk_a2=0, family="alkane"),
+    "c96h24": dict(ene_tol=1e-8, quirk_a2=0, family="flake"),
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
 BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0   # dense BF16 MFMA peak, three products per FP32 product
@@ -67,9 +67,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False):
+def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False, weights=None):
     import ygpu
     T = ygpu.TinySCF(xyz, niters=niters)        # parse + upload shells (untimed)
+    T.L.tinyscf_set_guess_weights(T.T, (weights or "").encode())
     T.setd("ene_tol", settings["ene_tol"])
     T.seti("quirk_a2", settings["quirk_a2"])
     T.seti("mixed", int(mixed))
@@ -83,7 +84,7 @@ def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False):
     res = dict(seconds=dt, iters=n, energy=T.getd("total_energy"), converged=T.geti("converged"),
                hist=hist, t_df=T.d("t_df").copy(), nbf=T.geti("nbasfuncs"), naux=T.geti("df_nbf"),
                n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"), switch=T.geti("mixed_switch_iter"),
-               F=T.mat("F_mat").copy(), eigval=T.d("eigval").copy())
+               F=T.mat("F_mat").copy(), eigval=T.d("eigval").copy(), guess_ml=T.geti("guess_ml"))
     import numpy as np
     mask = T.i("bf_pair_mask").reshape(res["nbf"], res["nbf"])
     iu = np.triu_indices(res["nbf"])
@@ -127,11 +128,24 @@ def main():
     ap.add_argument("--workload", default="c10h22", choices=sorted(WORKLOADS))
     ap.add_argument("--niters", type=int, default=60)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--guess", default="sad", choices=["sad", "learned"],
+                    help="initial guess of the timed solves: SAD, or SAD plus the learned pair correction "
+                         "(network trained on the workload's family, yatscf-df_amd/data/guess_mlp_<family>.bin)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed-compare", action="store_true")
     args = ap.parse_args()
     settings = SETTINGS[args.workload]
     mixed = args.precision == "mixed"
+    weights = None
+    if args.guess == "learned":
+        if not settings["family"]:
+            print("no learned-guess network for workload %s" % args.workload, file=sys.stderr)
+            sys.exit(2)
+        weights = os.path.join(PKG, "data", "guess_mlp_%s.bin" % settings["family"])
+        if not os.path.exists(weights):
+            print("learned-guess network %s missing (train it with yatscf-df_amd/guess_train.py)" % weights,
+                  file=sys.stderr)
+            sys.exit(2)
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the path is not "
               "sharded across devices (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -143,13 +157,13 @@ def main():
         sys.exit(1)
     xyz = os.path.join(XYZ, args.workload + ".xyz")
     for w in range(args.warmup):
-        r = run_gpu_scf(xyz, args.niters, settings, mixed)
+        r = run_gpu_scf(xyz, args.niters, settings, mixed, weights=weights)
         log("warmup %d: %.3f s, %d iterations, E = %.10f" % (w, r["seconds"], r["iters"], r["energy"]))
     runs = []
     for k in range(args.steps):
         # the FP64 Fock check runs after the timer of the last step stops
         check = (not mixed) and (not args.no_mixed_compare) and k == args.steps - 1
-        r = run_gpu_scf(xyz, args.niters, settings, mixed, fock_check=check)
+        r = run_gpu_scf(xyz, args.niters, settings, mixed, fock_check=check, weights=weights)
         runs.append(r)
         log("step %d: %.3f s, %d iterations, E = %.10f%s" % (k, r["seconds"], r["iters"], r["energy"],
             (", FP64 from iteration %d" % r["switch"]) if mixed else ""))
@@ -192,6 +206,7 @@ def main():
         "dtype": "bf16x3-split/f32-acc K, f32-tensor incremental J, then f64" if mixed else "f64",
         "data": "synthetic (seeded generated geometry; reference basis-set files)",
         "config": {"workload": WORKLOADS[args.workload],
+                   "guess": ("SAD + learned pair correction" if last["guess_ml"] else "SAD"),
                    "precision": ("mixed until DIIS error < 1e-5, then FP64" if mixed else "FP64 (plain path)"),
                    "ene_tol": settings["ene_tol"],
                    "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
@@ -219,12 +234,17 @@ def main():
         out["mixed_switch_iteration"] = last["switch"]
     if not args.no_mixed_compare:
         # the other arithmetic on the same workload, for the mixed-vs-FP64 comparison
-        other = run_gpu_scf(xyz, args.niters, settings, not mixed, fock_check=mixed)
-        fp, mx = (other, last) if mixed else (last, other)
+        # mixed (with the timed run's guess) against the plain FP64 path (SAD guess)
+        if mixed:
+            fp, mx = run_gpu_scf(xyz, args.niters, settings, False, fock_check=True), last
+        else:
+            fp, mx = last, run_gpu_scf(xyz, args.niters, settings, True, weights=weights)
         if "fock_check" not in fp:
             fp = run_gpu_scf(xyz, args.niters, settings, False, fock_check=True)
         mh = rows(mx, True, True)
         out["mixed_vs_fp64"] = {
+            "fp64_guess": "learned" if fp["guess_ml"] else "SAD",
+            "mixed_guess": "learned" if mx["guess_ml"] else "SAD",
             "fp64_s": fp["seconds"], "mixed_s": mx["seconds"], "speedup": fp["seconds"] / mx["seconds"],
             "fp64_iters": fp["iters"], "mixed_iters": mx["iters"], "mixed_switch_iteration": mx["switch"],
             "dE_Eh": mx["energy"] - fp["energy"],

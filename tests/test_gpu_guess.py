@@ -13,7 +13,7 @@ from conftest import PKG
 
 pytestmark = pytest.mark.gpu
 
-WEIGHTS = os.path.join(PKG, "data", "guess_mlp_alkane.bin")
+WEIGHTS = os.path.join(PKG, "data", "guess_mlp_alkane.bin")   # trained by guess_train.py --family alkane
 
 
 def log(*a):

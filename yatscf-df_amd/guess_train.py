@@ -1,17 +1,20 @@
 """Training of the learned initial-guess network on densities the GPU SCF converges.
 
-Data: seeded all-trans C_nH_(2n+2), n uniform in 4..20, random torsions in
-+/-30 deg about the C-C bonds and 0.05 A jitter per coordinate, each converged
-by the TinySCF engine on the GPU (cc-pVDZ / cc-pVDZ-RI, FP64, |dE| < 1e-10).
+Data, family "alkane": seeded all-trans C_nH_(2n+2), n uniform in 4..20,
+random torsions in +/-30 deg about the C-C bonds and 0.05 A jitter per
+coordinate.  Family "flake": H-capped hexagonal graphene flakes C6H6, C24H12
+and C54H18 with 0.05 A jitter.  Each geometry is converged by the TinySCF
+engine on the GPU (cc-pVDZ / cc-pVDZ-RI, FP64, |dE| < 1e-10).
 Samples: every atom pair within 6 A of every molecule, with the pair model's
 features (host/guess_ml.c) and the projection of D_converged - D_SAD onto its
 coefficients.  Network: 3 x 128 SiLU in FP32, trained on the GPU by the
 library's forward / backward / Adam kernels (ygpu_mlp_trainer_*).  Evaluation:
 SCF iterations from SAD against iterations from the learned guess on held-out
-molecules (the C30H62 workload among them), converged energies compared.
+molecules (the C30H62 workload and fresh alkanes for "alkane", the C96H24
+workload for "flake"), converged energies compared.
 
 Usage:
-  python guess_train.py [--geoms 256] [--epochs 120] [--out FILE] [--report FILE]
+  python guess_train.py [--family alkane|flake] [--geoms N] [--epochs 120] [--out FILE] [--report FILE]
 """
 import argparse
 import json
@@ -31,7 +34,11 @@ import geometry  # noqa: E402
 import ygpu  # noqa: E402
 
 NH = 128
-DEFAULT_WEIGHTS = os.path.join(HERE, "data", "guess_mlp_alkane.bin")
+XYZ = os.path.join(os.path.dirname(HERE), "tests", "golden", "xyz")
+
+
+def weights_path(family):
+    return os.path.join(HERE, "data", "guess_mlp_%s.bin" % family)
 
 
 def log(*a):
@@ -49,6 +56,37 @@ def write_alkane(dirname, n, seed):
     path = os.path.join(dirname, "c%dh%d_%d.xyz" % (n, 2 * n + 2, seed))
     geometry.write_xyz(path, syms, xyz)
     return path
+
+
+def flake_set(n_geoms, seed):
+    """(rings, seed): C6H6 : C24H12 : C54H18 in the ratio 3 : 3 : 1."""
+    rng = np.random.RandomState(seed)
+    rings = [0, 0, 0, 1, 1, 1, 2]
+    return [(rings[k % len(rings)], int(rng.randint(1, 2 ** 31 - 1))) for k in range(n_geoms)]
+
+
+def write_flake(dirname, rings, seed):
+    syms, xyz = geometry.graphene(rings, jitter=0.05, seed=seed)
+    path = os.path.join(dirname, "flake%d_%d.xyz" % (rings, seed))
+    geometry.write_xyz(path, syms, xyz)
+    return path
+
+
+def training_set(family, n_geoms, seed, dirname):
+    """(label, xyz path writer) of each training geometry."""
+    if family == "alkane":
+        return [("C%dH%d" % (n, 2 * n + 2), (lambda n=n, s=s: write_alkane(dirname, n, s)))
+                for n, s in alkane_set(n_geoms, seed)]
+    return [("flake%d" % r, (lambda r=r, s=s: write_flake(dirname, r, s))) for r, s in flake_set(n_geoms, seed)]
+
+
+def held_out(family, seed, dirname):
+    if family == "alkane":
+        ev = [("c30h62", os.path.join(XYZ, "c30h62.xyz"), 1e-9)]
+        for n, s in alkane_set(4, seed + 7):
+            ev.append(("c%dh%d seed %d" % (n, 2 * n + 2, s), write_alkane(dirname, n, s), 1e-9))
+        return ev
+    return [("c96h24", os.path.join(XYZ, "c96h24.xyz"), 1e-8)]
 
 
 def solve(xyz, weights=None, ene_tol=1e-10, niters=80):
@@ -168,24 +206,29 @@ def train(X, Y, M, t, epochs, batch, lr, seed, val_frac=0.1):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--geoms", type=int, default=256)
+    ap.add_argument("--family", default="alkane", choices=["alkane", "flake"])
+    ap.add_argument("--geoms", type=int, default=None, help="default 256 (alkane), 112 (flake)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--out", default=None, help="default data/guess_mlp_<family>.bin")
     ap.add_argument("--report", default=None)
     args = ap.parse_args()
+    if args.geoms is None:
+        args.geoms = 256 if args.family == "alkane" else 112
+    if args.out is None:
+        args.out = weights_path(args.family)
     if ygpu.device_count() < 1:
         log("no GPU visible")
         sys.exit(1)
     tmp = tempfile.mkdtemp(prefix="ysml_")
     Xs, Ys, Ms, Ts = [], [], [], []
     t0 = time.perf_counter()
-    for k, (n, s) in enumerate(alkane_set(args.geoms, args.seed)):
-        r = solve(write_alkane(tmp, n, s))
+    for k, (label, make) in enumerate(training_set(args.family, args.geoms, args.seed, tmp)):
+        r = solve(make())
         if not r["converged"]:
-            log("geometry %d (C%dH%d, seed %d) did not converge; skipped" % (k, n, 2 * n + 2, s))
+            log("geometry %d (%s) did not converge; skipped" % (k, label))
             continue
         X, Y, M, t = samples(r)
         Xs.append(X); Ys.append(Y); Ms.append(M); Ts.append(t)
@@ -198,17 +241,15 @@ def main():
     t2 = time.perf_counter()
     write_weights(args.out, w["nh"], w["in_mean"], w["in_std"], w["out_mean"], w["out_std"], w["params"])
     log("weights written to %s (training %.1f s)" % (args.out, t2 - t1))
-    # held-out evaluation: the C30H62 workload and fresh alkanes of other seeds
-    evals = [("c30h62", os.path.join(os.path.dirname(HERE), "tests", "golden", "xyz", "c30h62.xyz"))]
-    for n, s in alkane_set(4, args.seed + 7):
-        evals.append(("c%dh%d seed %d" % (n, 2 * n + 2, s), write_alkane(tmp, n, s)))
-    report = dict(samples=int(len(X)), molecules=len(Xs), data_seconds=t1 - t0, train_seconds=t2 - t1,
-                  final_train_loss=w["history"][-1][1], final_validation_loss=w["history"][-1][2], held_out=[])
-    for name, path in evals:
+    # held-out evaluation
+    report = dict(family=args.family, samples=int(len(X)), molecules=len(Xs), data_seconds=t1 - t0,
+                  train_seconds=t2 - t1, final_train_loss=w["history"][-1][1],
+                  final_validation_loss=w["history"][-1][2], held_out=[])
+    for name, path, tol in held_out(args.family, args.seed, tmp):
         if not os.path.exists(path):
             continue
-        a = solve(path, None, ene_tol=1e-9)
-        b = solve(path, args.out, ene_tol=1e-9)
+        a = solve(path, None, ene_tol=tol)
+        b = solve(path, args.out, ene_tol=tol)
         row = dict(molecule=name, sad_iters=a["iters"], learned_iters=b["iters"], iterations_saved=a["iters"] - b["iters"],
                    sad_seconds=a["seconds"], learned_seconds=b["seconds"], dE_Eh=b["energy"] - a["energy"],
                    learned_applied=bool(b["guess_ml"]), converged=bool(a["converged"] and b["converged"]))

@@ -81,7 +81,11 @@ void TinySCF_DIIS(TinySCF_t T)
     T->DIIS_rhs[T->DIIS_len] = -1.0;
     double Bc[(MAX_DIIS + 1) * (MAX_DIIS + 1)];
     memcpy(Bc, T->B_mat, sizeof(Bc));
-    solve_small(T->DIIS_len + 1, Bc, ldB, T->DIIS_rhs);
+    if (solve_small(T->DIIS_len + 1, Bc, ldB, T->DIIS_rhs) != 0) {
+        /* singular system (e.g. two identical residuals): keep the newest Fock matrix */
+        memset(T->DIIS_rhs, 0, DBL_SIZE * (MAX_DIIS + 1));
+        T->DIIS_rhs[idx] = 1.0;
+    }
     YS_GPU(ygpu_diis_extrapolate(g, T->DIIS_len, T->DIIS_rhs, T->F_mat));
     T->t_diis = get_wtime_sec() - st;
 }

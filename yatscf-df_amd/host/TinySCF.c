@@ -295,7 +295,10 @@ void TinySCF_do_SCF(TinySCF_t T)
                     T->DIIS_len = 0;
                     T->DIIS_bmax_id = 0;
                     T->DIIS_bmax = -DBL_MAX;
-                    memset(T->B_mat, 0, DBL_SIZE * (MAX_DIIS + 1) * (MAX_DIIS + 1));
+                    /* B as initialised (TinySCF_init_free.c:178-182): -1 off the diagonal
+                     * (the constraint row and column), 0 on it */
+                    for (int i = 0; i < (MAX_DIIS + 1) * (MAX_DIIS + 1); i++) T->B_mat[i] = -1.0;
+                    for (int i = 0; i < MAX_DIIS + 1; i++) T->B_mat[i * (MAX_DIIS + 1) + i] = 0.0;
                 }
             }
         } else if (T->iter == T->mixed_switch_iter) {
