@@ -133,7 +133,21 @@ def main():
                          "(network trained on the workload's family, yatscf-df_amd/data/guess_mlp_<family>.bin)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed-compare", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (torch copy kernel): the known-size "
+                         "reference for scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                          "copies": 5}))
+        return
     settings = SETTINGS[args.workload]
     mixed = args.precision == "mixed"
     weights = None

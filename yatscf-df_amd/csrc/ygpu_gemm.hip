@@ -213,6 +213,105 @@ __global__ __launch_bounds__(NTHR) void k_syrk_part(const double *__restrict__ X
     });
 }
 
+// ---- SYRK with 128x128 tiles: 4 wavefronts, each a 64x64 sub-tile of 4x4 MFMA blocks.
+// Twice the flops per loaded byte of the 64x64 engine; the k range is split in chunks
+// and workgroups are ordered chunk-major and XCD-contiguous, so the tiles that read the
+// same panels of one chunk run on one XCD at the same time.
+constexpr int S_T = 128, S_BK = 16, S_PAD = 2;
+
+__device__ __forceinline__ long xcd_remap64(long b, long total)
+{
+    const long per = total / 8, rem = total % 8, x = b % 8, slot = b / 8;
+    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+__global__ __launch_bounds__(NTHR) void k_syrk128(const double *__restrict__ X, int nbf, long kdim, int ntb,
+                                                  int ntiles, long kchunk, long total, double *__restrict__ part)
+{
+    __shared__ double As[S_BK][S_T + S_PAD];
+    __shared__ double Bs[S_BK][S_T + S_PAD];
+    const long w_id = xcd_remap64(blockIdx.x, total);
+    int tile = (int) (w_id % ntiles);
+    const long z = w_id / ntiles;
+    int bi = 0;
+    while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
+    const int bj = bi + tile;
+    const int m0 = bi * S_T, n0 = bj * S_T;
+    const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+    // loader: thread t moves 8 consecutive k of row t/2 (half t%2) for both panels
+    const int lr = t >> 1, lk = (t & 1) * 8;
+    const bool ina = m0 + lr < nbf, inb = n0 + lr < nbf;
+    const double *pa = X + (size_t) (ina ? m0 + lr : 0) * kdim;
+    const double *pb = X + (size_t) (inb ? n0 + lr : 0) * kdim;
+    v4d acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    double ra[8], rb[8];
+    // k extents are multiples of 8 (kdim and the chunks), so a thread's 8 values are
+    // all inside or all outside the range: 16-byte loads
+    auto load = [&](long k0) {
+        const long k = k0 + lk;
+        const bool kin = k < ke;
+        if (ina && kin) {
+            const double2 *p2 = (const double2 *) (pa + k);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { const double2 v = p2[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ra[q] = 0.0;
+        }
+        if (inb && kin) {
+            const double2 *p2 = (const double2 *) (pb + k);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { const double2 v = p2[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rb[q] = 0.0;
+        }
+    };
+    if (kb < ke) load(kb);
+    for (long k0 = kb; k0 < ke; k0 += S_BK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            As[lk + q][lr] = ra[q];
+            Bs[lk + q][lr] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + S_BK < ke) load(k0 + S_BK);
+#pragma unroll
+        for (int kk = 0; kk < S_BK; kk += 4) {
+            const int kr = kk + (lane >> 4), c = lane & 15;
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = As[kr][wm + 16 * i + c];
+                b[i] = Bs[kr][wn + 16 * i + c];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    double *P = part + (size_t) z * nbf * nbf;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = acc[i][j][v];
+            }
+}
+
 // K[m][n] = K[n][m] = sum_z part[z][m][n] for m <= n
 __global__ void k_syrk_reduce(const double *__restrict__ part, int nsplit, int nbf, double *__restrict__ K)
 {
@@ -256,6 +355,30 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
 
 int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part)
 {
+    if (nbf >= 256) {
+        // 128x128 tiles; about 2048 workgroups (four rounds of two per CU), chunks of at least
+        // 1024 along k
+        if (kdim % 8) {
+            ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
+            return YGPU_ERR_INVALID;
+        }
+        const int ntb = (nbf + S_T - 1) / S_T;
+        const int ntiles = ntb * (ntb + 1) / 2;
+        int nsplit = (int) std::max<long>(1, std::min<long>((2048 + ntiles - 1) / ntiles, (kdim + 1023) / 1024));
+        long kchunk = (kdim + nsplit - 1) / nsplit;
+        kchunk = (kchunk + S_BK - 1) / S_BK * S_BK;
+        nsplit = (int) ((kdim + kchunk - 1) / kchunk);
+        YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(double)));
+        const long total = (long) ntiles * nsplit;
+        hipLaunchKernelGGL(k_syrk128, dim3((unsigned) total), dim3(NTHR), 0, s, X, nbf, kdim, ntb, ntiles, kchunk,
+                           total, part.as<double>());
+        YG_HIP(hipGetLastError());
+        const long tot = (long) nbf * nbf;
+        hipLaunchKernelGGL(k_syrk_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<double>(), nsplit, nbf,
+                           K);
+        YG_HIP(hipGetLastError());
+        return YGPU_OK;
+    }
     const int ntb = (nbf + BM - 1) / BM;
     const int ntiles = ntb * (ntb + 1) / 2;
     // enough workgroups to fill the chip, chunks of at least 512 along k
