@@ -67,6 +67,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(workload, mixed):
+    """HBM bytes per launch of the roofline kernel from its committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    name = "k_xbuild_%s_%s_traffic.md" % (workload, "mixed" if mixed else "fp64")
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1]), "profiles/" + name
+    return None, None
+
+
 def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False, weights=None):
     import ygpu
     T = ygpu.TinySCF(xyz, niters=niters)        # parse + upload shells (untimed)
@@ -204,6 +217,7 @@ def main():
     xb_flops = 2.0 * nnz * n_occ * naux
     achieved = xb_flops / (xb_ms * 1e-3) / 1e12
     peak = BF16_SPLIT_PEAK_TFLOPS if mixed else FP64_MFMA_PEAK_TFLOPS
+    traffic, traffic_src = profiled_traffic(args.workload, mixed)
     # J reads the packed tensor twice (FP32 copy in the mixed phase) and D, J once
     j_bytes = 2.0 * n_up * naux * (4 if mixed else 8) + 16.0 * nbf * nbf
     out = {
@@ -238,7 +252,7 @@ def main():
         "roofline": {"kernel": ("k_xbuild_bf3 (X_i = C_occ^T B rows of i, BF16-split MFMA 32x32x16)" if mixed
                                else "k_xbuild (X_i = C_occ^T B rows of i, FP64 MFMA 16x16x4)"),
                      "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "launch_ms": xb_ms, "flops_per_launch": xb_flops},
         "j_build": {"bound": "hbm", "ms": j_ms, "bytes_per_call": j_bytes,
                     "achieved_GBs": j_bytes / (j_ms * 1e-3) / 1e9, "peak_GBs": HBM_PEAK_GBS},

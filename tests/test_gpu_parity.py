@@ -288,3 +288,36 @@ def test_device_diis(ygpu, name):
         Fx = e.diis_extrapolate(coef)
         ref = sum(c * (X.T @ F @ X) for c, F in zip(coef, Fs))
         assert np.max(np.abs(Fx - ref)) < 1e-11
+
+
+@pytest.mark.parametrize("n_occ", [5, 100, 171, 250, 300, 330])
+def test_k_build_tile_variants(ygpu, n_occ):
+    """K = sum_P (B_P C)(B_P C)^T for synthetic occupied blocks of every width the X-build
+    kernels specialise on (all occupied rows of a basis function in one workgroup up to 320,
+    row tiles beyond), FP64 and mixed, against the closed form in NumPy on the oracle's tensor."""
+    log("case: K tile variants n_occ", n_occ)
+    st = oracle_state("c10h22")
+    o, n = st.o, st.sys.nbf
+    nnz, naux = o.geti("num_bf_pair_scr"), st.sys.naux
+    B = o.d("df_tensor").reshape(nnz, naux)
+    mask = o.i("bf_pair_mask").reshape(n, n)
+    C = np.random.RandomState(n_occ).randn(n, n_occ) * 0.3
+    # X_i = C(partners)^T B(rows of i); K = sum over (s, P) of X_i X_j
+    X = np.zeros((n, n_occ, naux))
+    for i in range(n):
+        js = np.nonzero(mask[i] >= 0)[0]
+        X[i] = C[js].T @ B[mask[i, js]]
+    Xf = X.reshape(n, -1)
+    K = Xf @ Xf.T
+    S = np.abs(Xf) @ np.abs(Xf).T
+    with make_engine(ygpu, st) as e:
+        set_oracle_sparsity(e, st)
+        e.set_df_tensor(o.d("df_tensor"))
+        r64 = e.build_fock(st.D0, C, st.o.mat("Hcore_mat"))
+        e.set_fock_precision(ygpu.FOCK_MIXED)
+        rmx = e.build_fock(st.D0, C, st.o.mat("Hcore_mat"))
+    e64 = np.abs(r64["K"] - K).max() / np.abs(K).max()
+    emx = (np.abs(rmx["K"] - K) / S).max()
+    log("  FP64 max|dK|/max|K| %.3e; mixed max|dK|/S %.3e" % (e64, emx))
+    assert e64 < 1e-12
+    assert emx < 2.0 ** -14

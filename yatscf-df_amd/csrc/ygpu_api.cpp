@@ -157,6 +157,8 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
                             n, hipMemcpyDeviceToDevice, c->stream));
     YG_TRY(yg_scale_rows(c->stream, n, n, dW, ld, c->d_eig.as<double>(), 1));
     YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, dA, ld, dW, ld, 0.0, dOut, ld));
+    // the two triangles round the scaled products differently: make the result exactly symmetric
+    YG_TRY(yg_symmetrize(c->stream, n, dOut, ld));
     return YGPU_OK;
 }
 
@@ -520,8 +522,10 @@ try {
     YG_TRY(inv_sqrt_device(c, c->naux, (int) ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
                            c->d_Jpq.as<double>(), &mine));
     YG_HIP(hipEventRecord(c->ev[3], c->stream));
-    YG_TRY(yg_dgemm(c->stream, 0, 0, (int) c->n_up, (int) ld, (int) ld, 1.0, c->d_pqA.as<double>(), (int) ld,
-                    c->d_Jpq.as<double>(), (int) ld, 0.0, c->d_B.as<double>(), (int) ld));
+    // B = pqA J^{-1/2}; J^{-1/2} is exactly symmetric (inv_sqrt_device symmetrises it), so its rows
+    // serve as the k-contiguous second operand
+    YG_TRY(yg_dgemm_nt128(c->stream, (int) c->n_up, (int) ld, ld, c->d_pqA.as<double>(), ld,
+                          c->d_Jpq.as<double>(), ld, c->d_B.as<double>(), ld));
     YG_HIP(hipEventRecord(c->ev[4], c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     if (t) {

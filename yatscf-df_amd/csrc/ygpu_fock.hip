@@ -202,6 +202,18 @@ __global__ void k_antisym(int n, const double *__restrict__ A, double *__restric
     out[idx] = A[idx] - A[(size_t) j * n + i];
 }
 
+// A = (A + A^T) / 2 on the n x n block (one thread per upper element)
+__global__ void k_symmetrize(int n, double *__restrict__ A, long lda)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) n * n) return;
+    const int i = (int) (idx / n), j = (int) (idx % n);
+    if (i >= j) return;
+    const double v = 0.5 * (A[(size_t) i * lda + j] + A[(size_t) j * lda + i]);
+    A[(size_t) i * lda + j] = v;
+    A[(size_t) j * lda + i] = v;
+}
+
 // difference density of the incremental J: dD = D - Dprev, then Dprev = D
 __global__ void k_delta(long n, const double *__restrict__ D, double *__restrict__ Dprev, double *__restrict__ dD)
 {
@@ -276,6 +288,14 @@ int yg_antisym(hipStream_t s, int n, const double *A, double *out)
 {
     const long tot = (long) n * n;
     hipLaunchKernelGGL(k_antisym, dim3((tot + 255) / 256), dim3(256), 0, s, n, A, out);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_symmetrize(hipStream_t s, int n, double *A, long lda)
+{
+    const long tot = (long) n * n;
+    hipLaunchKernelGGL(k_symmetrize, dim3((tot + 255) / 256), dim3(256), 0, s, n, A, lda);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -225,34 +225,27 @@ __device__ __forceinline__ long xcd_remap64(long b, long total)
     return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
-__global__ __launch_bounds__(NTHR) void k_syrk128(const double *__restrict__ X, int nbf, long kdim, int ntb,
-                                                  int ntiles, long kchunk, long total, double *__restrict__ part)
+// acc += A(rows m0.., k in [kb, ke)) B(rows n0.., same k)^T with both operands stored as
+// k-contiguous rows (lda, ldb); rows at or beyond arows / brows read as zero.  k extents and
+// row starts are multiples of 8 doubles, so a thread's 8 values are all inside or all outside.
+__device__ __forceinline__ void tile128_nt(const double *__restrict__ A, long lda, int arows, int m0,
+                                           const double *__restrict__ B, long ldb, int brows, int n0, long kb,
+                                           long ke, v4d (&acc)[4][4])
 {
     __shared__ double As[S_BK][S_T + S_PAD];
     __shared__ double Bs[S_BK][S_T + S_PAD];
-    const long w_id = xcd_remap64(blockIdx.x, total);
-    int tile = (int) (w_id % ntiles);
-    const long z = w_id / ntiles;
-    int bi = 0;
-    while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
-    const int bj = bi + tile;
-    const int m0 = bi * S_T, n0 = bj * S_T;
-    const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
-    // loader: thread t moves 8 consecutive k of row t/2 (half t%2) for both panels
+    // loader: thread t moves 8 consecutive k of row t/2 (half t%2) of both panels
     const int lr = t >> 1, lk = (t & 1) * 8;
-    const bool ina = m0 + lr < nbf, inb = n0 + lr < nbf;
-    const double *pa = X + (size_t) (ina ? m0 + lr : 0) * kdim;
-    const double *pb = X + (size_t) (inb ? n0 + lr : 0) * kdim;
-    v4d acc[4][4];
+    const bool ina = m0 + lr < arows, inb = n0 + lr < brows;
+    const double *pa = A + (size_t) (ina ? m0 + lr : 0) * lda;
+    const double *pb = B + (size_t) (inb ? n0 + lr : 0) * ldb;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     double ra[8], rb[8];
-    // k extents are multiples of 8 (kdim and the chunks), so a thread's 8 values are
-    // all inside or all outside the range: 16-byte loads
     auto load = [&](long k0) {
         const long k = k0 + lk;
         const bool kin = k < ke;
@@ -299,17 +292,57 @@ __global__ __launch_bounds__(NTHR) void k_syrk128(const double *__restrict__ X, 
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
-    double *P = part + (size_t) z * nbf * nbf;
+}
+
+template <class F>
+__device__ __forceinline__ void tile128_epilogue(const v4d (&acc)[4][4], F &&store)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int m = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
-                const int n = n0 + wn + 16 * j + (lane & 15);
-                if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = acc[i][j][v];
-            }
+            for (int v = 0; v < 4; ++v)
+                store(wm + 16 * i + (lane >> 4) + 4 * v, wn + 16 * j + (lane & 15), acc[i][j][v]);
+}
+
+__global__ __launch_bounds__(NTHR) void k_syrk128(const double *__restrict__ X, int nbf, long kdim, int ntb,
+                                                  int ntiles, long kchunk, long total, double *__restrict__ part)
+{
+    const long w_id = xcd_remap64(blockIdx.x, total);
+    int tile = (int) (w_id % ntiles);
+    const long z = w_id / ntiles;
+    int bi = 0;
+    while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
+    const int bj = bi + tile;
+    const int m0 = bi * S_T, n0 = bj * S_T;
+    const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
+    v4d acc[4][4];
+    tile128_nt(X, kdim, nbf, m0, X, kdim, nbf, n0, kb, ke, acc);
+    double *P = part + (size_t) z * nbf * nbf;
+    tile128_epilogue(acc, [&](int r, int c, double v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = v;
+    });
+}
+
+// C [M][N] (ldc) = A [M][K] B[N][K]^T, 128x128 tiles, column tiles of one row panel
+// consecutive on one XCD
+__global__ __launch_bounds__(NTHR) void k_gemm128_nt(const double *__restrict__ A, long lda,
+                                                     const double *__restrict__ B, long ldb, int M, int N, long K,
+                                                     int ncolt, long total, double *__restrict__ C, long ldc)
+{
+    const long w_id = xcd_remap64(blockIdx.x, total);
+    const int ct = (int) (w_id % ncolt), rt = (int) (w_id / ncolt);
+    const int m0 = rt * S_T, n0 = ct * S_T;
+    v4d acc[4][4];
+    tile128_nt(A, lda, M, m0, B, ldb, N, n0, 0, K, acc);
+    tile128_epilogue(acc, [&](int r, int c, double v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < M && n < N) C[(size_t) m * ldc + n] = v;
+    });
 }
 
 // K[m][n] = K[n][m] = sum_z part[z][m][n] for m <= n
@@ -338,6 +371,26 @@ int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, c
     else if (!ta && tb) hipLaunchKernelGGL((k_gemm<MODE_N, MODE_T>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
     else if (ta && !tb) hipLaunchKernelGGL((k_gemm<MODE_T, MODE_N>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
     else hipLaunchKernelGGL((k_gemm<MODE_T, MODE_T>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
+                   double *C, long ldc)
+{
+    if (m <= 0 || n <= 0) return YGPU_OK;
+    if (k % 8 || lda % 8 || ldb % 8) {
+        ygpu_set_error(YGPU_ERR_INVALID, "yg_dgemm_nt128: k and leading dimensions must be multiples of 8");
+        return YGPU_ERR_INVALID;
+    }
+    const int ncolt = (n + S_T - 1) / S_T, nrowt = (m + S_T - 1) / S_T;
+    const long total = (long) ncolt * nrowt;
+    if (total > 0x7fffffffL) {
+        ygpu_set_error(YGPU_ERR_INVALID, "yg_dgemm_nt128: %ld workgroups exceed the grid limit", total);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_gemm128_nt, dim3((unsigned) total), dim3(NTHR), 0, s, A, lda, B, ldb, m, n, k, ncolt, total, C,
+                       ldc);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -112,6 +112,9 @@ int yg_launch_eri3(ygpu_ctx *c, double *dpqA);
 // C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
              const double *B, int ldb, double beta, double *C, int ldc);
+// C [m][n] = A [m][k] B [n][k]^T with k-contiguous rows (k, lda, ldb multiples of 8), 128x128 tiles
+int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
+                   double *C, long ldc);
 // X[i] = Cocc(partners of row i)^T * Bup(rows of i) for every basis function i
 int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
@@ -120,6 +123,7 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
 // J from density dD (upper triangle of dJ; the rest zero); B32 != nullptr reads the FP32 copy
 int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms);
 int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
+int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                      // A = (A + A^T) / 2
 int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)
