@@ -31,9 +31,14 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int NTHR = 256;     // SYRK workgroup (2 x 2 wavefronts)
-constexpr int BK = 32;          // k per LDS stage: two 16-deep MFMA steps
-constexpr int KV = BK / 8;      // 8-value groups per row per stage
-constexpr int KVP = KV + 1;     // padded LDS row stride (16-byte units)
+// k per LDS stage (BK = 32 or 64: two or four 16-deep MFMA steps); rows are stored as
+// BK / 8 groups of 8 BF16 values plus one group of padding
+template <int BK> struct Stage {
+    static constexpr int KV = BK / 8;
+    static constexpr int KVP = KV + 1;
+};
+constexpr int SBK = 64;   // SYRK stage depth (4 wavefronts have the registers for it)
+constexpr int XBK = 32;   // X-build stage depth (12-16 wavefronts per workgroup)
 
 __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 {
@@ -49,9 +54,10 @@ __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 // Loader contract: coords(idx, row, kv) names the tile row and 8-group of the
 // idx-th (row, kv) slot (idx < tile rows * KV); load(row, k8, kend, v) fills the
 // 8 values k8 .. k8+7 of that row (zero beyond kend or outside the matrix).
-template <int WM, int WN, class LA, class LB>
+template <int WM, int WN, int BK, class LA, class LB>
 __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2])
 {
+    constexpr int KV = Stage<BK>::KV, KVP = Stage<BK>::KVP;
     constexpr int NT = 64 * WM * WN;
     constexpr int BMt = 64 * WM, BNt = 64 * WN;
     constexpr int SA = BMt * KV, SB = BNt * KV;                    // (row, 8-group) slots per stage
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
     XLoadA<64 * WM> la{Cocc32, xpartner + kb, n_occ, m0};
     XLoadB<64 * WN> lb{B32, xup + kb, naux_pad, n0};
     v16f acc[2][2];
-    bf3_tile<WM, WN>(la, lb, 0, klen, acc);
+    bf3_tile<WM, WN, XBK>(la, lb, 0, klen, acc);
     float *Xi = X + (size_t) i * n_occ * naux_pad;
     bf3_epilogue<WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
@@ -206,9 +212,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
 }
 
 // ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous
+template <int BK>
 struct SLoad {
     const float *x; long kdim; int nbf, r0;
-    __device__ void coords(int idx, int &row, int &kv) const { kv = idx % KV; row = idx / KV; }
+    __device__ void coords(int idx, int &row, int &kv) const
+    {
+        kv = idx % Stage<BK>::KV;
+        row = idx / Stage<BK>::KV;
+    }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
         const int r = r0 + row;
         if (r < nbf && k8 < kend) {
@@ -236,10 +247,10 @@ __global__ __launch_bounds__(NTHR) void k_syrk_bf3(const float *__restrict__ X, 
     while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
     const int bj = bi + tile;
     const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
-    SLoad la{X + kb, kdim, nbf, bi * ST};
-    SLoad lb{X + kb, kdim, nbf, bj * ST};
+    SLoad<SBK> la{X + kb, kdim, nbf, bi * ST};
+    SLoad<SBK> lb{X + kb, kdim, nbf, bj * ST};
     v16f acc[2][2];
-    bf3_tile<2, 2>(la, lb, 0, (int) (ke - kb), acc);
+    bf3_tile<2, 2, SBK>(la, lb, 0, (int) (ke - kb), acc);
     float *P = part + (size_t) z * nbf * nbf;
     const int m0 = bi * ST, n0 = bj * ST;
     bf3_epilogue<2>(acc, [&](int r, int c, float v) {
@@ -325,7 +336,7 @@ int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DB
     // chunks), enough of them for about 1024 workgroups
     long kchunk = kdim * ntiles / 1024;
     kchunk = std::max<long>(1024, std::min<long>(8192, kchunk));
-    kchunk = (kchunk + BK - 1) / BK * BK;
+    kchunk = (kchunk + SBK - 1) / SBK * SBK;
     const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
     YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
     const long total = (long) ntiles * nsplit;
