@@ -53,6 +53,16 @@ int ygpu_device_count(int *count);
 int ygpu_create(int device, ygpu_ctx_t **ctx);
 int ygpu_destroy(ygpu_ctx_t *ctx);
 
+/* Page-locked host memory for the matrices the drop-in exchanges with the
+ * device on every SCF iteration (F, D, J, K, C_occ, eigenvalues, X, Hcore, S)
+ * and for J^{-1/2}: the reference allocates them with ALIGN64B_MALLOC
+ * (TinySCF_init_free.c:131-143, :194) and frees them with ALIGN64B_FREE
+ * (:232-246); page-locked, the per-iteration copies run at DMA rate instead of
+ * through the runtime's staging buffers.  64-byte aligned or better.  Free
+ * with ygpu_host_free only; ygpu_host_free(NULL) is a no-op. */
+int ygpu_host_alloc(size_t bytes, void **ptr);
+int ygpu_host_free(void *ptr);
+
 /* Shells of the orbital basis (which = 0) or of the density-fitting basis
  * (which = 1), in the reference's shell order (CMS_parse_molecule,
  * CMS_BasisSet.c:192-299) with normalised contraction coefficients (the

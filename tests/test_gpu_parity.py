@@ -231,6 +231,54 @@ def test_density(ygpu, name):
         assert np.all(idx == np.arange(n))
 
 
+def s_shells(n):
+    """n single-primitive s shells: a basis that only sets nbf = n for the density build."""
+    return dict(nshells=n, am=np.zeros(n), nprim=np.ones(n), prim_off=np.arange(n),
+                xyz=np.zeros((n, 3)), alpha=np.ones(n), coef=np.ones(n))
+
+
+def spectrum_case(n, kind, seed):
+    """Symmetric F' and n_occ with a gap of at least 0.5 above the occupied levels."""
+    rng = np.random.RandomState(seed)
+    nocc = max(1, n // 4)
+    lam = np.sort(rng.uniform(-12.0, 3.0, n))
+    lam[nocc:] += 0.5
+    if kind == "cluster" and nocc >= 4:
+        lam[1:4] = lam[1]                       # an exactly degenerate occupied triple
+    if kind == "diag":
+        return np.diag(lam[rng.permutation(n)]), nocc
+    Q, _ = np.linalg.qr(rng.randn(n, n))
+    F = (Q * lam) @ Q.T
+    return 0.5 * (F + F.T), nocc
+
+
+# 272 is the largest size of the one-workgroup register-resident tridiagonal reduction;
+# 273 and 300 take rocSOLVER dsyevd
+@pytest.mark.parametrize("n,kind", [(1, "dense"), (2, "dense"), (3, "dense"), (17, "cluster"), (31, "dense"),
+                                    (32, "dense"), (33, "diag"), (64, "cluster"), (200, "diag"), (255, "dense"),
+                                    (271, "cluster"), (272, "dense"), (273, "dense"), (300, "cluster")])
+def test_density_eigensolver_sizes(ygpu, n, kind):
+    log("case: density eigensolver n=%d %s" % (n, kind))
+    F, nocc = spectrum_case(n, kind, 1000 + n)
+    rng = np.random.RandomState(n)
+    X = np.eye(n) + 0.05 * rng.randn(n, n) / np.sqrt(n)
+    with make_engine(ygpu) as e:
+        e.set_basis(0, s_shells(n))
+        D, C, ev, idx = e.build_density(F, X, nocc)
+    w, V = np.linalg.eigh(F)
+    Dref = X @ V[:, :nocc] @ V[:, :nocc].T @ X.T
+    scale = max(1.0, np.abs(w).max())
+    errE, errD = np.max(np.abs(ev - w)), np.max(np.abs(D - Dref))
+    # orthonormal occupied orbitals in the orthogonal basis: C0 = X^-1 C
+    C0 = np.linalg.solve(X, C)
+    errO = np.max(np.abs(C0.T @ C0 - np.eye(nocc)))
+    log("  max|d eps| %.2e  max|dD| %.2e  orthonormality %.2e" % (errE, errD, errO))
+    assert errE < 1e-12 * scale * max(1.0, n / 16)
+    assert errD < 1e-11
+    assert errO < 1e-12
+    assert np.max(np.abs(C @ C.T - D)) < 1e-12
+
+
 @pytest.mark.parametrize("name,a2", [("water", True), ("c10h22", True), ("c10h22", False)])
 def test_full_scf_energy(ygpu, name, a2):
     """Drop-in TinySCF engine (host C + device) against the oracle SCF; a2=False takes the

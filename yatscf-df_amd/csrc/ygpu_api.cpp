@@ -108,6 +108,7 @@ int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
     }
     (void) rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, nullptr, ld, nullptr, nullptr,
                             nullptr);
+    (void) rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, nullptr, nullptr, nullptr, ld, nullptr);
     if (rocblas_stop_device_memory_size_query(c->rb, &need) != rocblas_status_success) {
         ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_stop_device_memory_size_query failed");
         return YGPU_ERR_SOLVER;
@@ -230,6 +231,27 @@ try {
     YG_TRY(upload(c->boys, tab.data(), tab.size() * sizeof(double), c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     *out = c;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_host_alloc(size_t bytes, void **ptr)
+try {
+    if (!ptr) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_host_alloc: null result pointer");
+        return YGPU_ERR_INVALID;
+    }
+    *ptr = nullptr;
+    YG_HIP(hipHostMalloc(ptr, bytes ? bytes : 64, hipHostMallocDefault));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_host_free(void *ptr)
+try {
+    if (ptr) YG_HIP(hipHostFree(ptr));
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -534,9 +556,15 @@ try {
         t->t_invsqrt = elapsed(c->ev[2], c->ev[3]) * 1e-3;
         t->t_dfgemm = elapsed(c->ev[3], c->ev[4]) * 1e-3;
     }
-    if (Jpq_out)
-        YG_HIP(hipMemcpy2D(Jpq_out, c->naux * sizeof(double), c->d_Jpq.p, ld * sizeof(double), c->naux * sizeof(double),
-                           c->naux, hipMemcpyDeviceToHost));
+    if (Jpq_out) {
+        // one contiguous copy: the pitched device-to-host copy runs row by row
+        double *packed = c->d_mat1.as<double>();
+        YG_HIP(hipMemcpy2DAsync(packed, c->naux * sizeof(double), c->d_Jpq.p, ld * sizeof(double),
+                                c->naux * sizeof(double), c->naux, hipMemcpyDeviceToDevice, c->stream));
+        YG_HIP(hipMemcpyAsync(Jpq_out, packed, (size_t) c->naux * c->naux * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        YG_HIP(hipStreamSynchronize(c->stream));
+    }
     if (pqA_out) YG_TRY(unpack_to_reference(c, c->d_pqA.as<double>(), pqA_out));
     if (df_out) YG_TRY(unpack_to_reference(c, c->d_B.as<double>(), df_out));
     c->d_pqA.release();   // (mn|P) is not needed once B is formed
@@ -720,16 +748,37 @@ try {
         dX = c->d_mat1.as<double>();
     }
     YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
-    rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf, c->d_mat0.as<double>(),
-                                         nbf, c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
-    if (st != rocblas_status_success) {
-        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", nbf, (int) st);
-        return YGPU_ERR_SOLVER;
+    if (nbf <= YG_EIG_NMAX_SMALL) {
+        // F' = Q T Q^T on one workgroup, T = Z diag Z^T by divide and conquer, then only the
+        // n_occ lowest eigenvectors are carried back: V_occ^T = (Q Z_occ)^T in d_U (rows)
+        YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
+        YG_TRY(c->d_U.ensure((size_t) nbf * n_occ * sizeof(double)));
+        YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), c->d_eig.as<double>(), c->d_evE.as<double>(),
+                              c->d_tau.as<double>()));
+        rocblas_status st = rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, nbf, c->d_eig.as<double>(),
+                                             c->d_evE.as<double>(), c->d_mat2.as<double>(), nbf, c->d_info.as<int>());
+        if (st != rocblas_status_success) {
+            ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", nbf, (int) st);
+            return YGPU_ERR_SOLVER;
+        }
+        YG_TRY(yg_backtransform_rows(c->stream, nbf, n_occ, c->d_mat0.as<double>(), c->d_tau.as<double>(),
+                                     c->d_mat2.as<double>(), c->d_U.as<double>()));
+        // C_occ = X V_occ
+        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_U.as<double>(), nbf, 0.0,
+                        c->d_Cocc.as<double>(), n_occ));
+    } else {
+        rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf,
+                                             c->d_mat0.as<double>(), nbf, c->d_eig.as<double>(),
+                                             c->d_evE.as<double>(), c->d_info.as<int>());
+        if (st != rocblas_status_success) {
+            ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", nbf, (int) st);
+            return YGPU_ERR_SOLVER;
+        }
+        // C = X V: with V column-major in d_mat0, V(k,n) = d_mat0[n*nbf+k] = op_T access
+        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, dX, nbf, c->d_mat0.as<double>(), nbf, 0.0,
+                        c->d_mat2.as<double>(), nbf));
+        YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
     }
-    // C = X V: with V column-major in d_mat0, V(k,n) = d_mat0[n*nbf+k] = op_T access
-    YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, dX, nbf, c->d_mat0.as<double>(), nbf, 0.0,
-                    c->d_mat2.as<double>(), nbf));
-    YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
     YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, c->d_Cocc.as<double>(), n_occ,
                     0.0, c->d_D.as<double>(), nbf));
     int info = 0;
@@ -739,7 +788,7 @@ try {
     YG_HIP(hipMemcpyAsync(Cocc, c->d_Cocc.p, (size_t) nbf * n_occ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     if (info != 0) {
-        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) info=%d", nbf, info);
+        ygpu_set_error(YGPU_ERR_SOLVER, "symmetric eigensolver (n=%d) info=%d", nbf, info);
         return YGPU_ERR_SOLVER;
     }
     if (ev_idx)

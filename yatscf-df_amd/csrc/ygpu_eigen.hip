@@ -1,0 +1,394 @@
+// ygpu_eigen.hip -- the eigenproblem of the SCF density build for small n.
+//
+// build_density.c:54 diagonalises F' = X^T F X (dsyev) every iteration.  For
+// n <= YG_EIG_NMAX_SMALL the device does it as
+//   1. k_sytrd_reg: Householder reduction to tridiagonal form, F' = Q T Q^T, by one
+//      workgroup of 512 threads with the whole lower triangle resident in registers
+//      (LAPACK dsytd2 steps, lower variant: reflector k annihilates column k below the
+//      subdiagonal and the trailing block takes the symmetric rank-2 update
+//      A -= v w^T + w v^T).  The n - 2 dependent steps then touch no memory but LDS:
+//      a step is one symmetric matrix-vector product and one rank-2 update on
+//      registers, three workgroup barriers, and the reflector written out for step 3;
+//   2. rocSOLVER dstedc on T (divide and conquer, all eigenpairs of T);
+//   3. k_backtransform_rows: u = Q z = H_0 (H_1 (... H_{n-3} z)) only for the
+//      eigenvectors the density needs (the n_occ lowest), one wavefront each.
+// rocSOLVER's dsyevd spends most of its time at these sizes in the
+// latency-bound panel kernels of its own tridiagonal reduction.
+//
+// Register layout of k_sytrd_reg: thread (a, b), a < 16 (row class), b < 32 (column
+// class), owns the elements (i, j) = (a + 16 ri, b + 32 rj) for the 81 slot pairs
+// ri < 17, rj < 9, ri >= 2 rj, which covers the lower triangle of n <= 272 exactly
+// once.  Slots with ri = 2 rj or ri = 2 rj + 1 straddle the diagonal: depending on
+// (a, b) they hold a strictly lower element, a diagonal one, or an upper one.  Upper
+// slots hold 0 and stay 0; diagonal elements are held halved, so that the symmetric
+// product (row i collects A(i, j) v_j, column j collects A(i, j) v_i over every held
+// element) needs no masks.  Elements outside n hold 0 and stay 0 (the reflector and w
+// vanish there).  Columns j <= k are finished after step k: the steps run in five phases
+// whose arithmetic skips the finished blocks of 32 columns (phase R = 0, 2, 4, 6, 8 covers
+// 32 R - 1 <= k < 32 R + 63 and touches blocks rj >= R only).  Fewer, wider phases keep the
+// 81 slots and the phases' shared LDS addresses inside 256 VGPRs without spills.
+#include "ygpu_internal.h"
+
+namespace {
+
+constexpr int RTHR = 512;        // tridiagonal reduction: one workgroup
+constexpr int NRI = 17, NRJ = 9; // row / column slots per thread
+constexpr int NSLOT = 81;
+constexpr int NREG = 288;        // LDS vector length: row indices < 16 * 17, column indices < 32 * 9
+constexpr int SRS = 33, SCS = 17;// LDS row strides of the matrix-vector partial sums
+static_assert(YG_EIG_NMAX_SMALL <= 16 * NRI, "register tiling covers n <= 272");
+
+__device__ constexpr int slot_of(int ri, int rj) { return rj * NRI - rj * (rj - 1) + (ri - 2 * rj); }
+
+// DPP move of a double within rows of 16 lanes (full permutations only: every lane has a source)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// sum over the wavefront, the same bits in every lane: butterflies within each row of 16
+// (xor 1, xor 2, half-row mirror, row mirror; each pairs equal partial sums, so all lanes
+// of a row agree), then the four row sums read out of lanes 0, 16, 32, 48
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_d<0xB1>(v);                        // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);                        // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);                       // row_half_mirror
+    v += dpp_d<0x140>(v);                       // row_mirror
+    return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+
+// xs[i] = A(i, k) for k <= i < n from the threads of column class k % 32 (slot column KRJ)
+template <int KRJ>
+__device__ __forceinline__ void extract_col(const double (&A)[NSLOT], int a, int b, int k, int n, double *xs)
+{
+    if (b != (k & 31)) return;
+    // i = a + 16 ri against k and n through per-step offsets, so that no per-slot index is
+    // kept live across steps
+    const int ka = k - a, na = n - a;
+    double *x = xs + a;
+#pragma unroll
+    for (int ri = 2 * KRJ; ri < NRI; ++ri) {
+        if (16 * ri >= ka && 16 * ri < na) x[16 * ri] = (16 * ri == ka) ? 2.0 * A[slot_of(ri, KRJ)] : A[slot_of(ri, KRJ)];
+    }
+}
+
+__device__ __forceinline__ void extract(const double (&A)[NSLOT], int a, int b, int k, int n, double *xs)
+{
+    switch (k >> 5) {                           // uniform over the workgroup
+    case 0: extract_col<0>(A, a, b, k, n, xs); break;
+    case 1: extract_col<1>(A, a, b, k, n, xs); break;
+    case 2: extract_col<2>(A, a, b, k, n, xs); break;
+    case 3: extract_col<3>(A, a, b, k, n, xs); break;
+    case 4: extract_col<4>(A, a, b, k, n, xs); break;
+    case 5: extract_col<5>(A, a, b, k, n, xs); break;
+    case 6: extract_col<6>(A, a, b, k, n, xs); break;
+    case 7: extract_col<7>(A, a, b, k, n, xs); break;
+    default: extract_col<8>(A, a, b, k, n, xs); break;
+    }
+}
+
+// partial sums of A v over the column blocks RJA <= rj < RJB (rows i < 32 RJA hold no
+// element of those blocks); ACC adds the row sums to those of an earlier column range
+template <int RJA, int RJB, bool ACC>
+__device__ __forceinline__ void matvec_cols(const double (&A)[NSLOT], int a, int b, const double *vs, double *Srow,
+                                            double *Scol)
+{
+    double vb[NRJ], colp[NRJ];
+#pragma unroll
+    for (int rj = RJA; rj < RJB; ++rj) {
+        vb[rj] = vs[b + 32 * rj];
+        colp[rj] = 0.0;
+    }
+#pragma unroll
+    for (int ri = 2 * RJA; ri < NRI; ++ri) {
+        const double va = vs[a + 16 * ri];
+        double rowp = 0.0;
+#pragma unroll
+        for (int rj = RJA; rj < RJB && rj <= ri / 2; ++rj) {
+            const double x = A[slot_of(ri, rj)];
+            rowp += x * vb[rj];
+            colp[rj] += x * va;
+        }
+        double *dst = Srow + (a + 16 * ri) * SRS + b;
+        if (ACC) *dst += rowp;
+        else *dst = rowp;
+    }
+#pragma unroll
+    for (int rj = RJA; rj < RJB; ++rj) Scol[(b + 32 * rj) * SCS + a] = colp[rj];
+}
+
+// A -= v w^T + w v^T on the column blocks RJA <= rj < RJB; f0 / f1 scale the
+// diagonal-straddling slots (1 strictly lower, 1/2 diagonal, 0 upper)
+template <int RJA, int RJB>
+__device__ __forceinline__ void update_cols(double (&A)[NSLOT], int a, int b, const double *vs, const double *ps,
+                                            double c, double f0, double f1)
+{
+    double vb[NRJ], wb[NRJ];
+#pragma unroll
+    for (int rj = RJA; rj < RJB; ++rj) {
+        vb[rj] = vs[b + 32 * rj];
+        wb[rj] = ps[b + 32 * rj] + c * vb[rj];
+    }
+#pragma unroll
+    for (int ri = 2 * RJA; ri < NRI; ++ri) {
+        const double va = vs[a + 16 * ri];
+        const double wa = ps[a + 16 * ri] + c * va;
+#pragma unroll
+        for (int rj = RJA; rj < RJB && rj <= ri / 2; ++rj) {
+            const double u = va * wb[rj] + wa * vb[rj];
+            if (ri == 2 * rj) A[slot_of(ri, rj)] -= f0 * u;
+            else if (ri == 2 * rj + 1) A[slot_of(ri, rj)] -= f1 * u;
+            else A[slot_of(ri, rj)] -= u;
+        }
+    }
+}
+
+// the live column blocks R <= rj < 9 in at most two ranges, [R, 5) and [5, 9), so that at
+// most five column values of v (and of w) are held at a time
+template <int R>
+__device__ __forceinline__ void matvec_part(const double (&A)[NSLOT], int a, int b, const double *vs, double *Srow,
+                                            double *Scol)
+{
+    if constexpr (R < 5) {
+        matvec_cols<R, 5, false>(A, a, b, vs, Srow, Scol);
+        matvec_cols<5, NRJ, true>(A, a, b, vs, Srow, Scol);
+    } else {
+        matvec_cols<R, NRJ, false>(A, a, b, vs, Srow, Scol);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void update_part(double (&A)[NSLOT], int a, int b, const double *vs, const double *ps,
+                                            double c, double f0, double f1)
+{
+    if constexpr (R < 5) {
+        update_cols<R, 5>(A, a, b, vs, ps, c, f0, f1);
+        update_cols<5, NRJ>(A, a, b, vs, ps, c, f0, f1);
+    } else {
+        update_cols<R, NRJ>(A, a, b, vs, ps, c, f0, f1);
+    }
+}
+
+struct SytrdLds {
+    double xs[NREG], vs[NREG], ps[NREG];
+    double red[RTHR / 64];
+    double sc, sbeta, sdiag;
+    double Scol[NREG * SCS];                    // Scol[j][a]: column-class partial sums of (A v)_j
+    double Srow[NREG * SRS];                    // Srow[i][b]: row-class partial sums of (A v)_i
+};
+
+// steps k0 <= k < k1, all with the first live column block R = (k + 1) / 32
+template <int R>
+__device__ __forceinline__ void sytrd_phase(double (&A)[NSLOT], SytrdLds &L, int k0, int k1, int n, double *Ag,
+                                            double *d, double *e, double *tau, double f0, double f1)
+{
+    const int t = threadIdx.x, a = t >> 5, b = t & 31, lane = t & 63;
+    for (int k = k0; k < k1; ++k) {
+        // column k lies in block R - 1 (first step), R or R + 1
+        if (R > 0 && (k >> 5) < R) extract_col<(R > 0 ? R - 1 : 0)>(A, a, b, k, n, L.xs);
+        else if ((k >> 5) == R) extract_col<R>(A, a, b, k, n, L.xs);
+        else extract_col<(R < NRJ - 1 ? R + 1 : R)>(A, a, b, k, n, L.xs);
+        __syncthreads();
+        if (t < 64) {
+            // dlarfg on x = A(k+1:n, k): H x = beta e_1, H = I - tau v v^T, v_0 = 1
+            const double alpha = L.xs[k + 1];
+            double ss = 0.0;
+            for (int i = k + 2 + t; i < n; i += 64) ss += L.xs[i] * L.xs[i];
+            ss = wave_sum(ss);
+            double tk = 0.0, beta = alpha, scale = 0.0;
+            if (ss > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tk = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int i = t; i < NREG; i += 64)
+                L.vs[i] = (i == k + 1) ? 1.0 : (i > k + 1 && i < n) ? L.xs[i] * scale : 0.0;
+            if (t == 0) {
+                L.sdiag = L.xs[k];
+                L.sbeta = beta;
+                L.sc = tk;
+            }
+        }
+        __syncthreads();
+        const double tk = L.sc;
+        matvec_part<R>(A, a, b, L.vs, L.Srow, L.Scol);
+        __syncthreads();
+        // p = tau A22 v on the trailing indices (0 elsewhere), and p . v
+        double pv = 0.0;
+        if (t < NREG) {
+            double p = 0.0;
+            if (t > k && t < n) {
+                double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                for (int q = 0; q < 32; q += 2) {
+                    s0 += L.Srow[t * SRS + q];
+                    s1 += L.Srow[t * SRS + q + 1];
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q += 2) {
+                    s0 += L.Scol[t * SCS + q];
+                    s1 += L.Scol[t * SCS + q + 1];
+                }
+                p = tk * (s0 + s1);
+            }
+            L.ps[t] = p;
+            pv = p * L.vs[t];
+        }
+        pv = wave_sum(pv);
+        if (lane == 0) L.red[t >> 6] = pv;
+        __syncthreads();
+        double vp = 0.0;
+#pragma unroll
+        for (int q = 0; q < RTHR / 64; ++q) vp += L.red[q];
+        const double c = -0.5 * tk * vp;        // w = p + c v
+        // the reflector and the step's scalars leave for memory here, so that the barrier
+        // of the next step (which waits for this wave's stores) finds them done
+        if (t > k && t < n) Ag[(size_t) k * n + t] = L.vs[t];
+        if (t == 0) {
+            d[k] = L.sdiag;
+            e[k] = L.sbeta;
+            tau[k] = tk;
+        }
+        update_part<R>(A, a, b, L.vs, L.ps, c, f0, f1);
+    }
+}
+
+// In place on the row-major n x n matrix A (leading dimension n; the upper triangle is
+// read, as dsyevd with fill_upper does).  Outputs: d[0..n), e[0..n-1), tau[0..n-1); the
+// reflector k (v with v_0 = 1, over indices k+1..n-1) is left in row k right of the diagonal.
+__global__ __launch_bounds__(RTHR) void k_sytrd_reg(int n, double *__restrict__ Ag, double *__restrict__ d,
+                                                    double *__restrict__ e, double *__restrict__ tau)
+{
+    __shared__ SytrdLds L;
+    const int t = threadIdx.x, a = t >> 5, b = t & 31;
+    // diagonal-straddling slots: ri = 2 rj gives i - j = a - b, ri = 2 rj + 1 gives 16 + a - b
+    const double f0 = (a > b) ? 1.0 : (a == b) ? 0.5 : 0.0;
+    const double f1 = (b < a + 16) ? 1.0 : (b == a + 16) ? 0.5 : 0.0;
+
+    double A[NSLOT];
+#pragma unroll
+    for (int rj = 0; rj < NRJ; ++rj) {
+#pragma unroll
+        for (int ri = 2 * rj; ri < NRI; ++ri) {
+            const int i = a + 16 * ri, j = b + 32 * rj;
+            double x = 0.0;
+            if (i < n && j < n && i >= j) x = Ag[(size_t) j * n + i];
+            A[slot_of(ri, rj)] = (i == j) ? 0.5 * x : x;
+        }
+    }
+    __syncthreads();                            // every load is done before row k is overwritten
+
+    // phase R: steps 32 R - 1 <= k < 32 R + 63 (and k < n - 2), R = 0, 2, 4, 6, 8
+#define YG_PHASE(R) sytrd_phase<R>(A, L, (R) == 0 ? 0 : 32 * (R) - 1, min(n - 2, 32 * (R) + 63), n, Ag, d, e, tau, f0, f1)
+    YG_PHASE(0);
+    YG_PHASE(2);
+    YG_PHASE(4);
+    YG_PHASE(6);
+    YG_PHASE(8);
+#undef YG_PHASE
+    // the last 2 x 2 block
+    if (n >= 2) {
+        extract(A, a, b, n - 2, n, L.xs);
+        __syncthreads();
+        if (t == 0) {
+            d[n - 2] = L.xs[n - 2];
+            e[n - 2] = L.xs[n - 1];
+            tau[n - 2] = 0.0;
+        }
+        __syncthreads();
+    }
+    extract(A, a, b, n - 1, n, L.xs);
+    __syncthreads();
+    if (t == 0) d[n - 1] = L.xs[n - 1];
+}
+
+constexpr int BVPL = (YG_EIG_NMAX_SMALL + 63) / 64;   // reflector elements per lane
+
+// U[j] = Q z_j for the first nvec eigenvectors of T; Z holds them column-major (z_j in
+// Z[j*n .. j*n+n), the dstedc layout), U row j receives u_j.  One wavefront per vector;
+// the next reflector is loaded while the current one is applied.
+__global__ __launch_bounds__(64) void k_backtransform_rows(int n, const double *__restrict__ A,
+                                                           const double *__restrict__ tau,
+                                                           const double *__restrict__ Z, double *__restrict__ U)
+{
+    __shared__ double r[YG_EIG_NMAX_SMALL];
+    const int lane = threadIdx.x, j = blockIdx.x;
+    for (int i = lane; i < n; i += 64) r[i] = Z[(size_t) j * n + i];
+    __syncthreads();
+    double vc[BVPL], vn[BVPL];
+    double tc = 0.0, tn = 0.0;
+    if (n >= 3) {
+        const int k = n - 3, m = n - k - 1;
+#pragma unroll
+        for (int q = 0; q < BVPL; ++q) {
+            const int i = lane + 64 * q;
+            vc[q] = (i < m) ? A[(size_t) k * n + k + 1 + i] : 0.0;
+        }
+        tc = tau[k];
+    }
+    for (int k = n - 3; k >= 0; --k) {
+        const int m = n - k - 1;
+        if (k > 0) {
+            const int kn = k - 1, mn = m + 1;
+#pragma unroll
+            for (int q = 0; q < BVPL; ++q) {
+                const int i = lane + 64 * q;
+                vn[q] = (i < mn) ? A[(size_t) kn * n + kn + 1 + i] : 0.0;
+            }
+            tn = tau[kn];
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < BVPL; ++q) {
+            const int i = lane + 64 * q;
+            if (i < m) s += vc[q] * r[k + 1 + i];
+        }
+        const double f = tc * wave_sum(s);
+#pragma unroll
+        for (int q = 0; q < BVPL; ++q) {
+            const int i = lane + 64 * q;
+            if (i < m) r[k + 1 + i] -= f * vc[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BVPL; ++q) vc[q] = vn[q];
+        tc = tn;
+    }
+    for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = r[i];
+}
+
+}  // namespace
+
+int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double *tau)
+{
+    if (n < 1 || n > YG_EIG_NMAX_SMALL) {
+        ygpu_set_error(YGPU_ERR_INVALID, "small tridiagonal reduction: n = %d outside 1..%d", n, YG_EIG_NMAX_SMALL);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(RTHR), 0, s, n, A, d, e, tau);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
+                          double *U)
+{
+    if (n < 1 || n > YG_EIG_NMAX_SMALL || nvec < 0 || nvec > n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "back-transform: n = %d, nvec = %d out of range", n, nvec);
+        return YGPU_ERR_INVALID;
+    }
+    if (nvec == 0) return YGPU_OK;
+    hipLaunchKernelGGL(k_backtransform_rows, dim3(nvec), dim3(64), 0, s, n, A, tau, Z, U);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}

@@ -30,6 +30,9 @@ void ygpu_set_error(int code, const char *fmt, ...);
         if (s_ != YGPU_OK) return s_; \
     } while (0)
 
+// largest n whose density-build eigenproblem runs on the one-workgroup tridiagonal reduction
+#define YG_EIG_NMAX_SMALL 272
+
 // growable device buffer, freed when released or destroyed (not copyable)
 struct DBuf {
     void *p = nullptr;
@@ -100,6 +103,9 @@ struct ygpu_ctx {
     int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
     int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
+
+    // small-n density eigensolver: reflector scalars and the occupied eigenvectors (rows)
+    DBuf d_tau, d_U;
 };
 
 // --------------------------------------------------------------------------
@@ -124,6 +130,13 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
 int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms);
 int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
 int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                      // A = (A + A^T) / 2
+// tridiagonal reduction of a row-major n x n symmetric matrix (upper triangle read) on one
+// workgroup (n <= YG_EIG_NMAX_SMALL); reflectors left in the rows of A
+int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double *tau);
+// U (nvec x n, rows) = Q z_j for the first nvec eigenvectors z_j of the tridiagonal matrix
+// (column-major in Z, as rocSOLVER dstedc leaves them)
+int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
+                          double *U);
 int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)

@@ -30,6 +30,18 @@ static void *xalloc(size_t bytes, double *mem)
     return p;
 }
 
+/* page-locked: the matrices copied to or from the device on every iteration */
+static void *xalloc_pinned(size_t bytes, double *mem)
+{
+    void *p = NULL;
+    if (ygpu_host_alloc(bytes, &p) != YGPU_OK || !p) {
+        fprintf(stderr, "TinySCF: page-locked allocation of %zu bytes failed: %s\n", bytes, ygpu_last_error());
+        abort();
+    }
+    *mem += (double) bytes;
+    return p;
+}
+
 static int env_int(const char *name, int def)
 {
     const char *v = getenv(name);
@@ -143,17 +155,17 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     }
     T->df_shell_bf_sind[T->df_nshells] = T->df_nbf;
 
-    T->Hcore_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->S_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->F_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->D_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->J_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->K_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->X_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->tmp_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->Cocc_mat = (double *) xalloc(DBL_SIZE * T->n_occ * T->nbasfuncs, mem);
-    T->eigval = (double *) xalloc(DBL_SIZE * T->nbasfuncs, mem);
-    T->ev_idx = (int *) xalloc(INT_SIZE * T->nbasfuncs, mem);
+    T->Hcore_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->S_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->F_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->D_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->J_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->K_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->X_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->tmp_mat = (double *) xalloc_pinned(DBL_SIZE * ms, mem);
+    T->Cocc_mat = (double *) xalloc_pinned(DBL_SIZE * T->n_occ * T->nbasfuncs, mem);
+    T->eigval = (double *) xalloc_pinned(DBL_SIZE * T->nbasfuncs, mem);
+    T->ev_idx = (int *) xalloc_pinned(INT_SIZE * T->nbasfuncs, mem);
     memset(T->J_mat, 0, DBL_SIZE * ms);
     memset(T->K_mat, 0, DBL_SIZE * ms);
 
@@ -172,7 +184,7 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->DIIS_bmax = -DBL_MAX;
 
     T->df_nbf_16 = (T->df_nbf + 15) / 16 * 16;
-    T->Jpq = (double *) xalloc(DBL_SIZE * (size_t) T->df_nbf * T->df_nbf, mem);
+    T->Jpq = (double *) xalloc_pinned(DBL_SIZE * (size_t) T->df_nbf * T->df_nbf, mem);
     T->temp_J = NULL;
     T->temp_K = NULL;
     T->pqA = NULL;
@@ -217,11 +229,12 @@ void free_TinySCF(TinySCF_t T)
     if (T->gpu) ygpu_destroy((ygpu_ctx_t *) T->gpu);
     void *ptrs[] = {T->Accum_Fock_buf, T->sp_scrval, T->bf_pair_scrval, T->df_sp_scrval, T->uniq_sp_lid,
                     T->uniq_sp_rid, T->shell_bf_sind, T->shell_bf_num, T->df_shell_bf_sind, T->df_shell_bf_num,
-                    T->bf_pair_mask, T->bf_pair_j, T->bf_pair_diag, T->bf_mask_displs, T->Hcore_mat, T->S_mat,
-                    T->F_mat, T->D_mat, T->J_mat, T->K_mat, T->X_mat, T->tmp_mat, T->Cocc_mat, T->eigval,
-                    T->ev_idx, T->F0_mat, T->R_mat, T->B_mat, T->FDS_mat, T->DIIS_rhs, T->DIIS_ipiv, T->Jpq,
-                    T->temp_J, T->temp_K, T->pqA, T->df_tensor};
+                    T->bf_pair_mask, T->bf_pair_j, T->bf_pair_diag, T->bf_mask_displs, T->F0_mat, T->R_mat,
+                    T->B_mat, T->FDS_mat, T->DIIS_rhs, T->DIIS_ipiv, T->temp_J, T->temp_K, T->pqA, T->df_tensor};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) ALIGN64B_FREE(ptrs[i]);
+    void *pinned[] = {T->Hcore_mat, T->S_mat, T->F_mat, T->D_mat, T->J_mat, T->K_mat, T->X_mat,
+                      T->tmp_mat, T->Cocc_mat, T->eigval, T->ev_idx, T->Jpq};
+    for (size_t i = 0; i < sizeof(pinned) / sizeof(pinned[0]); i++) (void) ygpu_host_free(pinned[i]);
     CMS_destroyBasisSet(T->basis);
     CMS_destroyBasisSet(T->df_basis);
     free(T);   /* the reference frees the struct itself (TinySCF_init_free.c:267) */
