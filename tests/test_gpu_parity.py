@@ -279,6 +279,32 @@ def test_density_eigensolver_sizes(ygpu, n, kind):
     assert np.max(np.abs(C @ C.T - D)) < 1e-12
 
 
+# S^-1/2 and small metrics: one-workgroup reduction + dstedc up to 272, dsyevd above
+@pytest.mark.parametrize("n", [1, 5, 64, 260, 272, 273])
+def test_sym_inv_sqrt_sizes(ygpu, n):
+    log("case: symmetric inverse square root n=%d" % n)
+    rng = np.random.RandomState(77 + n)
+    Q, _ = np.linalg.qr(rng.randn(n, n))
+    lam = np.logspace(-4, 1, n)
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    with make_engine(ygpu) as e:
+        W, mn = e.sym_inv_sqrt(A)
+    Wref = (Q / np.sqrt(lam)) @ Q.T
+    w_np, V_np = np.linalg.eigh(A)              # LAPACK's own error level on the same matrix
+    W_np = (V_np / np.sqrt(w_np)) @ V_np.T
+    res = np.max(np.abs(W @ A @ W - np.eye(n)))
+    res_np = np.max(np.abs(W_np @ A @ W_np - np.eye(n)))
+    rel = np.max(np.abs(W - Wref)) / np.abs(Wref).max()
+    rel_np = np.max(np.abs(W_np - Wref)) / np.abs(Wref).max()
+    log("  |W A W - I| %.2e (numpy %.2e)  rel |dW| %.2e (numpy %.2e)  min eig %.6e"
+        % (res, res_np, rel, rel_np, mn))
+    assert abs(mn - lam[0]) < 1e-12 * lam[-1]
+    assert np.array_equal(W, W.T)
+    assert res < 10 * res_np + 1e-12
+    assert rel < 10 * rel_np + 1e-13
+
+
 @pytest.mark.parametrize("name,a2", [("water", True), ("c10h22", True), ("c10h22", False)])
 def test_full_scf_energy(ygpu, name, a2):
     """Drop-in TinySCF engine (host C + device) against the oracle SCF; a2=False takes the

@@ -132,11 +132,27 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
     YG_TRY(c->d_evE.ensure((size_t) n * sizeof(double)));
     YG_TRY(c->d_info.ensure(sizeof(int)));
     YG_TRY(ensure_syevd_workspace(c, n, ld));
-    rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, dA, ld,
-                                         c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
-    if (st != rocblas_status_success) {
-        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", n, (int) st);
-        return YGPU_ERR_SOLVER;
+    const bool small = n <= YG_EIG_NMAX_SMALL && ld == n;
+    if (small) {
+        // A = Q T Q^T on one workgroup, T = Z diag Z^T by divide and conquer, V = Q Z: the rows
+        // of dOut receive the eigenvectors, then the rows of dA as the code below expects
+        YG_TRY(c->d_tau.ensure((size_t) n * sizeof(double)));
+        YG_TRY(yg_sytrd_small(c->stream, n, dA, c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_tau.as<double>()));
+        rocblas_status st = rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, c->d_eig.as<double>(),
+                                             c->d_evE.as<double>(), dW, n, c->d_info.as<int>());
+        if (st != rocblas_status_success) {
+            ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", n, (int) st);
+            return YGPU_ERR_SOLVER;
+        }
+        YG_TRY(yg_backtransform_rows(c->stream, n, n, dA, c->d_tau.as<double>(), dW, dOut));
+        YG_HIP(hipMemcpyAsync(dA, dOut, (size_t) n * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, dA, ld,
+                                             c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
+        if (st != rocblas_status_success) {
+            ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) returned %d", n, (int) st);
+            return YGPU_ERR_SOLVER;
+        }
     }
     int info = 0;
     std::vector<double> ev(n);
@@ -144,7 +160,7 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
     YG_HIP(hipMemcpyAsync(ev.data(), c->d_eig.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     if (info != 0) {
-        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dsyevd(n=%d) info=%d", n, info);
+        ygpu_set_error(YGPU_ERR_SOLVER, "symmetric eigensolver (n=%d) info=%d", n, info);
         return YGPU_ERR_SOLVER;
     }
     if (min_eig) *min_eig = ev[0];
@@ -749,18 +765,18 @@ try {
     }
     YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
     if (nbf <= YG_EIG_NMAX_SMALL) {
-        // F' = Q T Q^T on one workgroup, T = Z diag Z^T by divide and conquer, then only the
-        // n_occ lowest eigenvectors are carried back: V_occ^T = (Q Z_occ)^T in d_U (rows)
+        // F' = Q T Q^T on one workgroup; all eigenvalues of T by multisection and its n_occ
+        // lowest eigenvectors by inverse iteration; only those are carried back:
+        // V_occ^T = (Q Z_occ)^T in d_U (rows)
+        const size_t nvp = (size_t) (n_occ + 63) / 64 * 64;
         YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
         YG_TRY(c->d_U.ensure((size_t) nbf * n_occ * sizeof(double)));
-        YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), c->d_eig.as<double>(), c->d_evE.as<double>(),
-                              c->d_tau.as<double>()));
-        rocblas_status st = rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, nbf, c->d_eig.as<double>(),
-                                             c->d_evE.as<double>(), c->d_mat2.as<double>(), nbf, c->d_info.as<int>());
-        if (st != rocblas_status_success) {
-            ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", nbf, (int) st);
-            return YGPU_ERR_SOLVER;
-        }
+        YG_TRY(c->d_tri.ensure((3 * nvp + 2) * nbf * sizeof(double)));
+        double *tri_ws = c->d_tri.as<double>();
+        double *dT = tri_ws + 3 * nvp * nbf, *eT = dT + nbf;
+        YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), dT, eT, c->d_tau.as<double>()));
+        YG_TRY(yg_tridiag_eig(c->stream, nbf, n_occ, dT, eT, c->d_eig.as<double>(), c->d_mat2.as<double>(), tri_ws));
+        YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
         YG_TRY(yg_backtransform_rows(c->stream, nbf, n_occ, c->d_mat0.as<double>(), c->d_tau.as<double>(),
                                      c->d_mat2.as<double>(), c->d_U.as<double>()));
         // C_occ = X V_occ

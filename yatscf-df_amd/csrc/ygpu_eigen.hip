@@ -367,6 +367,265 @@ __global__ __launch_bounds__(64) void k_backtransform_rows(int n, const double *
     for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = r[i];
 }
 
+// ---------------------------------------------------------------------------------
+// Eigenpairs of the symmetric tridiagonal T = tridiag(e, d, e) (n <= YG_EIG_NMAX_SMALL):
+// all eigenvalues by Sturm-count multisection, the lowest nvec eigenvectors by inverse
+// iteration, orthogonalised within clusters.  The steps follow LAPACK's dstebz (Sturm
+// counts with the pivmin guard, Gershgorin start interval) and dstein (shift separation
+// 10 eps |lambda| inside runs of equal eigenvalues, clusters of eigenvalues closer than
+// 1e-3 ||T||, Gram-Schmidt inside a cluster); the parallel form is this file's own: 256
+// trial points per workgroup and round, one lane per vector in the inverse iteration.
+
+constexpr int NBIS = 256;        // bisection: one workgroup per eigenvalue
+constexpr int NORTH = 256;       // cluster orthogonalisation: one workgroup
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    return v;
+}
+
+struct TriBounds {
+    double gl, gu, tnorm, pivmin;
+};
+
+// Gershgorin interval (widened as dstebz does), ||T|| and the Sturm pivot floor, by one
+// wavefront (every lane returns them)
+__device__ TriBounds tri_bounds(int n, const double *d, const double *e)
+{
+    const int lane = threadIdx.x & 63;
+    double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const double el = (i > 0) ? fabs(e[i - 1]) : 0.0, er = (i < n - 1) ? fabs(e[i]) : 0.0;
+        gl = fmin(gl, d[i] - el - er);
+        gu = fmax(gu, d[i] + el + er);
+        emax2 = fmax(emax2, er * er);
+    }
+    TriBounds b;
+    b.gl = wave_min(gl);
+    b.gu = wave_max(gu);
+    b.tnorm = fmax(fabs(b.gl), fabs(b.gu));
+    b.pivmin = DBL_MIN * fmax(1.0, wave_max(emax2));
+    const double widen = 2.0 * b.tnorm * DBL_EPSILON * n + 2.0 * b.pivmin;
+    b.gl -= widen;
+    b.gu += widen;
+    return b;
+}
+
+// 1/q to full double precision: the hardware reciprocal and two Newton steps (the
+// recurrence below waits on this chain, which is half as long as IEEE division's)
+__device__ __forceinline__ double recip(double q)
+{
+    double r = __builtin_amdgcn_rcp(q);
+    double t = fma(-q, r, 1.0);
+    r = fma(r, t, r);
+    t = fma(-q, r, 1.0);
+    return fma(r, t, r);
+}
+
+// number of eigenvalues of T below x (|q| >= pivmin keeps 1/q finite)
+__device__ __forceinline__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin)
+{
+    double q = d[0] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+    for (int i = 1; i < n; ++i) {
+        q = (d[i] - x) - e2[i - 1] * recip(q);
+        if (fabs(q) <= pivmin) q = -pivmin;
+        cnt += q < 0.0;
+    }
+    return cnt;
+}
+
+// w[j], j < n, ascending: workgroup j narrows [lo, hi] around lambda_j by testing the
+// 256 interior points lo + (hi - lo) (k + 1) / 257 per round, one per thread
+__global__ __launch_bounds__(NBIS) void k_tridiag_bisect(int n, const double *__restrict__ d,
+                                                         const double *__restrict__ e, double *__restrict__ w)
+{
+    __shared__ double sd[YG_EIG_NMAX_SMALL], se2[YG_EIG_NMAX_SMALL];
+    __shared__ int below[2][NBIS / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < n; i += NBIS) {
+        sd[i] = d[i];
+        se2[i] = (i < n - 1) ? e[i] * e[i] : 0.0;
+    }
+    __syncthreads();
+    const TriBounds bd = tri_bounds(n, sd, e);
+    const int j = blockIdx.x;
+    double lo = bd.gl, hi = bd.gu;
+    for (int r = 0; r < 16; ++r) {              // 257^7 > 1e16: converged after ~7 rounds
+        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + DBL_MIN) break;   // uniform
+        const double x = lo + (hi - lo) * ((double) (t + 1) / (double) (NBIS + 1));
+        const int c = sturm_count(n, sd, se2, x, bd.pivmin);
+        const int Lw = __popcll(__ballot(c <= j));
+        if (lane == 0) below[r & 1][wv] = Lw;
+        __syncthreads();                        // double-buffered: one barrier per round
+        int L = 0;
+#pragma unroll
+        for (int q = 0; q < NBIS / 64; ++q) L += below[r & 1][q];   // points at or below lambda_j: 0..L-1
+        const double nlo = (L > 0) ? lo + (hi - lo) * ((double) L / (double) (NBIS + 1)) : lo;
+        const double nhi = (L < NBIS) ? lo + (hi - lo) * ((double) (L + 1) / (double) (NBIS + 1)) : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    if (t == 0) w[j] = 0.5 * (lo + hi);
+}
+
+__device__ __forceinline__ double start_value(int i, int j)
+{
+    unsigned h = (unsigned) i * 2654435761u ^ ((unsigned) j * 40503u + 12345u) * 2246822519u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    return (double) h * (2.0 / 4294967296.0) - 1.0;
+}
+
+// Inverse iteration, lane j of the grid on eigenvalue j (lanes past nvec repeat the last
+// one; their columns are discarded).  T - sigma I = L D L^T without pivoting, tiny pivots
+// replaced by +-eps ||T||; three solves from a seeded start vector, each rescaled by its
+// largest element; the result normalised into column j of Zt (n x nvp).  The iterate
+// lives in LDS (one wavefront per workgroup), the factors in Lt, Dt.
+__global__ __launch_bounds__(64) void k_tridiag_invit(int n, int nvec, int nvp, const double *__restrict__ d,
+                                                      const double *__restrict__ e, const double *__restrict__ w,
+                                                      double *__restrict__ Zt, double *__restrict__ Lt,
+                                                      double *__restrict__ Dt)
+{
+    __shared__ double zs[YG_EIG_NMAX_SMALL * 64];
+    const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
+    const int jj = min(j, nvec - 1);
+    const TriBounds bd = tri_bounds(n, d, e);
+    // dstein: inside a run of (nearly) equal eigenvalues the shifts are spread by 10 eps |lambda|
+    const double pt = 10.0 * DBL_EPSILON, floor_ = bd.tnorm * DBL_EPSILON;
+    int m = 0;
+    while (jj - m - 1 >= 0 && w[jj - m] - w[jj - m - 1] < pt * fmax(fabs(w[jj - m]), floor_)) ++m;
+    const double sig = w[jj] + m * pt * fmax(fabs(w[jj]), floor_);
+    const double ptiny = DBL_EPSILON * bd.tnorm + DBL_MIN;
+    double D = d[0] - sig;
+    if (fabs(D) < ptiny) D = (D < 0.0) ? -ptiny : ptiny;
+    double id = 1.0 / D;
+    Dt[j] = id;
+    zs[lane] = start_value(0, jj);
+    for (int i = 1; i < n; ++i) {
+        const double ei = e[i - 1];
+        const double l = ei * id;
+        Lt[(size_t) (i - 1) * nvp + j] = l;
+        D = (d[i] - sig) - l * ei;
+        if (fabs(D) < ptiny) D = (D < 0.0) ? -ptiny : ptiny;
+        id = 1.0 / D;
+        Dt[(size_t) i * nvp + j] = id;
+        zs[i * 64 + lane] = start_value(i, jj);
+    }
+    __threadfence_block();                      // the factors are read back by this lane only
+    double s = 1.0;
+    for (int it = 0; it < 3; ++it) {
+        // forward: u_i = s x_i - l_{i-1} u_{i-1}, kept as u_i / D_i
+        double u = s * zs[lane];
+        zs[lane] = u * Dt[j];
+#pragma unroll 8
+        for (int i = 1; i < n; ++i) {
+            const size_t o = (size_t) i * nvp + j;
+            u = s * zs[i * 64 + lane] - Lt[o - nvp] * u;
+            zs[i * 64 + lane] = u * Dt[o];
+        }
+        // back: y_i = u_i / D_i - l_i y_{i+1}
+        double y = zs[(n - 1) * 64 + lane], amax = fabs(y);
+#pragma unroll 8
+        for (int i = n - 2; i >= 0; --i) {
+            y = zs[i * 64 + lane] - Lt[(size_t) i * nvp + j] * y;
+            zs[i * 64 + lane] = y;
+            amax = fmax(amax, fabs(y));
+        }
+        s = 1.0 / amax;
+    }
+    double ss = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double v = s * zs[i * 64 + lane];
+        ss += v * v;
+    }
+    const double f = s / sqrt(ss);
+    for (int i = 0; i < n; ++i) Zt[(size_t) i * nvp + j] = f * zs[i * 64 + lane];
+}
+
+constexpr int CLMAX = 64;        // clusters up to this size are orthogonalised in LDS
+
+// Gram-Schmidt (two passes) inside each cluster of eigenvalues closer than 1e-3 ||T||:
+// workgroup j works only if eigenvalue j starts a cluster, on that cluster, and writes its
+// rows Z[j'][i] = Zt[i][j']
+__global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp, const double *__restrict__ d,
+                                                        const double *__restrict__ e, const double *__restrict__ w,
+                                                        double *__restrict__ Zt, double *__restrict__ Z)
+{
+    __shared__ double cs[CLMAX * YG_EIG_NMAX_SMALL];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int j0 = blockIdx.x;
+    const TriBounds bd = tri_bounds(n, d, e);
+    const double ortol = 1e-3 * bd.tnorm;
+    if (j0 > 0 && w[j0] - w[j0 - 1] < ortol) return;   // not a cluster start (uniform)
+    int j1 = j0 + 1;
+    while (j1 < nvec && w[j1] - w[j1 - 1] < ortol) ++j1;
+    const int csz = j1 - j0;
+    if (csz <= CLMAX) {
+        for (long idx = t; idx < (long) csz * n; idx += NORTH) {
+            const int v = (int) (idx / n), i = (int) (idx % n);
+            cs[idx] = Zt[(size_t) i * nvp + j0 + v];
+        }
+        __syncthreads();
+        if (wv == 0) {
+            for (int v = 1; v < csz; ++v) {
+                double *zv = cs + (size_t) v * n;
+                for (int pass = 0; pass < 2; ++pass) {
+                    for (int k = 0; k < v; ++k) {
+                        const double *zk = cs + (size_t) k * n;
+                        double dot = 0.0;
+                        for (int i = lane; i < n; i += 64) dot += zv[i] * zk[i];
+                        dot = wave_sum(dot);
+                        for (int i = lane; i < n; i += 64) zv[i] -= dot * zk[i];
+                    }
+                }
+                double ss = 0.0;
+                for (int i = lane; i < n; i += 64) ss += zv[i] * zv[i];
+                const double f = 1.0 / sqrt(wave_sum(ss));
+                for (int i = lane; i < n; i += 64) zv[i] *= f;
+            }
+        }
+        __syncthreads();
+        for (long idx = t; idx < (long) csz * n; idx += NORTH) Z[(size_t) j0 * n + idx] = cs[idx];
+        return;
+    }
+    // a larger cluster: the same in place in Zt by wavefront 0, visible to itself through fences
+    if (wv == 0) {
+        for (int j = j0 + 1; j < j1; ++j) {
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int k = j0; k < j; ++k) {
+                    double dot = 0.0;
+                    for (int i = lane; i < n; i += 64) dot += Zt[(size_t) i * nvp + j] * Zt[(size_t) i * nvp + k];
+                    dot = wave_sum(dot);
+                    for (int i = lane; i < n; i += 64) Zt[(size_t) i * nvp + j] -= dot * Zt[(size_t) i * nvp + k];
+                    __threadfence();
+                }
+            }
+            double ss = 0.0;
+            for (int i = lane; i < n; i += 64) ss += Zt[(size_t) i * nvp + j] * Zt[(size_t) i * nvp + j];
+            const double f = 1.0 / sqrt(wave_sum(ss));
+            for (int i = lane; i < n; i += 64) Zt[(size_t) i * nvp + j] *= f;
+            __threadfence();
+        }
+    }
+    __syncthreads();
+    for (long idx = t; idx < (long) csz * n; idx += NORTH) {
+        const int v = (int) (idx / n), i = (int) (idx % n);
+        Z[(size_t) (j0 + v) * n + i] = Zt[(size_t) i * nvp + j0 + v];
+    }
+}
+
 }  // namespace
 
 int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double *tau)
@@ -389,6 +648,24 @@ int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const
     }
     if (nvec == 0) return YGPU_OK;
     hipLaunchKernelGGL(k_backtransform_rows, dim3(nvec), dim3(64), 0, s, n, A, tau, Z, U);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double *e, double *w, double *Z,
+                   double *ws)
+{
+    if (n < 1 || n > YG_EIG_NMAX_SMALL || nvec < 1 || nvec > n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "tridiagonal eigensolver: n = %d, nvec = %d out of range", n, nvec);
+        return YGPU_ERR_INVALID;
+    }
+    const int nvp = (nvec + 63) / 64 * 64;
+    double *Zt = ws, *Lt = ws + (size_t) n * nvp, *Dt = ws + 2 * (size_t) n * nvp;
+    hipLaunchKernelGGL(k_tridiag_bisect, dim3(n), dim3(NBIS), 0, s, n, d, e, w);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tridiag_invit, dim3(nvp / 64), dim3(64), 0, s, n, nvec, nvp, d, e, w, Zt, Lt, Dt);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tridiag_orth, dim3(nvec), dim3(NORTH), 0, s, n, nvec, nvp, d, e, w, Zt, Z);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -105,7 +105,7 @@ struct ygpu_ctx {
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
 
     // small-n density eigensolver: reflector scalars and the occupied eigenvectors (rows)
-    DBuf d_tau, d_U;
+    DBuf d_tau, d_U, d_tri;
 };
 
 // --------------------------------------------------------------------------
@@ -137,6 +137,11 @@ int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double
 // (column-major in Z, as rocSOLVER dstedc leaves them)
 int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
                           double *U);
+// all eigenvalues w (ascending) of the symmetric tridiagonal tridiag(e, d, e) and the
+// eigenvectors of the nvec lowest as the rows of Z (nvec x n); ws holds
+// 3 n ((nvec + 63) / 64 * 64) doubles
+int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double *e, double *w, double *Z,
+                   double *ws);
 int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)
