@@ -101,16 +101,18 @@ __device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&r
 // Core: accumulate a 64x64 tile over k in [kbeg, kend)
 template <int MA, int MB>
 __device__ __forceinline__ void tile_mma(const Operand &A, const Operand &B, int M, int N, int K, int m0, int n0,
-                                         int kbeg, int kend, v4d (&acc)[2][2])
+                                         int kbeg, int kend, v4d (&acc)[2][2], bool zero = true)
 {
     __shared__ double As[BK][BM + PAD];
     __shared__ double Bs[BK][BN + PAD];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    if (zero) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
     double ra[4], rb[4];
     if (kbeg < kend) {
         load_a<MA>(A, M, kend, m0, kbeg, ra);
@@ -173,19 +175,37 @@ __global__ __launch_bounds__(NTHR) void k_gemm(Operand A, Operand B, int M, int 
     });
 }
 
-// X_i = Cocc[partners]^T * B[up rows], one z-slice per basis function i
+constexpr int XKCH64 = 256;      // gather indices staged in LDS per k chunk
+
+// X_i = Cocc[partners]^T * B[up rows]: workgroup t takes column tile t % ncolt, row tile
+// (t / ncolt) % nrowt of basis function i = t / (nrowt ncolt), so consecutive workgroups read
+// neighbouring segments of the same B rows (measured faster than grouping the row tiles of
+// one B panel on an XCD).  The gather indices of each k chunk are staged in LDS: a stage's
+// loads then wait on one memory latency, not on an index load followed by a data load.
 __global__ __launch_bounds__(NTHR) void k_xbuild(const int *__restrict__ xptr, const int *__restrict__ xup,
                                                  const int *__restrict__ xpartner, const double *__restrict__ Cocc,
                                                  const double *__restrict__ Bup, double *__restrict__ X, int n_occ,
-                                                 int naux_pad)
+                                                 int naux_pad, int ncolt, int nrowt)
 {
-    const int i = blockIdx.z;
+    const long t = blockIdx.x;
+    const int ct = (int) (t % ncolt), rt = (int) ((t / ncolt) % nrowt), i = (int) (t / ((long) nrowt * ncolt));
     const int kb = xptr[i], klen = xptr[i + 1] - kb;
-    Operand A{Cocc, n_occ, xpartner + kb};
-    Operand B{Bup, naux_pad, xup + kb};
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int m0 = rt * BM, n0 = ct * BN;
+    __shared__ int s_pa[XKCH64], s_up[XKCH64];
     v4d acc[2][2];
-    tile_mma<MODE_GATHER_T, MODE_GATHER_N>(A, B, n_occ, naux_pad, klen, m0, n0, 0, klen, acc);
+    for (int kc = 0; kc < klen || kc == 0; kc += XKCH64) {
+        const int kce = min(klen, kc + XKCH64);
+        if (kc > 0) __syncthreads();            // the previous chunk's loads are done with the indices
+        for (int k = kc + (int) threadIdx.x; k < kce; k += NTHR) {
+            s_pa[k - kc] = xpartner[kb + k];
+            s_up[k - kc] = xup[kb + k];
+        }
+        __syncthreads();
+        Operand A{Cocc, n_occ, s_pa};
+        Operand B{Bup, naux_pad, s_up};
+        tile_mma<MODE_GATHER_T, MODE_GATHER_N>(A, B, n_occ, naux_pad, kce - kc, m0, n0, 0, kce - kc, acc, kc == 0);
+        if (kce >= klen) break;
+    }
     double *Xi = X + (size_t) i * n_occ * naux_pad;
     tile_epilogue(m0, n0, acc, [&](int m, int n, double v) {
         if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = v;
@@ -400,8 +420,14 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
 {
     (void) max_row_len;
     if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
-    dim3 grid((naux_pad + BN - 1) / BN, (n_occ + BM - 1) / BM, nbf);
-    hipLaunchKernelGGL(k_xbuild, grid, dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X, n_occ, naux_pad);
+    const int ncolt = (naux_pad + BN - 1) / BN, nrowt = (n_occ + BM - 1) / BM;
+    const long total = (long) ncolt * nrowt * nbf;
+    if (total > 0x7fffffffL) {
+        ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_xbuild, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X, n_occ,
+                       naux_pad, ncolt, nrowt);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -39,6 +39,7 @@ template <int BK> struct Stage {
 };
 constexpr int SBK = 64;   // SYRK stage depth (4 wavefronts have the registers for it)
 constexpr int XBK = 32;   // X-build stage depth (12-16 wavefronts per workgroup)
+constexpr int XNBUF = 1;  // X-build LDS stages (two measured no faster: the loads, not the split, set the pace)
 
 __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 {
@@ -54,23 +55,30 @@ __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 // Loader contract: coords(idx, row, kv) names the tile row and 8-group of the
 // idx-th (row, kv) slot (idx < tile rows * KV); load(row, k8, kend, v) fills the
 // 8 values k8 .. k8+7 of that row (zero beyond kend or outside the matrix).
-template <int WM, int WN, int BK, class LA, class LB>
-__device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2])
+// NBUF = 1: one LDS stage, split and MFMA phases separated by two barriers per stage.
+// NBUF = 2: two LDS stages; stage s + 1 is split into one buffer while the MFMAs of
+// stage s read the other, with one barrier per stage, so the split of some wavefronts
+// overlaps the matrix-core work of others.
+template <int WM, int WN, int BK, int NBUF, class LA, class LB>
+__device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2],
+                                         bool zero = true)
 {
     constexpr int KV = Stage<BK>::KV, KVP = Stage<BK>::KVP;
     constexpr int NT = 64 * WM * WN;
     constexpr int BMt = 64 * WM, BNt = 64 * WN;
     constexpr int SA = BMt * KV, SB = BNt * KV;                    // (row, 8-group) slots per stage
     constexpr int GA = (SA + NT - 1) / NT, GB = (SB + NT - 1) / NT;
-    __shared__ v8bf Ah[BMt][KVP], Al[BMt][KVP], Bh[BNt][KVP], Bl[BNt][KVP];
+    __shared__ v8bf Ah[NBUF][BMt][KVP], Al[NBUF][BMt][KVP], Bh[NBUF][BNt][KVP], Bl[NBUF][BNt][KVP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w / WN) * 64, wn = (w % WN) * 64;
+    if (zero) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+                for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    }
     // slot g of this thread is idx = threadIdx.x + NT g; slots beyond SA / SB are idle
     int rowa[GA], kva[GA], rowb[GB], kvb[GB];
     bool ina[GA], inb[GB];
@@ -87,41 +95,33 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
         lb.coords(inb[g] ? idx : 0, rowb[g], kvb[g]);
     }
     float ra[GA][8], rb[GB][8];
-    if (kbeg < kend) {
+    auto load_stage = [&](int k0) {
 #pragma unroll
         for (int g = 0; g < GA; ++g)
-            if (ina[g]) la.load(rowa[g], kbeg + 8 * kva[g], kend, ra[g]);
+            if (ina[g]) la.load(rowa[g], k0 + 8 * kva[g], kend, ra[g]);
 #pragma unroll
         for (int g = 0; g < GB; ++g)
-            if (inb[g]) lb.load(rowb[g], kbeg + 8 * kvb[g], kend, rb[g]);
-    }
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        __syncthreads();
+            if (inb[g]) lb.load(rowb[g], k0 + 8 * kvb[g], kend, rb[g]);
+    };
+    auto split_stage = [&](int buf) {
 #pragma unroll
         for (int g = 0; g < GA; ++g)
-            if (ina[g]) split8(ra[g], Ah[rowa[g]][kva[g]], Al[rowa[g]][kva[g]]);
+            if (ina[g]) split8(ra[g], Ah[buf][rowa[g]][kva[g]], Al[buf][rowa[g]][kva[g]]);
 #pragma unroll
         for (int g = 0; g < GB; ++g)
-            if (inb[g]) split8(rb[g], Bh[rowb[g]][kvb[g]], Bl[rowb[g]][kvb[g]]);
-        __syncthreads();
-        if (k0 + BK < kend) {
-#pragma unroll
-            for (int g = 0; g < GA; ++g)
-                if (ina[g]) la.load(rowa[g], k0 + BK + 8 * kva[g], kend, ra[g]);
-#pragma unroll
-            for (int g = 0; g < GB; ++g)
-                if (inb[g]) lb.load(rowb[g], k0 + BK + 8 * kvb[g], kend, rb[g]);
-        }
+            if (inb[g]) split8(rb[g], Bh[buf][rowb[g]][kvb[g]], Bl[buf][rowb[g]][kvb[g]]);
+    };
+    auto mfma_stage = [&](int buf) {
 #pragma unroll
         for (int s = 0; s < BK / 16; ++s) {
             const int kv = 2 * s + (lane >> 5), r = lane & 31;
             v8bf ah[2], al[2], bh[2], bl[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                ah[i] = Ah[wm + 32 * i + r][kv];
-                al[i] = Al[wm + 32 * i + r][kv];
-                bh[i] = Bh[wn + 32 * i + r][kv];
-                bl[i] = Bl[wn + 32 * i + r][kv];
+                ah[i] = Ah[buf][wm + 32 * i + r][kv];
+                al[i] = Al[buf][wm + 32 * i + r][kv];
+                bh[i] = Bh[buf][wn + 32 * i + r][kv];
+                bl[i] = Bl[buf][wn + 32 * i + r][kv];
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -131,6 +131,32 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 }
+        }
+    };
+    if (kbeg >= kend) return;
+    load_stage(kbeg);
+    if constexpr (NBUF == 1) {
+        for (int k0 = kbeg; k0 < kend; k0 += BK) {
+            __syncthreads();
+            split_stage(0);
+            __syncthreads();
+            if (k0 + BK < kend) load_stage(k0 + BK);
+            mfma_stage(0);
+        }
+    } else {
+        split_stage(0);
+        if (kbeg + BK < kend) load_stage(kbeg + BK);
+        __syncthreads();
+        int cur = 0;
+        for (int k0 = kbeg; k0 < kend; k0 += BK) {
+            // buffer cur ^ 1 was last read by the MFMAs before the previous barrier
+            if (k0 + BK < kend) {
+                split_stage(cur ^ 1);
+                if (k0 + 2 * BK < kend) load_stage(k0 + 2 * BK);
+            }
+            mfma_stage(cur);
+            __syncthreads();
+            cur ^= 1;
         }
     }
 }
@@ -158,34 +184,38 @@ __device__ __forceinline__ long xcd_remap(long b, long total)
     return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
-// ---- X build: A(m,k) = Cocc32[partner[k]][m], B(k,n) = B32[up[k]][n]
+// ---- X build: A(m,k) = Cocc32[partner[k]][m], B(k,n) = B32[up[k]][n]; the gather
+// indices of the current k chunk sit in LDS (pa[k - koff], up[k - koff]), so a stage's data
+// loads wait on one memory latency, not on an index load followed by a data load
 template <int BMt>
 struct XLoadA {
-    const float *cocc; const int *partner; int n_occ, m0;
+    const float *cocc; const int *pa; int koff, n_occ, m0;
     __device__ void coords(int idx, int &row, int &kv) const { row = idx % BMt; kv = idx / BMt; }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
         const int m = m0 + row;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int k = k8 + e;
-            v[e] = (k < kend && m < n_occ) ? cocc[(size_t) partner[k] * n_occ + m] : 0.f;
+            v[e] = (k < kend && m < n_occ) ? cocc[(size_t) pa[k - koff] * n_occ + m] : 0.f;
         }
     }
 };
 
 template <int BNt>
 struct XLoadB {
-    const float *b32; const int *up; long ld; int n0;
+    const float *b32; const int *up; int koff; long ld; int n0;
     __device__ void coords(int idx, int &row, int &kv) const { row = idx % BNt; kv = idx / BNt; }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
         const long n = n0 + row;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int k = k8 + e;
-            v[e] = (k < kend && n < ld) ? b32[(size_t) up[k] * ld + n] : 0.f;
+            v[e] = (k < kend && n < ld) ? b32[(size_t) up[k - koff] * ld + n] : 0.f;
         }
     }
 };
+
+constexpr int XKCH = 256;    // gather indices staged in LDS per k chunk (one index latency per 8 stages)
 
 // X build: (64 WM occupied rows) x (64 WN aux columns) per workgroup of WM x WN wavefronts
 template <int WM, int WN>
@@ -196,14 +226,27 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
                                                              const float *__restrict__ B32, float *__restrict__ X,
                                                              int n_occ, int naux_pad, int ncolt, int nrowt, long total)
 {
-    const long t = xcd_remap(blockIdx.x, total);
-    const int rt = (int) (t % nrowt), ct = (int) ((t / nrowt) % ncolt), i = (int) (t / ((long) nrowt * ncolt));
+    // column tiles fastest: consecutive workgroups read neighbouring segments of the same B
+    // rows (measured faster than grouping the row tiles of one panel on an XCD)
+    const long t = blockIdx.x;
+    const int ct = (int) (t % ncolt), rt = (int) ((t / ncolt) % nrowt), i = (int) (t / ((long) nrowt * ncolt));
     const int kb = xptr[i], klen = xptr[i + 1] - kb;
     const int m0 = rt * 64 * WM, n0 = ct * 64 * WN;
-    XLoadA<64 * WM> la{Cocc32, xpartner + kb, n_occ, m0};
-    XLoadB<64 * WN> lb{B32, xup + kb, naux_pad, n0};
+    __shared__ int s_pa[XKCH], s_up[XKCH];
     v16f acc[2][2];
-    bf3_tile<WM, WN, XBK>(la, lb, 0, klen, acc);
+    for (int kc = 0; kc < klen || kc == 0; kc += XKCH) {
+        const int kce = min(klen, kc + XKCH);
+        if (kc > 0) __syncthreads();            // the previous chunk's loads are done with the indices
+        for (int k = kc + (int) threadIdx.x; k < kce; k += 64 * WM * WN) {
+            s_pa[k - kc] = xpartner[kb + k];
+            s_up[k - kc] = xup[kb + k];
+        }
+        __syncthreads();
+        XLoadA<64 * WM> la{Cocc32, s_pa, kc, n_occ, m0};
+        XLoadB<64 * WN> lb{B32, s_up, kc, naux_pad, n0};
+        bf3_tile<WM, WN, XBK, XNBUF>(la, lb, kc, kce, acc, kc == 0);
+        if (kce >= klen) break;
+    }
     float *Xi = X + (size_t) i * n_occ * naux_pad;
     bf3_epilogue<WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
@@ -250,7 +293,7 @@ __global__ __launch_bounds__(NTHR) void k_syrk_bf3(const float *__restrict__ X, 
     SLoad<SBK> la{X + kb, kdim, nbf, bi * ST};
     SLoad<SBK> lb{X + kb, kdim, nbf, bj * ST};
     v16f acc[2][2];
-    bf3_tile<2, 2, SBK>(la, lb, 0, (int) (ke - kb), acc);
+    bf3_tile<2, 2, SBK, 1>(la, lb, 0, (int) (ke - kb), acc);
     float *P = part + (size_t) z * nbf * nbf;
     const int m0 = bi * ST, n0 = bj * ST;
     bf3_epilogue<2>(acc, [&](int r, int c, float v) {
