@@ -52,6 +52,10 @@ def test_errors_without_handle():
     assert L.ygpu_set_fock_precision(None, 1) == 1
     assert L.ygpu_diis_error(None, 0, None) == 1
     assert L.ygpu_destroy(None) == 0
+    # page-locked host memory: a null result pointer is rejected, freeing NULL is a no-op
+    assert L.ygpu_host_alloc(ctypes.c_size_t(64), None) == 1
+    assert b"null result pointer" in L.ygpu_last_error()
+    assert L.ygpu_host_free(None) == 0
 
 
 def test_quicksort_eigval_carries_indices():
