@@ -253,10 +253,11 @@ def spectrum_case(n, kind, seed):
 
 
 # 272 is the largest size of the one-workgroup register-resident tridiagonal reduction;
-# 273 and 300 take rocSOLVER dsyevd
+# 273 .. 2048 take the blocked two-kernel reduction with rocSOLVER dstedc and dormtr
 @pytest.mark.parametrize("n,kind", [(1, "dense"), (2, "dense"), (3, "dense"), (17, "cluster"), (31, "dense"),
                                     (32, "dense"), (33, "diag"), (64, "cluster"), (200, "diag"), (255, "dense"),
-                                    (271, "cluster"), (272, "dense"), (273, "dense"), (300, "cluster")])
+                                    (271, "cluster"), (272, "dense"), (273, "dense"), (300, "cluster"),
+                                    (600, "cluster")])
 def test_density_eigensolver_sizes(ygpu, n, kind):
     log("case: density eigensolver n=%d %s" % (n, kind))
     F, nocc = spectrum_case(n, kind, 1000 + n)
@@ -279,8 +280,9 @@ def test_density_eigensolver_sizes(ygpu, n, kind):
     assert np.max(np.abs(C @ C.T - D)) < 1e-12
 
 
-# S^-1/2 and small metrics: one-workgroup reduction + dstedc up to 272, dsyevd above
-@pytest.mark.parametrize("n", [1, 5, 64, 260, 272, 273])
+# S^-1/2 and metrics: one-workgroup reduction + dstedc up to 272, the blocked reduction +
+# dstedc + dormtr up to 2048, dsyevd above
+@pytest.mark.parametrize("n", [1, 5, 64, 260, 272, 273, 990, 1500])
 def test_sym_inv_sqrt_sizes(ygpu, n):
     log("case: symmetric inverse square root n=%d" % n)
     rng = np.random.RandomState(77 + n)

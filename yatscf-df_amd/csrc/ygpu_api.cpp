@@ -109,6 +109,8 @@ int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
     (void) rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, nullptr, ld, nullptr, nullptr,
                             nullptr);
     (void) rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, nullptr, nullptr, nullptr, ld, nullptr);
+    (void) rocsolver_dormtr(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, nullptr, ld,
+                            nullptr, nullptr, ld);
     if (rocblas_stop_device_memory_size_query(c->rb, &need) != rocblas_status_success) {
         ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_stop_device_memory_size_query failed");
         return YGPU_ERR_SOLVER;
@@ -121,6 +123,26 @@ int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
             return YGPU_ERR_SOLVER;
         }
         c->solver_ws_bytes = need;
+    }
+    return YGPU_OK;
+}
+
+// After yg_sytrd_blocked on A (reflectors in A, d in d_eig, e in d_evE, tau in d_tau): all
+// eigenpairs of T by rocSOLVER's divide and conquer into Z (column-major, leading dimension
+// ld), then the first ncols eigenvectors of A = Q Z[:, 0:ncols] in place (dormtr).
+int solver_tridiag_to_full(ygpu_ctx *c, int n, int ld, double *A, double *Z, int ncols)
+{
+    rocblas_status st = rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, c->d_eig.as<double>(),
+                                         c->d_evE.as<double>(), Z, ld, c->d_info.as<int>());
+    if (st != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", n, (int) st);
+        return YGPU_ERR_SOLVER;
+    }
+    st = rocsolver_dormtr(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, ncols, A, ld,
+                          c->d_tau.as<double>(), Z, ld);
+    if (st != rocblas_status_success) {
+        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dormtr(n=%d) returned %d", n, (int) st);
+        return YGPU_ERR_SOLVER;
     }
     return YGPU_OK;
 }
@@ -146,6 +168,15 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
         }
         YG_TRY(yg_backtransform_rows(c->stream, n, n, dA, c->d_tau.as<double>(), dW, dOut));
         YG_HIP(hipMemcpyAsync(dA, dOut, (size_t) n * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else if (n <= YG_SYTRD_BLOCKED_NMAX) {
+        // blocked reduction (reflectors left in dA), T = Z diag Z^T by divide and conquer into
+        // dW, V = Q Z by rocSOLVER's dormtr in place, then V back into dA (column-major view)
+        YG_TRY(c->d_tau.ensure((size_t) n * sizeof(double)));
+        YG_TRY(c->d_tri.ensure(yg_sytrd_blocked_ws(n)));
+        YG_TRY(yg_sytrd_blocked(c->stream, n, ld, dA, c->d_eig.as<double>(), c->d_evE.as<double>(),
+                                c->d_tau.as<double>(), c->d_tri.as<double>()));
+        YG_TRY(solver_tridiag_to_full(c, n, ld, dA, dW, n));
+        YG_HIP(hipMemcpyAsync(dA, dW, (size_t) ld * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     } else {
         rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, dA, ld,
                                              c->d_eig.as<double>(), c->d_evE.as<double>(), c->d_info.as<int>());
@@ -781,6 +812,16 @@ try {
                                      c->d_mat2.as<double>(), c->d_U.as<double>()));
         // C_occ = X V_occ
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_U.as<double>(), nbf, 0.0,
+                        c->d_Cocc.as<double>(), n_occ));
+    } else if (nbf <= YG_SYTRD_BLOCKED_NMAX) {
+        // blocked reduction, all eigenpairs of T, Q applied to the n_occ lowest only: their
+        // columns of d_mat2 (column-major) are the rows of V_occ^T
+        YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
+        YG_TRY(c->d_tri.ensure(yg_sytrd_blocked_ws(nbf)));
+        YG_TRY(yg_sytrd_blocked(c->stream, nbf, nbf, c->d_mat0.as<double>(), c->d_eig.as<double>(),
+                                c->d_evE.as<double>(), c->d_tau.as<double>(), c->d_tri.as<double>()));
+        YG_TRY(solver_tridiag_to_full(c, nbf, nbf, c->d_mat0.as<double>(), c->d_mat2.as<double>(), n_occ));
+        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
                         c->d_Cocc.as<double>(), n_occ));
     } else {
         rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf,

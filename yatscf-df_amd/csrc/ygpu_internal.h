@@ -32,6 +32,9 @@ void ygpu_set_error(int code, const char *fmt, ...);
 
 // largest n whose density-build eigenproblem runs on the one-workgroup tridiagonal reduction
 #define YG_EIG_NMAX_SMALL 272
+// largest n of the blocked two-kernel tridiagonal reduction (two rows per thread of its
+// one-workgroup column kernel); rocSOLVER dsyevd beyond
+#define YG_SYTRD_BLOCKED_NMAX 2048
 
 // growable device buffer, freed when released or destroyed (not copyable)
 struct DBuf {
@@ -137,6 +140,12 @@ int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double
 // (column-major in Z, as rocSOLVER dstedc leaves them)
 int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
                           double *U);
+// blocked tridiagonal reduction of a row-major symmetric n x n matrix (lower triangle read,
+// 3 <= n <= YG_SYTRD_BLOCKED_NMAX, leading dimension lda) in LAPACK dsytrd's lower layout of
+// the column-major view: d, e, tau, reflectors below the subdiagonal; ws holds
+// yg_sytrd_blocked_ws(n) bytes
+size_t yg_sytrd_blocked_ws(int n);
+int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, double *e, double *tau, double *ws);
 // all eigenvalues w (ascending) of the symmetric tridiagonal tridiag(e, d, e) and the
 // eigenvectors of the nvec lowest as the rows of Z (nvec x n); ws holds
 // 3 n ((nvec + 63) / 64 * 64) doubles

@@ -1,0 +1,371 @@
+// ygpu_sytrd.hip -- blocked Householder tridiagonalisation for the mid-size symmetric
+// eigenproblems of the SCF: J^{-1/2} (build_DF_tensor.c:205-237, dsyev of the metric) and
+// the density build above the one-workgroup size (build_density.c:54, dsyev of F').
+//
+// LAPACK dsytrd / dlatrd, lower variant, panels of NB columns.  Inside a panel the trailing
+// block keeps its panel-start values A0 and the transformations so far are held as
+// A = A0 - V W^T - W V^T (V the reflectors, W = tau A v corrected); a column i is
+//   1. finished from the previous column's product y: w = tau y - (tau^2 / 2)(y . v) v,
+//   2. brought up to date, x = A0(:, i) - V W(i, :)^T - W V(i, :)^T,
+//   3. reduced by a reflector (dlarfg) that zeroes x below the subdiagonal,
+//   4. multiplied into the trailing block: y = A0 v - V (W^T v) - W (V^T v).
+// One kernel per column (k_latrd_step): each of its workgroups repeats steps 1-3 and the
+// small products W^T v, V^T v for all rows (loads issued up front, values shared through
+// LDS only), then takes its share of the step-4 GEMV rows, one wavefront per row.  After a panel the
+// trailing block takes the rank-2 NB update on FP64 MFMA (two GEMMs).  V and W are stored
+// column-major (column l contiguous over the rows), so that the column kernel's per-l loads
+// across a wavefront are coalesced.  The reflectors are
+// left where LAPACK leaves them for the lower variant (v(i+2:n) in A(i+2:n, i) of the
+// column-major view, which is row i of this row-major storage), so rocSOLVER's dormtr
+// applies Q.  The input's lower triangle (the upper triangle of the column-major view that
+// dsyevd with fill_upper reads) is mirrored first.
+#include "ygpu_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int NB = 16;           // panel width
+constexpr int CT = 1024;         // column kernel: one workgroup
+constexpr int ST = 512;          // column step: threads per workgroup (four rows each at n = 2048)
+constexpr int LGRID = 64;        // workgroups of the column step (each repeats the column work, then a
+                                 // 1/64 share of the trailing GEMV)
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// wavefront sum, the same bits in every lane (row-of-16 butterflies, then lanes 0/16/32/48)
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+
+// sum over each row of 16 lanes (every lane of the row receives it)
+__device__ __forceinline__ double row16_sum(double v)
+{
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return v;
+}
+
+// workgroup sum over NT threads, returned to all; red holds NT / 64 values
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) s += red[q];
+    __syncthreads();
+    return s;
+}
+
+// A(r, c) = A(c, r) from the lower triangle (row-major r > c) into the upper
+__global__ void k_mirror_lower(int n, long lda, double *__restrict__ A)
+{
+    const long tot = (long) n * n;
+    for (long idx = (long) blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (long) gridDim.x * blockDim.x) {
+        const int r = (int) (idx / n), c = (int) (idx % n);
+        if (c > r) A[(size_t) r * lda + c] = A[(size_t) c * lda + r];
+    }
+}
+
+// End of a panel p .. p + nbe - 1, one workgroup: W(:, nbe - 1) = tau y - (tau^2 / 2)(y . v) v
+// for the panel's last column (v = vin, y = yin over rows >= p + nbe), and every column's
+// reflector into its row of A (e on the subdiagonal, v below it).  The column kernels leave
+// row i of A alone: their other workgroups may still be reading it.
+__global__ __launch_bounds__(CT) void k_latrd_finish(int n, long lda, double *__restrict__ A, int p, int nbe,
+                                                     const double *__restrict__ V, double *__restrict__ W,
+                                                     const double *__restrict__ vin, const double *__restrict__ yin,
+                                                     const double *__restrict__ e, const double *__restrict__ tau)
+{
+    __shared__ double red[CT / 64];
+    const int t = threadIdx.x, i = p + nbe;
+    const double tp = tau[i - 1];
+    double s = 0.0;
+    for (int r = i + t; r < n; r += CT) s += yin[r] * vin[r];
+    s = block_sum<CT>(s, red);
+    const double c = -0.5 * tp * tp * s;
+    for (int r = i + t; r < n; r += CT) W[(size_t) (nbe - 1) * n + r] = tp * yin[r] + c * vin[r];
+    for (int j = 0; j < nbe; ++j) {
+        const int ic = p + j;
+        for (int r = ic + 1 + t; r < n; r += CT)
+            A[(size_t) ic * lda + r] = (r == ic + 1) ? e[ic] : V[(size_t) j * n + r];
+    }
+}
+
+// One column i = p + j of a panel, by every workgroup of the grid.  Each workgroup computes
+// the column's steps 1-3 and the products W^T v, V^T v for all rows (the same operations in
+// the same order, so every workgroup holds the same v), then its share of the trailing
+// GEMV y = A0 v - V (W^T v) - W (V^T v).  Workgroup 0 also writes W(:, j - 1), V(:, j) and
+// d, e, tau.  vin / yin are the previous column's v and y, vout / yout this column's
+// (separate buffers: other workgroups still read the inputs).  Row i of A receives the
+// reflector only at the end of the panel (k_latrd_finish).  Thread t owns rows
+// i + t + ST k (k < 4); V and W are read once per row: with v = 1 at row i + 1 and
+// v = x scale below, W^T v = scale (W^T x)_{>= i+2} + W(i + 1, :), accumulated in the same
+// pass that forms x.
+__global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double *__restrict__ A, int i, int j,
+                                                   double *__restrict__ V, double *__restrict__ W,
+                                                   const double *__restrict__ vin, const double *__restrict__ yin,
+                                                   double *__restrict__ vout, double *__restrict__ yout,
+                                                   double *__restrict__ d, double *__restrict__ e,
+                                                   double *__restrict__ tau)
+{
+    constexpr int RK = YG_SYTRD_BLOCKED_NMAX / ST;   // rows per thread
+    constexpr int NS = ST / 16;                      // row-of-16 partial sums per workgroup
+    __shared__ double red[ST / 64];
+    __shared__ double red2[NS][2 * NB];
+    __shared__ double sv[YG_SYTRD_BLOCKED_NMAX], sw[YG_SYTRD_BLOCKED_NMAX];   // v_i and W(:, j - 1)
+    __shared__ double svi[NB], swi[NB], sv1[NB], sw1[NB], sab[2 * NB];       // V, W rows i and i + 1
+    __shared__ double sx[2];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const bool lead = blockIdx.x == 0;
+    const int jp = j - 1;                       // the previous column (V(:, jp) = vin, W(:, jp) from yin)
+    // ---- loads that do not depend on this kernel's results
+    double a_ir[RK], y_r[RK], vp[RK], wp[RK];
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+        const int r = i + t + ST * k;
+        const bool in = r < n;
+        a_ir[k] = in ? A[(size_t) i * lda + r] : 0.0;
+        y_r[k] = (in && j > 0) ? yin[r] : 0.0;
+        vp[k] = (in && j > 0) ? vin[r] : 0.0;
+        wp[k] = 0.0;
+    }
+    if (t < jp) {
+        svi[t] = V[(size_t) t * n + i];
+        swi[t] = W[(size_t) t * n + i];
+    }
+    const double tp = (j > 0) ? tau[i - 1] : 0.0;
+    // ---- step 1: W(:, jp) = tau y - (tau^2 / 2)(y . v) v from the previous column
+    if (j > 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < RK; ++k) s += y_r[k] * vp[k];
+        s = block_sum<ST>(s, red);              // also publishes svi, swi
+        const double c = -0.5 * tp * tp * s;
+#pragma unroll
+        for (int k = 0; k < RK; ++k) {
+            const int r = i + t + ST * k;
+            wp[k] = tp * y_r[k] + c * vp[k];
+            if (r < n) {
+                sw[r] = wp[k];
+                if (lead) W[(size_t) jp * n + r] = wp[k];
+            }
+        }
+        if (t == 0) {
+            svi[jp] = vp[0];                    // V(i, jp) (= 1)
+            swi[jp] = wp[0];                    // W(i, jp)
+        }
+        __syncthreads();
+    }
+    // ---- step 2: x_r = A0(i, r) - sum_l (V(r, l) W(i, l) + W(r, l) V(i, l)), r >= i, and the
+    // partial sums of W^T x, V^T x over r >= i + 2
+    double x[RK], ss = 0.0;
+    double pa[NB], pb[NB];
+#pragma unroll
+    for (int l = 0; l < NB; ++l) pa[l] = pb[l] = 0.0;
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+        const int r = i + t + ST * k;
+        double v = a_ir[k];
+        if (r < n) {
+            // all of the row's panel loads in flight at once: unconditional (every column index
+            // is inside V and W), the columns l >= jp masked afterwards
+            double vl[NB - 1], wl[NB - 1];
+#pragma unroll
+            for (int l = 0; l < NB - 1; ++l) {
+                vl[l] = V[(size_t) l * n + r];
+                wl[l] = W[(size_t) l * n + r];
+            }
+#pragma unroll
+            for (int l = 0; l < NB - 1; ++l) v -= (l < jp) ? vl[l] * swi[l] + wl[l] * svi[l] : 0.0;
+            if (j > 0) v -= vp[k] * swi[jp] + wp[k] * svi[jp];
+            if (r == i) sx[0] = v;
+            if (r == i + 1) {
+                sx[1] = v;
+#pragma unroll
+                for (int l = 0; l < NB - 1; ++l) {
+                    sv1[l] = (l < jp) ? vl[l] : 0.0;
+                    sw1[l] = (l < jp) ? wl[l] : 0.0;
+                }
+                if (j > 0) {
+                    sv1[jp] = vp[k];
+                    sw1[jp] = wp[k];
+                }
+            }
+            if (r >= i + 2) {
+                ss += v * v;
+#pragma unroll
+                for (int l = 0; l < NB - 1; ++l) {
+                    pa[l] += (l < jp) ? wl[l] * v : 0.0;
+                    pb[l] += (l < jp) ? vl[l] * v : 0.0;
+                }
+#pragma unroll
+                for (int l = 0; l < NB; ++l)
+                    if (l == jp) {
+                        pa[l] += wp[k] * v;
+                        pb[l] += vp[k] * v;
+                    }
+            }
+        }
+        x[k] = v;
+    }
+    // row-of-16 sums of the partial products (their last adds go through LDS)
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+        pa[l] = row16_sum(pa[l]);
+        pb[l] = row16_sum(pb[l]);
+    }
+    if ((lane & 15) == 0) {
+        const int slot = wv * 4 + (lane >> 4);
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            red2[slot][l] = pa[l];
+            red2[slot][NB + l] = pb[l];
+        }
+    }
+    ss = block_sum<ST>(ss, red);                // its barriers also publish sx, sv1, sw1, red2
+    // ---- step 3: dlarfg
+    const double diag = sx[0], alpha = sx[1];
+    double tk = 0.0, beta = alpha, scale = 0.0;
+    if (ss > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+    }
+    double v_r[RK];
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+        const int r = i + t + ST * k;
+        v_r[k] = (r == i + 1) ? 1.0 : (r >= i + 2 && r < n) ? x[k] * scale : 0.0;
+        if (r < n) sv[r] = v_r[k];
+    }
+    if (t < 2 * NB) {
+        const int l = t % NB;
+        double s = 0.0;
+        if (l < j) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) s += red2[q][t];
+            s = scale * s + ((t < NB) ? sw1[l] : sv1[l]);
+        }
+        sab[t] = s;
+    }
+    __syncthreads();                            // sv and sab complete
+    // ---- this workgroup's rows of the trailing GEMV: wavefront g of the grid takes rows
+    // i + 1 + g, i + 1 + g + (ST / 64) G, ...
+    const int G = gridDim.x;
+    for (int r = i + 1 + blockIdx.x * (ST / 64) + wv; r < n; r += G * (ST / 64)) {
+        const double *row = A + (size_t) r * lda;
+        double s0 = 0.0, s1 = 0.0;
+        int c = i + 1 + lane;
+#pragma unroll 4
+        for (; c + 64 < n; c += 128) {
+            s0 += row[c] * sv[c];
+            s1 += row[c + 64] * sv[c + 64];
+        }
+        if (c < n) s0 += row[c] * sv[c];
+        double s = s0 + s1;
+        if (lane < j) {
+            const double vl = (lane < jp) ? V[(size_t) lane * n + r] : vin[r];
+            const double wl = (lane < jp) ? W[(size_t) lane * n + r] : sw[r];
+            s -= vl * sab[lane] + wl * sab[NB + lane];
+        }
+        s = wave_sum(s);
+        if (lane == 0) yout[r] = s;
+    }
+    // ---- workgroup 0: this column's outputs
+    if (lead) {
+#pragma unroll
+        for (int k = 0; k < RK; ++k) {
+            const int r = i + t + ST * k;
+            if (r < n) {
+                vout[r] = v_r[k];
+                if (r >= i + 1) V[(size_t) j * n + r] = v_r[k];
+            }
+        }
+        if (t == 0) {
+            d[i] = diag;
+            e[i] = beta;
+            tau[i] = tk;
+        }
+    }
+}
+
+// the trailing 2 x 2 block after the last reflector
+__global__ void k_latrd_tail(int n, long lda, double *__restrict__ A, double *__restrict__ d, double *__restrict__ e,
+                             double *__restrict__ tau)
+{
+    if (threadIdx.x != 0) return;
+    d[n - 2] = A[(size_t) (n - 2) * lda + n - 2];
+    e[n - 2] = A[(size_t) (n - 1) * lda + n - 2];
+    d[n - 1] = A[(size_t) (n - 1) * lda + n - 1];
+    tau[n - 2] = 0.0;
+    A[(size_t) (n - 2) * lda + n - 1] = e[n - 2];
+}
+
+}  // namespace
+
+size_t yg_sytrd_blocked_ws(int n) { return ((size_t) 2 * n * NB + 4 * (size_t) n) * sizeof(double); }
+
+int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, double *e, double *tau, double *ws)
+{
+    if (n < 3 || n > YG_SYTRD_BLOCKED_NMAX || lda < n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "blocked tridiagonal reduction: n = %d (3..%d), lda = %ld", n,
+                       YG_SYTRD_BLOCKED_NMAX, lda);
+        return YGPU_ERR_INVALID;
+    }
+    double *V = ws, *W = ws + (size_t) n * NB;
+    double *vb[2] = {W + (size_t) n * NB, W + (size_t) n * NB + n};
+    double *yb[2] = {vb[1] + n, vb[1] + 2 * (size_t) n};
+    hipLaunchKernelGGL(k_mirror_lower, dim3(std::min<long>(((long) n * n + 255) / 256, 4096)), dim3(256), 0, s, n,
+                       lda, A);
+    YG_HIP(hipGetLastError());
+    const int last = n - 3;                     // reflectors 0 .. n - 3
+    int cur = 0;                                // vb[cur], yb[cur]: the previous column's v and y
+    for (int p = 0; p <= last; p += NB) {
+        const int nbe = std::min(NB, last - p + 1);
+        YG_HIP(hipMemsetAsync(V, 0, (size_t) 2 * n * NB * sizeof(double), s));
+        for (int j = 0; j < nbe; ++j) {
+            const int i = p + j;
+            hipLaunchKernelGGL(k_latrd_step, dim3(LGRID), dim3(ST), 0, s, n, lda, (const double *) A, i, j, V, W,
+                               (const double *) vb[cur], (const double *) yb[cur], vb[cur ^ 1], yb[cur ^ 1], d, e,
+                               tau);
+            YG_HIP(hipGetLastError());
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_latrd_finish, dim3(1), dim3(CT), 0, s, n, lda, A, p, nbe, (const double *) V, W,
+                           (const double *) vb[cur], (const double *) yb[cur], (const double *) e,
+                           (const double *) tau);
+        YG_HIP(hipGetLastError());
+        // trailing block rows / columns >= q: A -= V W^T + W V^T
+        const int q = p + nbe, m = n - q;
+        double *C = A + (size_t) q * lda + q;
+        // V, W column-major (leading dimension n): op(A)(r, l) = V[l n + q + r], op(B)(l, c) = W[l n + q + c]
+        YG_TRY(yg_dgemm(s, 1, 0, m, m, nbe, -1.0, V + q, n, W + q, n, 1.0, C, (int) lda));
+        YG_TRY(yg_dgemm(s, 1, 0, m, m, nbe, -1.0, W + q, n, V + q, n, 1.0, C, (int) lda));
+    }
+    hipLaunchKernelGGL(k_latrd_tail, dim3(1), dim3(64), 0, s, n, lda, A, d, e, tau);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
