@@ -9,9 +9,11 @@
 //      A -= v w^T + w v^T).  The n - 2 dependent steps then touch no memory but LDS:
 //      a step is one symmetric matrix-vector product and one rank-2 update on
 //      registers, three workgroup barriers, and the reflector written out for step 3;
-//   2. rocSOLVER dstedc on T (divide and conquer, all eigenpairs of T);
+//   2. the eigenpairs of T: for the density all eigenvalues by Sturm multisection and the
+//      n_occ lowest eigenvectors by inverse iteration (k_tridiag_*, below); for S^{-1/2}
+//      and J^{-1/2} all eigenpairs by rocSOLVER dstedc;
 //   3. k_backtransform_rows: u = Q z = H_0 (H_1 (... H_{n-3} z)) only for the
-//      eigenvectors the density needs (the n_occ lowest), one wavefront each.
+//      eigenvectors needed, one wavefront each, reflectors staged through LDS.
 // rocSOLVER's dsyevd spends most of its time at these sizes in the
 // latency-bound panel kernels of its own tridiagonal reduction.
 //
@@ -312,59 +314,63 @@ __global__ __launch_bounds__(RTHR) void k_sytrd_reg(int n, double *__restrict__ 
     if (t == 0) d[n - 1] = L.xs[n - 1];
 }
 
-constexpr int BVPL = (YG_EIG_NMAX_SMALL + 63) / 64;   // reflector elements per lane
+constexpr int BTV = 4;           // back-transform: vectors (wavefronts) per workgroup
+constexpr int BTK = 16;          // reflectors staged in LDS per block
 
 // U[j] = Q z_j for the first nvec eigenvectors of T; Z holds them column-major (z_j in
-// Z[j*n .. j*n+n), the dstedc layout), U row j receives u_j.  One wavefront per vector;
-// the next reflector is loaded while the current one is applied.
-__global__ __launch_bounds__(64) void k_backtransform_rows(int n, const double *__restrict__ A,
-                                                           const double *__restrict__ tau,
-                                                           const double *__restrict__ Z, double *__restrict__ U)
+// Z[j*n .. j*n+n), the dstedc layout), U row j receives u_j.  One wavefront per vector,
+// BTV vectors per workgroup; the reflectors are staged in LDS BTK at a time (the next
+// block's loads in registers while the current one is applied), so the sequential
+// reflector steps touch only LDS.
+__global__ __launch_bounds__(64 * BTV) void k_backtransform_rows(int n, int nvec, const double *__restrict__ A,
+                                                                 const double *__restrict__ tau,
+                                                                 const double *__restrict__ Z, double *__restrict__ U)
 {
-    __shared__ double r[YG_EIG_NMAX_SMALL];
-    const int lane = threadIdx.x, j = blockIdx.x;
-    for (int i = lane; i < n; i += 64) r[i] = Z[(size_t) j * n + i];
-    __syncthreads();
-    double vc[BVPL], vn[BVPL];
-    double tc = 0.0, tn = 0.0;
-    if (n >= 3) {
-        const int k = n - 3, m = n - k - 1;
+    constexpr int NT = 64 * BTV;
+    constexpr int PER = (BTK * YG_EIG_NMAX_SMALL + NT - 1) / NT;
+    __shared__ double sV[BTK][YG_EIG_NMAX_SMALL];
+    __shared__ double stau[BTK];
+    __shared__ double r[BTV][YG_EIG_NMAX_SMALL];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int j = blockIdx.x * BTV + wv;
+    const int jj = min(j, nvec - 1);            // surplus wavefronts repeat the last vector, store nothing
+    double *rv = r[wv];
+    for (int i = lane; i < n; i += 64) rv[i] = Z[(size_t) jj * n + i];
+    // block b holds reflectors ktop(b) - q, q < BTK, ktop(b) = n - 3 - BTK b; element c of
+    // reflector k sits in row k of A at column c (c > k), staged at sV[q][c]
+    double pre[PER];
+    auto load_block = [&](int ktop) {
 #pragma unroll
-        for (int q = 0; q < BVPL; ++q) {
-            const int i = lane + 64 * q;
-            vc[q] = (i < m) ? A[(size_t) k * n + k + 1 + i] : 0.0;
+        for (int g = 0; g < PER; ++g) {
+            const int idx = t + NT * g, q = idx / YG_EIG_NMAX_SMALL, c = idx % YG_EIG_NMAX_SMALL;
+            const int k = ktop - q;
+            pre[g] = (q < BTK && k >= 0 && c > k && c < n) ? A[(size_t) k * n + c] : 0.0;
         }
-        tc = tau[k];
-    }
-    for (int k = n - 3; k >= 0; --k) {
-        const int m = n - k - 1;
-        if (k > 0) {
-            const int kn = k - 1, mn = m + 1;
+    };
+    const int ktop0 = n - 3;
+    if (ktop0 >= 0) load_block(ktop0);
+    for (int ktop = ktop0; ktop >= 0; ktop -= BTK) {
+        __syncthreads();                        // the previous block's reflectors are applied
 #pragma unroll
-            for (int q = 0; q < BVPL; ++q) {
-                const int i = lane + 64 * q;
-                vn[q] = (i < mn) ? A[(size_t) kn * n + kn + 1 + i] : 0.0;
-            }
-            tn = tau[kn];
+        for (int g = 0; g < PER; ++g) {
+            const int idx = t + NT * g, q = idx / YG_EIG_NMAX_SMALL, c = idx % YG_EIG_NMAX_SMALL;
+            if (q < BTK) sV[q][c] = pre[g];
         }
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < BVPL; ++q) {
-            const int i = lane + 64 * q;
-            if (i < m) s += vc[q] * r[k + 1 + i];
-        }
-        const double f = tc * wave_sum(s);
-#pragma unroll
-        for (int q = 0; q < BVPL; ++q) {
-            const int i = lane + 64 * q;
-            if (i < m) r[k + 1 + i] -= f * vc[q];
-        }
+        if (t < BTK) stau[t] = (ktop - t >= 0) ? tau[ktop - t] : 0.0;
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < BVPL; ++q) vc[q] = vn[q];
-        tc = tn;
+        if (ktop - BTK >= 0) load_block(ktop - BTK);
+        const int qn = min(BTK, ktop + 1);
+        for (int q = 0; q < qn; ++q) {
+            const int k = ktop - q;
+            const double *v = sV[q];
+            double s = 0.0;
+            for (int c = k + 1 + lane; c < n; c += 64) s += v[c] * rv[c];
+            const double f = stau[q] * wave_sum(s);
+            for (int c = k + 1 + lane; c < n; c += 64) rv[c] -= f * v[c];
+        }
     }
-    for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = r[i];
+    if (j < nvec)
+        for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = rv[i];
 }
 
 // ---------------------------------------------------------------------------------
@@ -647,7 +653,8 @@ int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const
         return YGPU_ERR_INVALID;
     }
     if (nvec == 0) return YGPU_OK;
-    hipLaunchKernelGGL(k_backtransform_rows, dim3(nvec), dim3(64), 0, s, n, A, tau, Z, U);
+    hipLaunchKernelGGL(k_backtransform_rows, dim3((nvec + BTV - 1) / BTV), dim3(64 * BTV), 0, s, n, nvec, A, tau, Z,
+                       U);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
