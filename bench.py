@@ -59,6 +59,8 @@ SETTINGS = {
     "c96h24": dict(ene_tol=1e-8, quirk_a2=0, family="flake"),
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP64_VECTOR_PEAK_TFLOPS = 78.6                 # MI355X FP64 vector rate (same as the FP64 matrix rate)
+CUS = 256
 BF16_SPLIT_PEAK_TFLOPS = 2500.0 / 3.0   # dense BF16 MFMA peak, three products per FP32 product
 HBM_PEAK_GBS = 8000.0
 
@@ -78,6 +80,33 @@ def profiled_traffic(workload, mixed):
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split("=")[1]), "profiles/" + name
     return None, None
+
+
+def profiled_dominant(workload, mixed, nbf):
+    """The longest-running kernel of one SCF of this workload from its committed rocprofv3
+    kernel-trace summary (first row of the table, sorted by total time), or None.  For the
+    one-workgroup tridiagonal reduction the achieved FP64 rate is given against one CU's
+    FP64 vector peak (the kernel runs on one CU by design) and against the chip's."""
+    name = "scf_%s_%s_kernel_trace.md" % (workload, "mixed_learned" if mixed else "fp64")
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| ") and not line.startswith("| kernel") and not line.startswith("|---"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            kern, calls, total_us, avg_us, pct = cells[0], int(cells[1]), float(cells[2]), float(cells[3]), float(cells[4])
+            dom = {"kernel": kern, "share_of_kernel_time": pct / 100.0, "calls": calls, "avg_us": avg_us,
+                   "source": "profiles/" + name}
+            if kern.startswith("k_sytrd_reg"):
+                flops = 4.0 * nbf ** 3 / 3.0                 # Householder tridiagonalisation
+                ach = flops / (avg_us * 1e-6) / 1e12
+                dom.update({"bound": "fp64 vector, one CU (latency of the n - 2 dependent steps)",
+                            "flops_per_launch": flops, "achieved_TFLOPs": ach,
+                            "one_cu_fp64_peak_TFLOPs": FP64_VECTOR_PEAK_TFLOPS / CUS,
+                            "frac_of_one_cu": ach / (FP64_VECTOR_PEAK_TFLOPS / CUS),
+                            "frac_of_chip": ach / FP64_VECTOR_PEAK_TFLOPS})
+            return dom
+    return None
 
 
 def run_gpu_scf(xyz, niters, settings, mixed=False, fock_check=False, weights=None):
@@ -257,6 +286,9 @@ def main():
         "j_build": {"bound": "hbm", "ms": j_ms, "bytes_per_call": j_bytes,
                     "achieved_GBs": j_bytes / (j_ms * 1e-3) / 1e9, "peak_GBs": HBM_PEAK_GBS},
         "syrk_ms": syrk_ms,
+        # the roofline entry above is the Fock build's MFMA kernel; at small nbf the longest-running
+        # kernel of the step is the density eigensolver's one-workgroup reduction
+        "dominant_kernel": profiled_dominant(args.workload, mixed, nbf),
     }
     if mixed:
         out["mixed_switch_iteration"] = last["switch"]
