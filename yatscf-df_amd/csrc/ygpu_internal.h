@@ -155,9 +155,11 @@ int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                  
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)
 int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
+// X (nbf x n_occ naux_pad) is stored pre-split: the BF16 hi values, then the BF16 lo values
+// (the bytes of an FP32 X); the X build writes it, the SYRK reads it
 int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                  const int *xpartner, const float *Cocc32, const float *B32, float *X);
-int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part);
+                  const int *xpartner, const float *Cocc32, const float *B32, void *X);
+int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part);
 int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const double *K, double *F);
 int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);
 int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);

@@ -94,22 +94,45 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
         inb[g] = idx < SB;
         lb.coords(inb[g] ? idx : 0, rowb[g], kvb[g]);
     }
+    // FP32 staging registers (split into hi / lo at the LDS store), or for loaders whose
+    // data is stored pre-split (kPresplit), the BF16 hi / lo groups themselves
     float ra[GA][8], rb[GB][8];
+    v8bf rah[GA], ral[GA], rbh[GB], rbl[GB];
     auto load_stage = [&](int k0) {
 #pragma unroll
         for (int g = 0; g < GA; ++g)
-            if (ina[g]) la.load(rowa[g], k0 + 8 * kva[g], kend, ra[g]);
+            if (ina[g]) {
+                if constexpr (LA::kPresplit) la.load2(rowa[g], k0 + 8 * kva[g], kend, rah[g], ral[g]);
+                else la.load(rowa[g], k0 + 8 * kva[g], kend, ra[g]);
+            }
 #pragma unroll
         for (int g = 0; g < GB; ++g)
-            if (inb[g]) lb.load(rowb[g], k0 + 8 * kvb[g], kend, rb[g]);
+            if (inb[g]) {
+                if constexpr (LB::kPresplit) lb.load2(rowb[g], k0 + 8 * kvb[g], kend, rbh[g], rbl[g]);
+                else lb.load(rowb[g], k0 + 8 * kvb[g], kend, rb[g]);
+            }
     };
     auto split_stage = [&](int buf) {
 #pragma unroll
         for (int g = 0; g < GA; ++g)
-            if (ina[g]) split8(ra[g], Ah[buf][rowa[g]][kva[g]], Al[buf][rowa[g]][kva[g]]);
+            if (ina[g]) {
+                if constexpr (LA::kPresplit) {
+                    Ah[buf][rowa[g]][kva[g]] = rah[g];
+                    Al[buf][rowa[g]][kva[g]] = ral[g];
+                } else {
+                    split8(ra[g], Ah[buf][rowa[g]][kva[g]], Al[buf][rowa[g]][kva[g]]);
+                }
+            }
 #pragma unroll
         for (int g = 0; g < GB; ++g)
-            if (inb[g]) split8(rb[g], Bh[buf][rowb[g]][kvb[g]], Bl[buf][rowb[g]][kvb[g]]);
+            if (inb[g]) {
+                if constexpr (LB::kPresplit) {
+                    Bh[buf][rowb[g]][kvb[g]] = rbh[g];
+                    Bl[buf][rowb[g]][kvb[g]] = rbl[g];
+                } else {
+                    split8(rb[g], Bh[buf][rowb[g]][kvb[g]], Bl[buf][rowb[g]][kvb[g]]);
+                }
+            }
     };
     auto mfma_stage = [&](int buf) {
 #pragma unroll
@@ -189,6 +212,7 @@ __device__ __forceinline__ long xcd_remap(long b, long total)
 // loads wait on one memory latency, not on an index load followed by a data load
 template <int BMt>
 struct XLoadA {
+    static constexpr bool kPresplit = false;
     const float *cocc; const int *pa; int koff, n_occ, m0;
     __device__ void coords(int idx, int &row, int &kv) const { row = idx % BMt; kv = idx / BMt; }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
@@ -203,6 +227,7 @@ struct XLoadA {
 
 template <int BNt>
 struct XLoadB {
+    static constexpr bool kPresplit = false;
     const float *b32; const int *up; int koff; long ld; int n0;
     __device__ void coords(int idx, int &row, int &kv) const { row = idx % BNt; kv = idx / BNt; }
     __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
@@ -223,8 +248,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
                                                              const int *__restrict__ xup,
                                                              const int *__restrict__ xpartner,
                                                              const float *__restrict__ Cocc32,
-                                                             const float *__restrict__ B32, float *__restrict__ X,
-                                                             int n_occ, int naux_pad, int ncolt, int nrowt, long total)
+                                                             const float *__restrict__ B32, __bf16 *__restrict__ Xh,
+                                                             __bf16 *__restrict__ Xl, int n_occ, int naux_pad,
+                                                             int ncolt, int nrowt, long total)
 {
     // column tiles fastest: consecutive workgroups read neighbouring segments of the same B
     // rows (measured faster than grouping the row tiles of one panel on an XCD)
@@ -247,32 +273,38 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
         bf3_tile<WM, WN, XBK, XNBUF>(la, lb, kc, kce, acc, kc == 0);
         if (kce >= klen) break;
     }
-    float *Xi = X + (size_t) i * n_occ * naux_pad;
+    // X stored pre-split for the SYRK: hi = bf16(v), lo = bf16(v - hi), exactly what the
+    // SYRK's own split of the FP32 value would give
+    const size_t xo = (size_t) i * n_occ * naux_pad;
     bf3_epilogue<WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
-        if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = v;
+        if (m < n_occ && n < naux_pad) {
+            const __bf16 h = (__bf16) v;
+            Xh[xo + (size_t) m * naux_pad + n] = h;
+            Xl[xo + (size_t) m * naux_pad + n] = (__bf16) (v - (float) h);
+        }
     });
 }
 
-// ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous
+// ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous, X pre-split into BF16
+// hi / lo arrays by the X build: a stage moves 16-byte groups straight into LDS
 template <int BK>
 struct SLoad {
-    const float *x; long kdim; int nbf, r0;
+    static constexpr bool kPresplit = true;
+    const __bf16 *xh, *xl; long kdim; int nbf, r0;
     __device__ void coords(int idx, int &row, int &kv) const
     {
         kv = idx % Stage<BK>::KV;
         row = idx / Stage<BK>::KV;
     }
-    __device__ void load(int row, int k8, int kend, float (&v)[8]) const {
+    __device__ void load2(int row, int k8, int kend, v8bf &h, v8bf &l) const {
         const int r = r0 + row;
         if (r < nbf && k8 < kend) {
-            const float4 *p = (const float4 *) (x + (size_t) r * kdim + k8);
-            const float4 a = p[0], b = p[1];
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            h = *(const v8bf *) (xh + (size_t) r * kdim + k8);
+            l = *(const v8bf *) (xl + (size_t) r * kdim + k8);
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = 0.f;
+            for (int e = 0; e < 8; ++e) h[e] = l[e] = (__bf16) 0.f;
         }
     }
 };
@@ -280,8 +312,9 @@ struct SLoad {
 constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
 
 // upper 128x128 tiles of X X^T over k chunks; part[z] is FP32 per chunk
-__global__ __launch_bounds__(NTHR) void k_syrk_bf3(const float *__restrict__ X, int nbf, long kdim, int ntb,
-                                                   int ntiles, long kchunk, long total, float *__restrict__ part)
+__global__ __launch_bounds__(NTHR) void k_syrk_bf3(const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
+                                                   int nbf, long kdim, int ntb, int ntiles, long kchunk, long total,
+                                                   float *__restrict__ part)
 {
     const long t = xcd_remap(blockIdx.x, total);
     int tile = (int) (t % ntiles);
@@ -290,8 +323,8 @@ __global__ __launch_bounds__(NTHR) void k_syrk_bf3(const float *__restrict__ X, 
     while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
     const int bj = bi + tile;
     const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
-    SLoad<SBK> la{X + kb, kdim, nbf, bi * ST};
-    SLoad<SBK> lb{X + kb, kdim, nbf, bj * ST};
+    SLoad<SBK> la{Xh + kb, Xl + kb, kdim, nbf, bi * ST};
+    SLoad<SBK> lb{Xh + kb, Xl + kb, kdim, nbf, bj * ST};
     v16f acc[2][2];
     bf3_tile<2, 2, SBK, 1>(la, lb, 0, (int) (ke - kb), acc);
     float *P = part + (size_t) z * nbf * nbf;
@@ -340,22 +373,23 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n)
 
 template <int WM, int WN>
 static int launch_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                             const int *xpartner, const float *Cocc32, const float *B32, float *X)
+                             const int *xpartner, const float *Cocc32, const float *B32, void *X)
 {
     const int ncolt = (naux_pad + 64 * WN - 1) / (64 * WN), nrowt = (n_occ + 64 * WM - 1) / (64 * WM);
+    __bf16 *Xh = (__bf16 *) X, *Xl = Xh + (size_t) nbf * n_occ * naux_pad;
     const long total = (long) ncolt * nrowt * nbf;
     if (total > 0x7fffffffL) {
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
     hipLaunchKernelGGL((k_xbuild_bf3<WM, WN>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, xup, xpartner,
-                       Cocc32, B32, X, n_occ, naux_pad, ncolt, nrowt, total);
+                       Cocc32, B32, Xh, Xl, n_occ, naux_pad, ncolt, nrowt, total);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
 
 int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                  const int *xpartner, const float *Cocc32, const float *B32, float *X)
+                  const int *xpartner, const float *Cocc32, const float *B32, void *X)
 {
     if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
     // all occupied rows in one workgroup up to 256 (row tiles of 256 beyond)
@@ -366,7 +400,7 @@ int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xp
     return launch_xbuild_bf3<4, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
 }
 
-int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DBuf &part)
+int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part)
 {
     if (nbf <= 0) return YGPU_OK;
     if (kdim % 8) {
@@ -383,8 +417,9 @@ int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const float *X, double *K, DB
     const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
     YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
     const long total = (long) ntiles * nsplit;
-    hipLaunchKernelGGL(k_syrk_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, X, nbf, kdim, ntb, ntiles, kchunk, total,
-                       part.as<float>());
+    const __bf16 *Xh = (const __bf16 *) X, *Xl = Xh + (size_t) nbf * kdim;
+    hipLaunchKernelGGL(k_syrk_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, Xh, Xl, nbf, kdim, ntb, ntiles, kchunk,
+                       total, part.as<float>());
     YG_HIP(hipGetLastError());
     const long tot = (long) nbf * nbf;
     hipLaunchKernelGGL(k_syrk_bf3_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, K);
