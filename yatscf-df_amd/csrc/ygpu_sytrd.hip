@@ -12,7 +12,7 @@
 // One kernel per column (k_latrd_step): each of its workgroups repeats steps 1-3 and the
 // small products W^T v, V^T v for all rows (loads issued up front, values shared through
 // LDS only), then takes its share of the step-4 GEMV rows, one wavefront per row.  After a panel the
-// trailing block takes the rank-2 NB update on FP64 MFMA (two GEMMs).  V and W are stored
+// trailing block takes the rank-2 NB update in one pass over it (k_syr2k_panel).  V and W are stored
 // column-major (column l contiguous over the rows), so that the column kernel's per-l loads
 // across a wavefront are coalesced.  The reflectors are
 // left where LAPACK leaves them for the lower variant (v(i+2:n) in A(i+2:n, i) of the
@@ -28,8 +28,9 @@ namespace {
 constexpr int NB = 16;           // panel width
 constexpr int CT = 1024;         // column kernel: one workgroup
 constexpr int ST = 512;          // column step: threads per workgroup (four rows each at n = 2048)
-constexpr int LGRID = 64;        // workgroups of the column step (each repeats the column work, then a
-                                 // 1/64 share of the trailing GEMV)
+constexpr int LGRID = 128;       // workgroups of the column step (each repeats the column work, then a
+                                 // 1/128 share of the trailing GEMV; 64 and 256 measured slower at n = 1560)
+constexpr int UT = 64;           // trailing update: C tile edge
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v)
@@ -114,17 +115,19 @@ __global__ __launch_bounds__(CT) void k_latrd_finish(int n, long lda, double *__
     }
 }
 
-// One column i = p + j of a panel, by every workgroup of the grid.  Each workgroup computes
+// One column i = p + J of a panel, by every workgroup of the grid (specialised on the panel
+// position J, so that only the J - 1 stored columns of V and W are loaded).  Each workgroup computes
 // the column's steps 1-3 and the products W^T v, V^T v for all rows (the same operations in
 // the same order, so every workgroup holds the same v), then its share of the trailing
-// GEMV y = A0 v - V (W^T v) - W (V^T v).  Workgroup 0 also writes W(:, j - 1), V(:, j) and
+// GEMV y = A0 v - V (W^T v) - W (V^T v).  Workgroup 0 also writes W(:, J - 1), V(:, J) and
 // d, e, tau.  vin / yin are the previous column's v and y, vout / yout this column's
 // (separate buffers: other workgroups still read the inputs).  Row i of A receives the
 // reflector only at the end of the panel (k_latrd_finish).  Thread t owns rows
 // i + t + ST k (k < 4); V and W are read once per row: with v = 1 at row i + 1 and
 // v = x scale below, W^T v = scale (W^T x)_{>= i+2} + W(i + 1, :), accumulated in the same
 // pass that forms x.
-__global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double *__restrict__ A, int i, int j,
+template <int J>
+__global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double *__restrict__ A, int i,
                                                    double *__restrict__ V, double *__restrict__ W,
                                                    const double *__restrict__ vin, const double *__restrict__ yin,
                                                    double *__restrict__ vout, double *__restrict__ yout,
@@ -135,12 +138,13 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
     constexpr int NS = ST / 16;                      // row-of-16 partial sums per workgroup
     __shared__ double red[ST / 64];
     __shared__ double red2[NS][2 * NB];
-    __shared__ double sv[YG_SYTRD_BLOCKED_NMAX], sw[YG_SYTRD_BLOCKED_NMAX];   // v_i and W(:, j - 1)
+    __shared__ double sv[YG_SYTRD_BLOCKED_NMAX], sw[YG_SYTRD_BLOCKED_NMAX];   // v_i and W(:, J - 1)
     __shared__ double svi[NB], swi[NB], sv1[NB], sw1[NB], sab[2 * NB];       // V, W rows i and i + 1
     __shared__ double sx[2];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const bool lead = blockIdx.x == 0;
-    const int jp = j - 1;                       // the previous column (V(:, jp) = vin, W(:, jp) from yin)
+    constexpr int JP = J - 1;                   // the previous column (V(:, JP) = vin, W(:, JP) from yin)
+    constexpr int NL = J > 1 ? J - 1 : 1, NP = J > 0 ? J : 1;
     // ---- loads that do not depend on this kernel's results
     double a_ir[RK], y_r[RK], vp[RK], wp[RK];
 #pragma unroll
@@ -148,17 +152,17 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
         const int r = i + t + ST * k;
         const bool in = r < n;
         a_ir[k] = in ? A[(size_t) i * lda + r] : 0.0;
-        y_r[k] = (in && j > 0) ? yin[r] : 0.0;
-        vp[k] = (in && j > 0) ? vin[r] : 0.0;
+        y_r[k] = (in && J > 0) ? yin[r] : 0.0;
+        vp[k] = (in && J > 0) ? vin[r] : 0.0;
         wp[k] = 0.0;
     }
-    if (t < jp) {
+    if (t < JP) {
         svi[t] = V[(size_t) t * n + i];
         swi[t] = W[(size_t) t * n + i];
     }
-    const double tp = (j > 0) ? tau[i - 1] : 0.0;
-    // ---- step 1: W(:, jp) = tau y - (tau^2 / 2)(y . v) v from the previous column
-    if (j > 0) {
+    const double tp = (J > 0) ? tau[i - 1] : 0.0;
+    // ---- step 1: W(:, JP) = tau y - (tau^2 / 2)(y . v) v from the previous column
+    if constexpr (J > 0) {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < RK; ++k) s += y_r[k] * vp[k];
@@ -170,77 +174,74 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
             wp[k] = tp * y_r[k] + c * vp[k];
             if (r < n) {
                 sw[r] = wp[k];
-                if (lead) W[(size_t) jp * n + r] = wp[k];
+                if (lead) W[(size_t) JP * n + r] = wp[k];
             }
         }
         if (t == 0) {
-            svi[jp] = vp[0];                    // V(i, jp) (= 1)
-            swi[jp] = wp[0];                    // W(i, jp)
+            svi[JP] = vp[0];                    // V(i, JP) (= 1)
+            swi[JP] = wp[0];                    // W(i, JP)
         }
         __syncthreads();
     }
     // ---- step 2: x_r = A0(i, r) - sum_l (V(r, l) W(i, l) + W(r, l) V(i, l)), r >= i, and the
     // partial sums of W^T x, V^T x over r >= i + 2
     double x[RK], ss = 0.0;
-    double pa[NB], pb[NB];
+    double pa[NP], pb[NP];
 #pragma unroll
-    for (int l = 0; l < NB; ++l) pa[l] = pb[l] = 0.0;
+    for (int l = 0; l < NP; ++l) pa[l] = pb[l] = 0.0;
 #pragma unroll
     for (int k = 0; k < RK; ++k) {
         const int r = i + t + ST * k;
         double v = a_ir[k];
         if (r < n) {
-            // all of the row's panel loads in flight at once: unconditional (every column index
-            // is inside V and W), the columns l >= jp masked afterwards
-            double vl[NB - 1], wl[NB - 1];
+            // all of the row's panel loads in flight at once
+            double vl[NL], wl[NL];
 #pragma unroll
-            for (int l = 0; l < NB - 1; ++l) {
+            for (int l = 0; l < JP; ++l) {
                 vl[l] = V[(size_t) l * n + r];
                 wl[l] = W[(size_t) l * n + r];
             }
 #pragma unroll
-            for (int l = 0; l < NB - 1; ++l) v -= (l < jp) ? vl[l] * swi[l] + wl[l] * svi[l] : 0.0;
-            if (j > 0) v -= vp[k] * swi[jp] + wp[k] * svi[jp];
+            for (int l = 0; l < JP; ++l) v -= vl[l] * swi[l] + wl[l] * svi[l];
+            if constexpr (J > 0) v -= vp[k] * swi[JP] + wp[k] * svi[JP];
             if (r == i) sx[0] = v;
             if (r == i + 1) {
                 sx[1] = v;
 #pragma unroll
-                for (int l = 0; l < NB - 1; ++l) {
-                    sv1[l] = (l < jp) ? vl[l] : 0.0;
-                    sw1[l] = (l < jp) ? wl[l] : 0.0;
+                for (int l = 0; l < JP; ++l) {
+                    sv1[l] = vl[l];
+                    sw1[l] = wl[l];
                 }
-                if (j > 0) {
-                    sv1[jp] = vp[k];
-                    sw1[jp] = wp[k];
+                if constexpr (J > 0) {
+                    sv1[JP] = vp[k];
+                    sw1[JP] = wp[k];
                 }
             }
             if (r >= i + 2) {
                 ss += v * v;
 #pragma unroll
-                for (int l = 0; l < NB - 1; ++l) {
-                    pa[l] += (l < jp) ? wl[l] * v : 0.0;
-                    pb[l] += (l < jp) ? vl[l] * v : 0.0;
+                for (int l = 0; l < JP; ++l) {
+                    pa[l] += wl[l] * v;
+                    pb[l] += vl[l] * v;
                 }
-#pragma unroll
-                for (int l = 0; l < NB; ++l)
-                    if (l == jp) {
-                        pa[l] += wp[k] * v;
-                        pb[l] += vp[k] * v;
-                    }
+                if constexpr (J > 0) {
+                    pa[JP] += wp[k] * v;
+                    pb[JP] += vp[k] * v;
+                }
             }
         }
         x[k] = v;
     }
     // row-of-16 sums of the partial products (their last adds go through LDS)
 #pragma unroll
-    for (int l = 0; l < NB; ++l) {
+    for (int l = 0; l < J; ++l) {
         pa[l] = row16_sum(pa[l]);
         pb[l] = row16_sum(pb[l]);
     }
     if ((lane & 15) == 0) {
         const int slot = wv * 4 + (lane >> 4);
 #pragma unroll
-        for (int l = 0; l < NB; ++l) {
+        for (int l = 0; l < J; ++l) {
             red2[slot][l] = pa[l];
             red2[slot][NB + l] = pb[l];
         }
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
     if (t < 2 * NB) {
         const int l = t % NB;
         double s = 0.0;
-        if (l < j) {
+        if (l < J) {
 #pragma unroll
             for (int q = 0; q < NS; ++q) s += red2[q][t];
             s = scale * s + ((t < NB) ? sw1[l] : sv1[l]);
@@ -286,9 +287,9 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
         }
         if (c < n) s0 += row[c] * sv[c];
         double s = s0 + s1;
-        if (lane < j) {
-            const double vl = (lane < jp) ? V[(size_t) lane * n + r] : vin[r];
-            const double wl = (lane < jp) ? W[(size_t) lane * n + r] : sw[r];
+        if (lane < J) {
+            const double vl = (lane < JP) ? V[(size_t) lane * n + r] : vin[r];
+            const double wl = (lane < JP) ? W[(size_t) lane * n + r] : sw[r];
             s -= vl * sab[lane] + wl * sab[NB + lane];
         }
         s = wave_sum(s);
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
             const int r = i + t + ST * k;
             if (r < n) {
                 vout[r] = v_r[k];
-                if (r >= i + 1) V[(size_t) j * n + r] = v_r[k];
+                if (r >= i + 1) V[(size_t) J * n + r] = v_r[k];
             }
         }
         if (t == 0) {
@@ -310,6 +311,55 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
             tau[i] = tk;
         }
     }
+}
+
+// Trailing update after a panel: C (m x m, row-major, ldc) -= V W^T + W V^T over the panel's
+// nbe columns (V, W column-major with leading dimension ldv).  One 64 x 64 tile of C per
+// workgroup of 16 x 16 threads, a 4 x 4 block per thread; the tile's rows and columns of V and
+// W staged in LDS.
+__global__ __launch_bounds__(256) void k_syr2k_panel(int m, int nbe, const double *__restrict__ V,
+                                                     const double *__restrict__ W, long ldv, double *__restrict__ C,
+                                                     long ldc)
+{
+    __shared__ double vr[NB][UT], wr[NB][UT], vc[NB][UT], wc[NB][UT];
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int r0 = blockIdx.y * UT, c0 = blockIdx.x * UT;
+    const int rb = r0 + 4 * ty, cb = c0 + 4 * tx;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            acc[a][b] = (rb + a < m && cb + b < m) ? C[(size_t) (rb + a) * ldc + cb + b] : 0.0;
+    for (int q = t; q < nbe * UT; q += 256) {
+        const int l = q / UT, k = q % UT;
+        const bool rin = r0 + k < m, cin = c0 + k < m;
+        vr[l][k] = rin ? V[(size_t) l * ldv + r0 + k] : 0.0;
+        wr[l][k] = rin ? W[(size_t) l * ldv + r0 + k] : 0.0;
+        vc[l][k] = cin ? V[(size_t) l * ldv + c0 + k] : 0.0;
+        wc[l][k] = cin ? W[(size_t) l * ldv + c0 + k] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int l = 0; l < nbe; ++l) {
+        double va[4], wa[4], vb[4], wb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            va[q] = vr[l][4 * ty + q];
+            wa[q] = wr[l][4 * ty + q];
+            vb[q] = vc[l][4 * tx + q];
+            wb[q] = wc[l][4 * tx + q];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] -= va[a] * wb[b] + wa[a] * vb[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (rb + a < m && cb + b < m) C[(size_t) (rb + a) * ldc + cb + b] = acc[a][b];
 }
 
 // the trailing 2 x 2 block after the last reflector
@@ -322,6 +372,18 @@ __global__ void k_latrd_tail(int n, long lda, double *__restrict__ A, double *__
     d[n - 1] = A[(size_t) (n - 1) * lda + n - 1];
     tau[n - 2] = 0.0;
     A[(size_t) (n - 2) * lda + n - 1] = e[n - 2];
+}
+
+// k_latrd_step<J> for the panel position j (0 <= j < NB)
+template <int J>
+void launch_step(hipStream_t s, int j, int n, long lda, const double *A, int i, double *V, double *W,
+                 const double *vin, const double *yin, double *vout, double *yout, double *d, double *e, double *tau)
+{
+    if (j == J)
+        hipLaunchKernelGGL(k_latrd_step<J>, dim3(LGRID), dim3(ST), 0, s, n, lda, A, i, V, W, vin, yin, vout, yout, d,
+                           e, tau);
+    else if constexpr (J + 1 < NB)
+        launch_step<J + 1>(s, j, n, lda, A, i, V, W, vin, yin, vout, yout, d, e, tau);
 }
 
 }  // namespace
@@ -341,16 +403,15 @@ int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, doubl
     hipLaunchKernelGGL(k_mirror_lower, dim3(std::min<long>(((long) n * n + 255) / 256, 4096)), dim3(256), 0, s, n,
                        lda, A);
     YG_HIP(hipGetLastError());
+    // no clearing of V and W per panel: column l of a panel starting at p is written for rows
+    // >= p + l + 1 before any kernel reads it, and every read is of such rows
     const int last = n - 3;                     // reflectors 0 .. n - 3
     int cur = 0;                                // vb[cur], yb[cur]: the previous column's v and y
     for (int p = 0; p <= last; p += NB) {
         const int nbe = std::min(NB, last - p + 1);
-        YG_HIP(hipMemsetAsync(V, 0, (size_t) 2 * n * NB * sizeof(double), s));
         for (int j = 0; j < nbe; ++j) {
             const int i = p + j;
-            hipLaunchKernelGGL(k_latrd_step, dim3(LGRID), dim3(ST), 0, s, n, lda, (const double *) A, i, j, V, W,
-                               (const double *) vb[cur], (const double *) yb[cur], vb[cur ^ 1], yb[cur ^ 1], d, e,
-                               tau);
+            launch_step<0>(s, j, n, lda, A, i, V, W, vb[cur], yb[cur], vb[cur ^ 1], yb[cur ^ 1], d, e, tau);
             YG_HIP(hipGetLastError());
             cur ^= 1;
         }
@@ -361,9 +422,10 @@ int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, doubl
         // trailing block rows / columns >= q: A -= V W^T + W V^T
         const int q = p + nbe, m = n - q;
         double *C = A + (size_t) q * lda + q;
-        // V, W column-major (leading dimension n): op(A)(r, l) = V[l n + q + r], op(B)(l, c) = W[l n + q + c]
-        YG_TRY(yg_dgemm(s, 1, 0, m, m, nbe, -1.0, V + q, n, W + q, n, 1.0, C, (int) lda));
-        YG_TRY(yg_dgemm(s, 1, 0, m, m, nbe, -1.0, W + q, n, V + q, n, 1.0, C, (int) lda));
+        const int nt = (m + UT - 1) / UT;
+        hipLaunchKernelGGL(k_syr2k_panel, dim3(nt, nt), dim3(256), 0, s, m, nbe, (const double *) V + q,
+                           (const double *) W + q, (long) n, C, lda);
+        YG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_latrd_tail, dim3(1), dim3(64), 0, s, n, lda, A, d, e, tau);
     YG_HIP(hipGetLastError());
