@@ -109,8 +109,6 @@ int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
     (void) rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, n, nullptr, ld, nullptr, nullptr,
                             nullptr);
     (void) rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, nullptr, nullptr, nullptr, ld, nullptr);
-    (void) rocsolver_dormtr(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, nullptr, ld,
-                            nullptr, nullptr, ld);
     if (rocblas_stop_device_memory_size_query(c->rb, &need) != rocblas_status_success) {
         ygpu_set_error(YGPU_ERR_SOLVER, "rocblas_stop_device_memory_size_query failed");
         return YGPU_ERR_SOLVER;
@@ -129,7 +127,8 @@ int ensure_syevd_workspace(ygpu_ctx *c, int n, int ld)
 
 // After yg_sytrd_blocked on A (reflectors in A, d in d_eig, e in d_evE, tau in d_tau): all
 // eigenpairs of T by rocSOLVER's divide and conquer into Z (column-major, leading dimension
-// ld), then the first ncols eigenvectors of A = Q Z[:, 0:ncols] in place (dormtr).
+// ld), then the first ncols eigenvectors of A = Q Z[:, 0:ncols] in place (the reflectors
+// applied one wavefront per vector, as LAPACK's dormtr would).
 int solver_tridiag_to_full(ygpu_ctx *c, int n, int ld, double *A, double *Z, int ncols)
 {
     rocblas_status st = rocsolver_dstedc(c->rb, rocblas_evect_tridiagonal, n, c->d_eig.as<double>(),
@@ -138,13 +137,7 @@ int solver_tridiag_to_full(ygpu_ctx *c, int n, int ld, double *A, double *Z, int
         ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", n, (int) st);
         return YGPU_ERR_SOLVER;
     }
-    st = rocsolver_dormtr(c->rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, ncols, A, ld,
-                          c->d_tau.as<double>(), Z, ld);
-    if (st != rocblas_status_success) {
-        ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dormtr(n=%d) returned %d", n, (int) st);
-        return YGPU_ERR_SOLVER;
-    }
-    return YGPU_OK;
+    return yg_backtransform_cols(c->stream, n, ncols, A, ld, c->d_tau.as<double>(), Z, ld);
 }
 
 // In-place A (n x n, leading dimension ld) -> out = A^{-1/2}; A is destroyed.
@@ -166,11 +159,11 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
             ygpu_set_error(YGPU_ERR_SOLVER, "rocsolver_dstedc(n=%d) returned %d", n, (int) st);
             return YGPU_ERR_SOLVER;
         }
-        YG_TRY(yg_backtransform_rows(c->stream, n, n, dA, c->d_tau.as<double>(), dW, dOut));
-        YG_HIP(hipMemcpyAsync(dA, dOut, (size_t) n * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        YG_TRY(yg_backtransform_cols(c->stream, n, n, dA, n, c->d_tau.as<double>(), dW, n));
+        YG_HIP(hipMemcpyAsync(dA, dW, (size_t) n * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     } else if (n <= YG_SYTRD_BLOCKED_NMAX) {
         // blocked reduction (reflectors left in dA), T = Z diag Z^T by divide and conquer into
-        // dW, V = Q Z by rocSOLVER's dormtr in place, then V back into dA (column-major view)
+        // dW, V = Q Z in place, then V back into dA (column-major view)
         YG_TRY(c->d_tau.ensure((size_t) n * sizeof(double)));
         YG_TRY(c->d_tri.ensure(yg_sytrd_blocked_ws(n)));
         YG_TRY(yg_sytrd_blocked(c->stream, n, ld, dA, c->d_eig.as<double>(), c->d_evE.as<double>(),
@@ -798,20 +791,19 @@ try {
     if (nbf <= YG_EIG_NMAX_SMALL) {
         // F' = Q T Q^T on one workgroup; all eigenvalues of T by multisection and its n_occ
         // lowest eigenvectors by inverse iteration; only those are carried back:
-        // V_occ^T = (Q Z_occ)^T in d_U (rows)
+        // V_occ = Q Z_occ in place of Z_occ (columns of d_mat2, column-major)
         const size_t nvp = (size_t) (n_occ + 63) / 64 * 64;
         YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
-        YG_TRY(c->d_U.ensure((size_t) nbf * n_occ * sizeof(double)));
         YG_TRY(c->d_tri.ensure((3 * nvp + 2) * nbf * sizeof(double)));
         double *tri_ws = c->d_tri.as<double>();
         double *dT = tri_ws + 3 * nvp * nbf, *eT = dT + nbf;
         YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), dT, eT, c->d_tau.as<double>()));
         YG_TRY(yg_tridiag_eig(c->stream, nbf, n_occ, dT, eT, c->d_eig.as<double>(), c->d_mat2.as<double>(), tri_ws));
         YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
-        YG_TRY(yg_backtransform_rows(c->stream, nbf, n_occ, c->d_mat0.as<double>(), c->d_tau.as<double>(),
-                                     c->d_mat2.as<double>(), c->d_U.as<double>()));
+        YG_TRY(yg_backtransform_cols(c->stream, nbf, n_occ, c->d_mat0.as<double>(), nbf, c->d_tau.as<double>(),
+                                     c->d_mat2.as<double>(), nbf));
         // C_occ = X V_occ
-        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_U.as<double>(), nbf, 0.0,
+        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
                         c->d_Cocc.as<double>(), n_occ));
     } else if (nbf <= YG_SYTRD_BLOCKED_NMAX) {
         // blocked reduction, all eigenpairs of T, Q applied to the n_occ lowest only: their

@@ -373,6 +373,82 @@ __global__ __launch_bounds__(64 * BTV) void k_backtransform_rows(int n, int nvec
         for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = rv[i];
 }
 
+constexpr int BLV = 4;           // back-transform above the one-workgroup size: vectors per workgroup
+
+// Z <- Q Z for the first nvec columns of Z (column j = z_j, leading dimension ldz; the
+// dstedc layout) with Q = H_0 H_1 ... H_{n-3} of the blocked reduction, whose reflector k
+// sits in row k of A (leading dimension lda) at columns k + 2 .. n - 1 with its unit
+// element at k + 1 implicit (LAPACK dsytrd's lower layout of the column-major view).  In
+// place: each wavefront reads its vector into registers (element l + 64 m in lane l,
+// m < MB) before anything is written.  The reflectors are staged in LDS BK at a time,
+// zero outside their support and padded to 64 MB columns, so that a reflector step reads
+// all MB chunks without branches; the next block's loads are held in registers (thread
+// t: columns t + 256 u) while the current one is applied.
+template <int MB, int BK>
+__global__ __launch_bounds__(64 * BLV) void k_backtransform_cols(int n, int nvec, const double *__restrict__ A,
+                                                                 long lda, const double *__restrict__ tau,
+                                                                 double *__restrict__ Z, long ldz)
+{
+    extern __shared__ double sV[];              // [BK][64 MB]
+    __shared__ double stau[BK];
+    constexpr int NT = 64 * BLV, UC = 64 * MB / NT, LN = 64 * MB;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int j = blockIdx.x * BLV + wv;
+    const int jj = min(j, nvec - 1);            // surplus wavefronts repeat the last vector, store nothing
+    double r[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+        const int c = lane + 64 * m;
+        r[m] = (c < n) ? Z[(size_t) jj * ldz + c] : 0.0;
+    }
+    // block at ktop holds reflectors ktop - q, q < BK: sV[q][c] = v_k(c) (1 at k + 1, 0 at c <= k)
+    double pre[BK][UC];
+    auto load_block = [&](int ktop) {
+#pragma unroll
+        for (int q = 0; q < BK; ++q) {
+            const int k = ktop - q;
+            const double *row = A + (size_t) max(k, 0) * lda;
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+                const int c = t + NT * u;
+                double v = 0.0;
+                if (k >= 0 && c < n) v = (c == k + 1) ? 1.0 : (c > k + 1) ? row[c] : 0.0;
+                pre[q][u] = v;
+            }
+        }
+    };
+    const int ktop0 = n - 3;
+    if (ktop0 >= 0) load_block(ktop0);
+    for (int ktop = ktop0; ktop >= 0; ktop -= BK) {
+        __syncthreads();                        // the previous block's reflectors are applied
+#pragma unroll
+        for (int q = 0; q < BK; ++q)
+#pragma unroll
+            for (int u = 0; u < UC; ++u) sV[q * LN + t + NT * u] = pre[q][u];
+        if (t < BK) stau[t] = (ktop - t >= 0) ? tau[ktop - t] : 0.0;
+        __syncthreads();
+        if (ktop - BK >= 0) load_block(ktop - BK);
+        const int qn = min(BK, ktop + 1);
+        for (int q = 0; q < qn; ++q) {
+            const double *v = sV + q * LN + lane;
+            double vm[MB], s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < MB; ++m) vm[m] = v[64 * m];
+#pragma unroll
+            for (int m = 0; m < MB; ++m) s4[m & 3] += vm[m] * r[m];
+            const double f = stau[q] * wave_sum((s4[0] + s4[1]) + (s4[2] + s4[3]));
+#pragma unroll
+            for (int m = 0; m < MB; ++m) r[m] -= f * vm[m];
+        }
+    }
+    if (j < nvec)
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+            const int c = lane + 64 * m;
+            if (c < n) Z[(size_t) j * ldz + c] = r[m];
+        }
+}
+
 // ---------------------------------------------------------------------------------
 // Eigenpairs of the symmetric tridiagonal T = tridiag(e, d, e) (n <= YG_EIG_NMAX_SMALL):
 // all eigenvalues by Sturm-count multisection, the lowest nvec eigenvectors by inverse
@@ -655,6 +731,30 @@ int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const
     if (nvec == 0) return YGPU_OK;
     hipLaunchKernelGGL(k_backtransform_rows, dim3((nvec + BTV - 1) / BTV), dim3(64 * BTV), 0, s, n, nvec, A, tau, Z,
                        U);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long lda, const double *tau, double *Z,
+                          long ldz)
+{
+    if (n < 1 || n > YG_SYTRD_BLOCKED_NMAX || nvec < 0 || nvec > n || lda < n || ldz < n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "back-transform: n = %d, nvec = %d, lda = %ld, ldz = %ld out of range", n,
+                       nvec, lda, ldz);
+        return YGPU_ERR_INVALID;
+    }
+    if (nvec == 0) return YGPU_OK;
+    const int mbn = (n + 63) / 64;
+    const dim3 grid((nvec + BLV - 1) / BLV), blk(64 * BLV);
+    auto lds = [](int mb, int bk) { return (size_t) bk * 64 * mb * sizeof(double); };
+    if (mbn <= 8)
+        hipLaunchKernelGGL((k_backtransform_cols<8, 8>), grid, blk, lds(8, 8), s, n, nvec, A, lda, tau, Z, ldz);
+    else if (mbn <= 16)
+        hipLaunchKernelGGL((k_backtransform_cols<16, 8>), grid, blk, lds(16, 8), s, n, nvec, A, lda, tau, Z, ldz);
+    else if (mbn <= 24)
+        hipLaunchKernelGGL((k_backtransform_cols<24, 4>), grid, blk, lds(24, 4), s, n, nvec, A, lda, tau, Z, ldz);
+    else
+        hipLaunchKernelGGL((k_backtransform_cols<32, 4>), grid, blk, lds(32, 4), s, n, nvec, A, lda, tau, Z, ldz);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -146,6 +146,11 @@ int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const
 // yg_sytrd_blocked_ws(n) bytes
 size_t yg_sytrd_blocked_ws(int n);
 int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, double *e, double *tau, double *ws);
+// Z <- Q Z for the first nvec columns of Z (column-major, leading dimension ldz) with the
+// reflectors yg_sytrd_blocked (or yg_sytrd_small, with lda = n) leaves in A
+// (1 <= n <= YG_SYTRD_BLOCKED_NMAX)
+int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long lda, const double *tau, double *Z,
+                          long ldz);
 // all eigenvalues w (ascending) of the symmetric tridiagonal tridiag(e, d, e) and the
 // eigenvectors of the nvec lowest as the rows of Z (nvec x n); ws holds
 // 3 n ((nvec + 63) / 64 * 64) doubles

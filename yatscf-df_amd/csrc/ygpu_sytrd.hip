@@ -16,8 +16,8 @@
 // column-major (column l contiguous over the rows), so that the column kernel's per-l loads
 // across a wavefront are coalesced.  The reflectors are
 // left where LAPACK leaves them for the lower variant (v(i+2:n) in A(i+2:n, i) of the
-// column-major view, which is row i of this row-major storage), so rocSOLVER's dormtr
-// applies Q.  The input's lower triangle (the upper triangle of the column-major view that
+// column-major view, which is row i of this row-major storage), where LAPACK's dormtr and
+// this library's yg_backtransform_cols read them.  The input's lower triangle (the upper triangle of the column-major view that
 // dsyevd with fill_upper reads) is mirrored first.
 #include "ygpu_internal.h"
 
@@ -25,7 +25,7 @@
 
 namespace {
 
-constexpr int NB = 16;           // panel width
+constexpr int NB = 8;            // panel width (4, 6, 12 and 16 measured slower at n = 300..1560)
 constexpr int CT = 1024;         // column kernel: one workgroup
 constexpr int ST = 512;          // column step: threads per workgroup (four rows each at n = 2048)
 constexpr int LGRID = 128;       // workgroups of the column step (each repeats the column work, then a
