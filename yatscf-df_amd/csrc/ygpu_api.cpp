@@ -3,8 +3,11 @@
 // Owns the device state of one DF-RHF engine: shells, Boys table, the packed
 // pair layout, the DF tensor B (rows i <= j only, aux index padded to a
 // multiple of 64), J^{-1/2}, and the per-iteration work buffers.  FP64
-// eigensolves use rocSOLVER dsyevd on the device; every product is one of the
-// MFMA kernels of ygpu_gemm.hip.
+// eigensolves reduce to tridiagonal form on the device (ygpu_eigen.hip up to
+// n = 272, ygpu_sytrd.hip up to 2048, rocSOLVER dsyevd beyond), solve T with the
+// device multisection / inverse iteration (density, n <= 272) or rocSOLVER dstedc,
+// and apply Q with the register back-transform; every product is one of the MFMA
+// kernels of ygpu_gemm.hip.
 #include "ygpu_internal.h"
 
 #include <rocsolver/rocsolver.h>

@@ -12,8 +12,9 @@
 //   2. the eigenpairs of T: for the density all eigenvalues by Sturm multisection and the
 //      n_occ lowest eigenvectors by inverse iteration (k_tridiag_*, below); for S^{-1/2}
 //      and J^{-1/2} all eigenpairs by rocSOLVER dstedc;
-//   3. k_backtransform_rows: u = Q z = H_0 (H_1 (... H_{n-3} z)) only for the
-//      eigenvectors needed, one wavefront each, reflectors staged through LDS.
+//   3. k_backtransform_cols: u = Q z = H_0 (H_1 (... H_{n-3} z)) only for the
+//      eigenvectors needed, one wavefront each with the vector in registers, reflectors
+//      staged through LDS (also for the blocked reduction's reflectors, n <= 2048).
 // rocSOLVER's dsyevd spends most of its time at these sizes in the
 // latency-bound panel kernels of its own tridiagonal reduction.
 //
@@ -312,65 +313,6 @@ __global__ __launch_bounds__(RTHR) void k_sytrd_reg(int n, double *__restrict__ 
     extract(A, a, b, n - 1, n, L.xs);
     __syncthreads();
     if (t == 0) d[n - 1] = L.xs[n - 1];
-}
-
-constexpr int BTV = 4;           // back-transform: vectors (wavefronts) per workgroup
-constexpr int BTK = 16;          // reflectors staged in LDS per block
-
-// U[j] = Q z_j for the first nvec eigenvectors of T; Z holds them column-major (z_j in
-// Z[j*n .. j*n+n), the dstedc layout), U row j receives u_j.  One wavefront per vector,
-// BTV vectors per workgroup; the reflectors are staged in LDS BTK at a time (the next
-// block's loads in registers while the current one is applied), so the sequential
-// reflector steps touch only LDS.
-__global__ __launch_bounds__(64 * BTV) void k_backtransform_rows(int n, int nvec, const double *__restrict__ A,
-                                                                 const double *__restrict__ tau,
-                                                                 const double *__restrict__ Z, double *__restrict__ U)
-{
-    constexpr int NT = 64 * BTV;
-    constexpr int PER = (BTK * YG_EIG_NMAX_SMALL + NT - 1) / NT;
-    __shared__ double sV[BTK][YG_EIG_NMAX_SMALL];
-    __shared__ double stau[BTK];
-    __shared__ double r[BTV][YG_EIG_NMAX_SMALL];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int j = blockIdx.x * BTV + wv;
-    const int jj = min(j, nvec - 1);            // surplus wavefronts repeat the last vector, store nothing
-    double *rv = r[wv];
-    for (int i = lane; i < n; i += 64) rv[i] = Z[(size_t) jj * n + i];
-    // block b holds reflectors ktop(b) - q, q < BTK, ktop(b) = n - 3 - BTK b; element c of
-    // reflector k sits in row k of A at column c (c > k), staged at sV[q][c]
-    double pre[PER];
-    auto load_block = [&](int ktop) {
-#pragma unroll
-        for (int g = 0; g < PER; ++g) {
-            const int idx = t + NT * g, q = idx / YG_EIG_NMAX_SMALL, c = idx % YG_EIG_NMAX_SMALL;
-            const int k = ktop - q;
-            pre[g] = (q < BTK && k >= 0 && c > k && c < n) ? A[(size_t) k * n + c] : 0.0;
-        }
-    };
-    const int ktop0 = n - 3;
-    if (ktop0 >= 0) load_block(ktop0);
-    for (int ktop = ktop0; ktop >= 0; ktop -= BTK) {
-        __syncthreads();                        // the previous block's reflectors are applied
-#pragma unroll
-        for (int g = 0; g < PER; ++g) {
-            const int idx = t + NT * g, q = idx / YG_EIG_NMAX_SMALL, c = idx % YG_EIG_NMAX_SMALL;
-            if (q < BTK) sV[q][c] = pre[g];
-        }
-        if (t < BTK) stau[t] = (ktop - t >= 0) ? tau[ktop - t] : 0.0;
-        __syncthreads();
-        if (ktop - BTK >= 0) load_block(ktop - BTK);
-        const int qn = min(BTK, ktop + 1);
-        for (int q = 0; q < qn; ++q) {
-            const int k = ktop - q;
-            const double *v = sV[q];
-            double s = 0.0;
-            for (int c = k + 1 + lane; c < n; c += 64) s += v[c] * rv[c];
-            const double f = stau[q] * wave_sum(s);
-            for (int c = k + 1 + lane; c < n; c += 64) rv[c] -= f * v[c];
-        }
-    }
-    if (j < nvec)
-        for (int i = lane; i < n; i += 64) U[(size_t) j * n + i] = rv[i];
 }
 
 constexpr int BLV = 4;           // back-transform above the one-workgroup size: vectors per workgroup
@@ -717,20 +659,6 @@ int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double
         return YGPU_ERR_INVALID;
     }
     hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(RTHR), 0, s, n, A, d, e, tau);
-    YG_HIP(hipGetLastError());
-    return YGPU_OK;
-}
-
-int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
-                          double *U)
-{
-    if (n < 1 || n > YG_EIG_NMAX_SMALL || nvec < 0 || nvec > n) {
-        ygpu_set_error(YGPU_ERR_INVALID, "back-transform: n = %d, nvec = %d out of range", n, nvec);
-        return YGPU_ERR_INVALID;
-    }
-    if (nvec == 0) return YGPU_OK;
-    hipLaunchKernelGGL(k_backtransform_rows, dim3((nvec + BTV - 1) / BTV), dim3(64 * BTV), 0, s, n, nvec, A, tau, Z,
-                       U);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -107,8 +107,8 @@ struct ygpu_ctx {
     int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
 
-    // small-n density eigensolver: reflector scalars and the occupied eigenvectors (rows)
-    DBuf d_tau, d_U, d_tri;
+    // own eigensolver paths: reflector scalars and tridiagonal workspace
+    DBuf d_tau, d_tri;
 };
 
 // --------------------------------------------------------------------------
@@ -136,10 +136,6 @@ int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                   
 // tridiagonal reduction of a row-major n x n symmetric matrix (upper triangle read) on one
 // workgroup (n <= YG_EIG_NMAX_SMALL); reflectors left in the rows of A
 int yg_sytrd_small(hipStream_t s, int n, double *A, double *d, double *e, double *tau);
-// U (nvec x n, rows) = Q z_j for the first nvec eigenvectors z_j of the tridiagonal matrix
-// (column-major in Z, as rocSOLVER dstedc leaves them)
-int yg_backtransform_rows(hipStream_t s, int n, int nvec, const double *A, const double *tau, const double *Z,
-                          double *U);
 // blocked tridiagonal reduction of a row-major symmetric n x n matrix (lower triangle read,
 // 3 <= n <= YG_SYTRD_BLOCKED_NMAX, leading dimension lda) in LAPACK dsytrd's lower layout of
 // the column-major view: d, e, tau, reflectors below the subdiagonal; ws holds
