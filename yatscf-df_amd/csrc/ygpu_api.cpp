@@ -797,9 +797,9 @@ try {
         // V_occ = Q Z_occ in place of Z_occ (columns of d_mat2, column-major)
         const size_t nvp = (size_t) (n_occ + 63) / 64 * 64;
         YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
-        YG_TRY(c->d_tri.ensure((3 * nvp + 2) * nbf * sizeof(double)));
+        YG_TRY(c->d_tri.ensure((nvp + 2) * nbf * sizeof(double)));
         double *tri_ws = c->d_tri.as<double>();
-        double *dT = tri_ws + 3 * nvp * nbf, *eT = dT + nbf;
+        double *dT = tri_ws + nvp * nbf, *eT = dT + nbf;
         YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), dT, eT, c->d_tau.as<double>()));
         YG_TRY(yg_tridiag_eig(c->stream, nbf, n_occ, dT, eT, c->d_eig.as<double>(), c->d_mat2.as<double>(), tri_ws));
         YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));

@@ -70,6 +70,21 @@ __device__ __forceinline__ double wave_sum(double v)
     return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
 }
 
+// four wave_sum reductions with their steps interleaved (the same operations per value)
+__device__ __forceinline__ void wave_sum4(double (&v)[4])
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += dpp_d<0xB1>(v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += dpp_d<0x4E>(v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += dpp_d<0x141>(v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += dpp_d<0x140>(v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = (lane_d(v[u], 0) + lane_d(v[u], 16)) + (lane_d(v[u], 32) + lane_d(v[u], 48));
+}
+
 // xs[i] = A(i, k) for k <= i < n from the threads of column class k % 32 (slot column KRJ)
 template <int KRJ>
 __device__ __forceinline__ void extract_col(const double (&A)[NSLOT], int a, int b, int k, int n, double *xs)
@@ -203,10 +218,19 @@ __device__ __forceinline__ void sytrd_phase(double (&A)[NSLOT], SytrdLds &L, int
         else extract_col<(R < NRJ - 1 ? R + 1 : R)>(A, a, b, k, n, L.xs);
         __syncthreads();
         if (t < 64) {
-            // dlarfg on x = A(k+1:n, k): H x = beta e_1, H = I - tau v v^T, v_0 = 1
+            // dlarfg on x = A(k+1:n, k): H x = beta e_1, H = I - tau v v^T, v_0 = 1; the LDS
+            // reads of each loop are issued together (fixed trip count)
+            constexpr int NQ = (NREG + 63) / 64;
             const double alpha = L.xs[k + 1];
+            double xv[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int i = k + 2 + t + 64 * q;
+                xv[q] = (i < n) ? L.xs[i] : 0.0;
+            }
             double ss = 0.0;
-            for (int i = k + 2 + t; i < n; i += 64) ss += L.xs[i] * L.xs[i];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) ss += xv[q] * xv[q];
             ss = wave_sum(ss);
             double tk = 0.0, beta = alpha, scale = 0.0;
             if (ss > 0.0) {
@@ -214,8 +238,13 @@ __device__ __forceinline__ void sytrd_phase(double (&A)[NSLOT], SytrdLds &L, int
                 tk = (beta - alpha) / beta;
                 scale = 1.0 / (alpha - beta);
             }
-            for (int i = t; i < NREG; i += 64)
-                L.vs[i] = (i == k + 1) ? 1.0 : (i > k + 1 && i < n) ? L.xs[i] * scale : 0.0;
+            // v over [k, NREG): entries below k are zero since earlier steps (v_{k-1} had its
+            // unit at k, cleared here)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int i = k + t + 64 * q;
+                if (i < NREG) L.vs[i] = (i == k + 1) ? 1.0 : (i > k + 1 && i < n) ? L.xs[i] * scale : 0.0;
+            }
             if (t == 0) {
                 L.sdiag = L.xs[k];
                 L.sbeta = beta;
@@ -461,6 +490,8 @@ __device__ __forceinline__ int sturm_count(int n, const double *d, const double 
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     int cnt = q < 0.0;
+    // unrolled so that the loads of d and e2 are issued ahead of the dependent chain
+#pragma unroll 8
     for (int i = 1; i < n; ++i) {
         q = (d[i] - x) - e2[i - 1] * recip(q);
         if (fabs(q) <= pivmin) q = -pivmin;
@@ -485,8 +516,11 @@ __global__ __launch_bounds__(NBIS) void k_tridiag_bisect(int n, const double *__
     const TriBounds bd = tri_bounds(n, sd, e);
     const int j = blockIdx.x;
     double lo = bd.gl, hi = bd.gu;
-    for (int r = 0; r < 16; ++r) {              // 257^7 > 1e16: converged after ~7 rounds
-        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + DBL_MIN) break;   // uniform
+    // dstebz's stopping rule: relative 2 ulp, or absolute ulp ||T|| (so that eigenvalues near 0
+    // stop too); 257^7 > 2 / ulp, so at most 7 rounds
+    const double atol = DBL_EPSILON * bd.tnorm;
+    for (int r = 0; r < 16; ++r) {
+        if (hi - lo <= fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), atol) + DBL_MIN) break;   // uniform
         const double x = lo + (hi - lo) * ((double) (t + 1) / (double) (NBIS + 1));
         const int c = sturm_count(n, sd, se2, x, bd.pivmin);
         const int Lw = __popcll(__ballot(c <= j));
@@ -512,70 +546,78 @@ __device__ __forceinline__ double start_value(int i, int j)
     return (double) h * (2.0 / 4294967296.0) - 1.0;
 }
 
-// Inverse iteration, lane j of the grid on eigenvalue j (lanes past nvec repeat the last
-// one; their columns are discarded).  T - sigma I = L D L^T without pivoting, tiny pivots
+constexpr int IVL = 16;          // inverse iteration: vectors per workgroup (lanes 0..15 of one wavefront)
+
+// Inverse iteration, lane l < IVL of workgroup b on eigenvalue j = IVL b + l (lanes past nvec
+// repeat the last one and store nothing).  T - sigma I = L D L^T without pivoting, tiny pivots
 // replaced by +-eps ||T||; three solves from a seeded start vector, each rescaled by its
-// largest element; the result normalised into column j of Zt (n x nvp).  The iterate
-// lives in LDS (one wavefront per workgroup), the factors in Lt, Dt.
+// largest element; the result normalised into column j of Zt (n x nvp).  The factors and the
+// iterate live in LDS, so the sequential sweeps wait on LDS latency only.
 __global__ __launch_bounds__(64) void k_tridiag_invit(int n, int nvec, int nvp, const double *__restrict__ d,
                                                       const double *__restrict__ e, const double *__restrict__ w,
-                                                      double *__restrict__ Zt, double *__restrict__ Lt,
-                                                      double *__restrict__ Dt)
+                                                      double *__restrict__ Zt)
 {
-    __shared__ double zs[YG_EIG_NMAX_SMALL * 64];
-    const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
+    __shared__ double zs[YG_EIG_NMAX_SMALL * IVL], ls[YG_EIG_NMAX_SMALL * IVL], ds[YG_EIG_NMAX_SMALL * IVL];
+    __shared__ double sd[YG_EIG_NMAX_SMALL], se[YG_EIG_NMAX_SMALL];
+    const int lane = threadIdx.x, j = blockIdx.x * IVL + lane;
+    for (int i = lane; i < n; i += 64) {
+        sd[i] = d[i];
+        se[i] = (i < n - 1) ? e[i] : 0.0;
+    }
+    __syncthreads();
+    const TriBounds bd = tri_bounds(n, sd, se);   // a wavefront-wide reduction: every lane takes part
+    if (lane >= IVL) return;
     const int jj = min(j, nvec - 1);
-    const TriBounds bd = tri_bounds(n, d, e);
     // dstein: inside a run of (nearly) equal eigenvalues the shifts are spread by 10 eps |lambda|
     const double pt = 10.0 * DBL_EPSILON, floor_ = bd.tnorm * DBL_EPSILON;
     int m = 0;
     while (jj - m - 1 >= 0 && w[jj - m] - w[jj - m - 1] < pt * fmax(fabs(w[jj - m]), floor_)) ++m;
     const double sig = w[jj] + m * pt * fmax(fabs(w[jj]), floor_);
     const double ptiny = DBL_EPSILON * bd.tnorm + DBL_MIN;
-    double D = d[0] - sig;
+    double D = sd[0] - sig;
     if (fabs(D) < ptiny) D = (D < 0.0) ? -ptiny : ptiny;
-    double id = 1.0 / D;
-    Dt[j] = id;
+    double id = recip(D);
+    ds[lane] = id;
     zs[lane] = start_value(0, jj);
+#pragma unroll 4
     for (int i = 1; i < n; ++i) {
-        const double ei = e[i - 1];
+        const double ei = se[i - 1];
         const double l = ei * id;
-        Lt[(size_t) (i - 1) * nvp + j] = l;
-        D = (d[i] - sig) - l * ei;
+        ls[(i - 1) * IVL + lane] = l;
+        D = (sd[i] - sig) - l * ei;
         if (fabs(D) < ptiny) D = (D < 0.0) ? -ptiny : ptiny;
-        id = 1.0 / D;
-        Dt[(size_t) i * nvp + j] = id;
-        zs[i * 64 + lane] = start_value(i, jj);
+        id = recip(D);
+        ds[i * IVL + lane] = id;
+        zs[i * IVL + lane] = start_value(i, jj);
     }
-    __threadfence_block();                      // the factors are read back by this lane only
     double s = 1.0;
     for (int it = 0; it < 3; ++it) {
         // forward: u_i = s x_i - l_{i-1} u_{i-1}, kept as u_i / D_i
         double u = s * zs[lane];
-        zs[lane] = u * Dt[j];
+        zs[lane] = u * ds[lane];
 #pragma unroll 8
         for (int i = 1; i < n; ++i) {
-            const size_t o = (size_t) i * nvp + j;
-            u = s * zs[i * 64 + lane] - Lt[o - nvp] * u;
-            zs[i * 64 + lane] = u * Dt[o];
+            u = s * zs[i * IVL + lane] - ls[(i - 1) * IVL + lane] * u;
+            zs[i * IVL + lane] = u * ds[i * IVL + lane];
         }
         // back: y_i = u_i / D_i - l_i y_{i+1}
-        double y = zs[(n - 1) * 64 + lane], amax = fabs(y);
+        double y = zs[(n - 1) * IVL + lane], amax = fabs(y);
 #pragma unroll 8
         for (int i = n - 2; i >= 0; --i) {
-            y = zs[i * 64 + lane] - Lt[(size_t) i * nvp + j] * y;
-            zs[i * 64 + lane] = y;
+            y = zs[i * IVL + lane] - ls[i * IVL + lane] * y;
+            zs[i * IVL + lane] = y;
             amax = fmax(amax, fabs(y));
         }
         s = 1.0 / amax;
     }
     double ss = 0.0;
     for (int i = 0; i < n; ++i) {
-        const double v = s * zs[i * 64 + lane];
+        const double v = s * zs[i * IVL + lane];
         ss += v * v;
     }
     const double f = s / sqrt(ss);
-    for (int i = 0; i < n; ++i) Zt[(size_t) i * nvp + j] = f * zs[i * 64 + lane];
+    if (j < nvec)
+        for (int i = 0; i < n; ++i) Zt[(size_t) i * nvp + j] = f * zs[i * IVL + lane];
 }
 
 constexpr int CLMAX = 64;        // clusters up to this size are orthogonalised in LDS
@@ -588,6 +630,7 @@ __global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp
                                                         double *__restrict__ Zt, double *__restrict__ Z)
 {
     __shared__ double cs[CLMAX * YG_EIG_NMAX_SMALL];
+    __shared__ double sdot[CLMAX];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int j0 = blockIdx.x;
     const TriBounds bd = tri_bounds(n, d, e);
@@ -603,21 +646,54 @@ __global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp
         }
         __syncthreads();
         if (wv == 0) {
+            // classical Gram-Schmidt applied twice, the vector in registers (element lane + 64 q):
+            // the dots of a pass against z_0 .. z_{v-1} are independent of each other
+            constexpr int NQ = (YG_EIG_NMAX_SMALL + 63) / 64;
             for (int v = 1; v < csz; ++v) {
                 double *zv = cs + (size_t) v * n;
+                double x[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const int i = lane + 64 * q;
+                    x[q] = (i < n) ? zv[i] : 0.0;
+                }
                 for (int pass = 0; pass < 2; ++pass) {
+                    for (int k0 = 0; k0 < v; k0 += 4) {
+                        double dot[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const double *zk = cs + (size_t) min(k0 + u, v - 1) * n;
+                            dot[u] = 0.0;
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                const int i = lane + 64 * q;
+                                dot[u] += (i < n) ? x[q] * zk[i] : 0.0;
+                            }
+                        }
+                        wave_sum4(dot);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (lane == 0 && k0 + u < v) sdot[k0 + u] = dot[u];
+                    }
                     for (int k = 0; k < v; ++k) {
                         const double *zk = cs + (size_t) k * n;
-                        double dot = 0.0;
-                        for (int i = lane; i < n; i += 64) dot += zv[i] * zk[i];
-                        dot = wave_sum(dot);
-                        for (int i = lane; i < n; i += 64) zv[i] -= dot * zk[i];
+                        const double dk = sdot[k];
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const int i = lane + 64 * q;
+                            if (i < n) x[q] -= dk * zk[i];
+                        }
                     }
                 }
                 double ss = 0.0;
-                for (int i = lane; i < n; i += 64) ss += zv[i] * zv[i];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) ss += x[q] * x[q];
                 const double f = 1.0 / sqrt(wave_sum(ss));
-                for (int i = lane; i < n; i += 64) zv[i] *= f;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const int i = lane + 64 * q;
+                    if (i < n) zv[i] = f * x[q];
+                }
             }
         }
         __syncthreads();
@@ -695,10 +771,10 @@ int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double
         return YGPU_ERR_INVALID;
     }
     const int nvp = (nvec + 63) / 64 * 64;
-    double *Zt = ws, *Lt = ws + (size_t) n * nvp, *Dt = ws + 2 * (size_t) n * nvp;
+    double *Zt = ws;
     hipLaunchKernelGGL(k_tridiag_bisect, dim3(n), dim3(NBIS), 0, s, n, d, e, w);
     YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tridiag_invit, dim3(nvp / 64), dim3(64), 0, s, n, nvec, nvp, d, e, w, Zt, Lt, Dt);
+    hipLaunchKernelGGL(k_tridiag_invit, dim3((nvec + IVL - 1) / IVL), dim3(64), 0, s, n, nvec, nvp, d, e, w, Zt);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tridiag_orth, dim3(nvec), dim3(NORTH), 0, s, n, nvec, nvp, d, e, w, Zt, Z);
     YG_HIP(hipGetLastError());

@@ -149,7 +149,7 @@ int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long 
                           long ldz);
 // all eigenvalues w (ascending) of the symmetric tridiagonal tridiag(e, d, e) and the
 // eigenvectors of the nvec lowest as the rows of Z (nvec x n); ws holds
-// 3 n ((nvec + 63) / 64 * 64) doubles
+// n ((nvec + 63) / 64 * 64) doubles
 int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double *e, double *w, double *Z,
                    double *ws);
 int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
