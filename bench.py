@@ -236,7 +236,9 @@ def main():
     hist = np.vstack([rows(r, mixed, mixed) for r in runs])
     if len(hist) == 0:
         hist = np.vstack([r["hist"][2:] for r in runs])
-    jk_ms = float(np.median(hist[:, 0] + hist[:, 1] + hist[:, 2] + hist[:, 3]) * 1e3)
+    # J and K run on two streams at once: the J/K figure is the wall time of the pair (start of
+    # J to F assembled); j_ms and k_ms are each timed on their own stream beside the other
+    jk_ms = float(np.median(hist[:, 5]) * 1e3)
     j_ms = float(np.median(hist[:, 0] + hist[:, 1]) * 1e3)
     k_ms = float(np.median(hist[:, 2] + hist[:, 3]) * 1e3)
     xb_ms = float(np.median(hist[:, 2]) * 1e3)

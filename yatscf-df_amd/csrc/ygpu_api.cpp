@@ -261,6 +261,9 @@ try {
     c->device = device;
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreate(&c->stream));
+    YG_HIP(hipStreamCreate(&c->side));
+    YG_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    YG_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     for (auto &e : c->ev) YG_HIP(hipEventCreate(&e));
     if (rocblas_create_handle(&c->rb) != rocblas_status_success ||
         rocblas_set_stream(c->rb, c->stream) != rocblas_status_success) {
@@ -306,6 +309,7 @@ try {
     (void) hipSetDevice(c->device);
     int status = YGPU_OK;
     hipError_t pend = hipStreamSynchronize(c->stream);
+    if (pend == hipSuccess && c->side) pend = hipStreamSynchronize(c->side);
     if (pend != hipSuccess) {
         ygpu_set_error(YGPU_ERR_HIP, "ygpu_destroy: pending device error %s", hipGetErrorString(pend));
         status = YGPU_ERR_HIP;
@@ -326,6 +330,9 @@ try {
         for (DBuf *x : bb) x->release();
     }
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+    if (c->fork) (void) hipEventDestroy(c->fork);
+    if (c->join) (void) hipEventDestroy(c->join);
+    if (c->side) (void) hipStreamDestroy(c->side);
     if (c->stream) (void) hipStreamDestroy(c->stream);
     delete c;
     return status;
@@ -561,6 +568,29 @@ static int unpack_to_reference(ygpu_ctx *c, const double *dsrc, double *host_out
     return YGPU_OK;
 }
 
+// Work forked onto the side stream is drained when the scope that forked it ends, on the
+// error paths too (nothing queued there outlives the call that queued it).
+struct SideDrain {
+    hipStream_t s;
+    ~SideDrain() { (void) hipStreamSynchronize(s); }
+};
+
+// side stream s waits for everything queued on the main stream so far
+static int fork_side(ygpu_ctx *c, hipStream_t s)
+{
+    YG_HIP(hipEventRecord(c->fork, c->stream));
+    YG_HIP(hipStreamWaitEvent(s, c->fork, 0));
+    return YGPU_OK;
+}
+
+// the main stream waits for everything queued on side stream s so far
+static int join_side(ygpu_ctx *c, hipStream_t s)
+{
+    YG_HIP(hipEventRecord(c->join, s));
+    YG_HIP(hipStreamWaitEvent(c->stream, c->join, 0));
+    return YGPU_OK;
+}
+
 int ygpu_build_df_tensor_ex(ygpu_ctx_t *c, double *pqA_out, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
 try {
     YG_TRY(check_ctx(c));
@@ -575,9 +605,11 @@ try {
     YG_TRY(c->d_Jpq.ensure((size_t) ld * ld * sizeof(double)));
     YG_TRY(c->d_mat0.ensure((size_t) ld * ld * sizeof(double)));
     YG_TRY(c->d_mat1.ensure((size_t) ld * ld * sizeof(double)));
+    // in sequence: beside the 3-centre integrals (which fill every CU) the latency-bound inverse
+    // square root of the metric measured longer by their whole duration
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
     YG_HIP(hipMemsetAsync(c->d_pqA.p, 0, tsz, c->stream));
-    YG_TRY(yg_launch_eri3(c, c->d_pqA.as<double>()));
+    YG_TRY(yg_launch_eri3(c, c->d_pqA.as<double>(), c->stream));
     YG_HIP(hipEventRecord(c->ev[1], c->stream));
     YG_HIP(hipMemsetAsync(c->d_mat0.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
     YG_TRY(yg_launch_eri2(c, c->d_mat0.as<double>(), (int) ld));
@@ -687,23 +719,28 @@ try {
         YG_TRY(c->d_Dprev.ensure(ms));
         YG_TRY(c->d_Jprev.ensure(ms));
         YG_TRY(c->d_dD.ensure(ms));
-        YG_HIP(hipEventRecord(c->ev[5], c->stream));
+        YG_TRY(c->d_Cocc32.ensure((size_t) nbf * n_occ * sizeof(float)));
+        // J on the side stream, K on the main stream (independent until F is assembled)
+        YG_TRY(fork_side(c, c->side));
+        SideDrain drain{c->side};
+        YG_HIP(hipEventRecord(c->ev[5], c->side));
         if (!c->jinc_valid) {
             // first build of a mixed run: full FP64 J, kept with its D as the base of the increments
-            YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
-            YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->stream));
-            YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->stream));
+            YG_TRY(yg_jbuild(c, c->side, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+            YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->side));
+            YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->side));
             c->jinc_valid = 1;
         } else {
             // J = J_prev + J(D - D_prev) from the FP32 tensor with FP64 arithmetic: the
             // rounding of the tensor enters only through the small difference density
-            YG_TRY(yg_delta(c->stream, nn, c->d_D.as<double>(), c->d_Dprev.as<double>(), c->d_dD.as<double>()));
-            YG_TRY(yg_jbuild(c, c->d_dD.as<double>(), c->d_J.as<double>(), c->d_B32.as<float>(), nullptr, nullptr));
-            YG_TRY(yg_accum(c->stream, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
+            YG_TRY(yg_delta(c->side, nn, c->d_D.as<double>(), c->d_Dprev.as<double>(), c->d_dD.as<double>()));
+            YG_TRY(yg_jbuild(c, c->side, c->d_dD.as<double>(), c->d_J.as<double>(), c->d_B32.as<float>(), nullptr,
+                             nullptr));
+            YG_TRY(yg_accum(c->side, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
         }
-        YG_HIP(hipEventRecord(c->ev[3], c->stream));
-        // K from BF16-split products; X in FP32 occupies the front half of the FP64 X buffer
-        YG_TRY(c->d_Cocc32.ensure((size_t) nbf * n_occ * sizeof(float)));
+        YG_HIP(hipEventRecord(c->ev[3], c->side));
+        // K from BF16-split products; X stored pre-split in the X buffer
+        YG_HIP(hipEventRecord(c->ev[8], c->stream));
         YG_TRY(yg_to_f32(c->stream, c->d_Cocc.as<double>(), c->d_Cocc32.as<float>(), (long) nbf * n_occ));
         YG_TRY(yg_xbuild_bf3(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
                              c->d_xpartner.as<int>(), c->d_Cocc32.as<float>(), c->d_B32.as<float>(),
@@ -712,10 +749,15 @@ try {
         YG_TRY(yg_syrk_bf3(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(), c->d_K.as<double>(),
                            c->d_Kpart32));
         YG_HIP(hipEventRecord(c->ev[6], c->stream));
+        YG_TRY(join_side(c, c->side));
     } else {
-        YG_HIP(hipEventRecord(c->ev[5], c->stream));
-        YG_TRY(yg_jbuild(c, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
-        YG_HIP(hipEventRecord(c->ev[3], c->stream));
+        // J on the side stream, K on the main stream (independent until F is assembled)
+        YG_TRY(fork_side(c, c->side));
+        SideDrain drain{c->side};
+        YG_HIP(hipEventRecord(c->ev[5], c->side));
+        YG_TRY(yg_jbuild(c, c->side, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+        YG_HIP(hipEventRecord(c->ev[3], c->side));
+        YG_HIP(hipEventRecord(c->ev[8], c->stream));
         YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
                          c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B.as<double>(), c->d_X.as<double>(),
                          c->max_row_len));
@@ -723,6 +765,7 @@ try {
         YG_TRY(yg_syrk_upper(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<double>(), c->d_K.as<double>(),
                              c->d_Kpart));
         YG_HIP(hipEventRecord(c->ev[6], c->stream));
+        YG_TRY(join_side(c, c->side));
     }
     YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
                             c->d_F.as<double>()));
@@ -736,7 +779,7 @@ try {
         tj2 = elapsed(c->ev[1], c->ev[2]);
         t->t_tempJ = tj1 * 1e-3;
         t->t_Jmat = tj2 * 1e-3;
-        t->t_tempK = elapsed(c->ev[3], c->ev[4]) * 1e-3;
+        t->t_tempK = elapsed(c->ev[8], c->ev[4]) * 1e-3;   // J (ev 5 .. 3) runs beside the K build
         t->t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
         t->t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
         t->t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;

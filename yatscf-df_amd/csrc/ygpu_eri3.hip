@@ -196,7 +196,7 @@ int launch_eri3_lc(int lc, const Eri3Args &g, hipStream_t s)
 
 }  // namespace
 
-int yg_launch_eri3(ygpu_ctx *c, double *dpqA)
+int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s)
 {
     const BasisDev &b = c->bas, &a = c->aux;
     if (b.max_am > 2 || a.max_am > 3) {
@@ -229,7 +229,8 @@ int yg_launch_eri3(ygpu_ctx *c, double *dpqA)
         }
     for (int lc = 0; lc < 4; ++lc) { off_p[lc] = all.size(); all.insert(all.end(), plist[lc].begin(), plist[lc].end()); }
     YG_TRY(c->d_gC.ensure(std::max<size_t>(1, all.size()) * sizeof(int)));
-    YG_HIP(hipMemcpyAsync(c->d_gC.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    YG_HIP(hipMemcpyAsync(c->d_gC.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    YG_HIP(hipStreamSynchronize(s));            // the host lists go out of scope on return
     const int *base = c->d_gC.as<int>();
     Eri3Args g{};
     g.bas = b.dev(); g.aux = a.dev();
@@ -247,14 +248,13 @@ int yg_launch_eri3(ygpu_ctx *c, double *dpqA)
                 g.plist = base + off_p[lc]; g.nP = (int) plist[lc].size();
                 g.npp_max = ppmax[la][lb];
                 int st = YGPU_OK;
-                if (la == 0) st = launch_eri3_lc<0, 0>(lc, g, c->stream);
-                else if (la == 1 && lb == 0) st = launch_eri3_lc<1, 0>(lc, g, c->stream);
-                else if (la == 1) st = launch_eri3_lc<1, 1>(lc, g, c->stream);
-                else if (lb == 0) st = launch_eri3_lc<2, 0>(lc, g, c->stream);
-                else if (lb == 1) st = launch_eri3_lc<2, 1>(lc, g, c->stream);
-                else st = launch_eri3_lc<2, 2>(lc, g, c->stream);
+                if (la == 0) st = launch_eri3_lc<0, 0>(lc, g, s);
+                else if (la == 1 && lb == 0) st = launch_eri3_lc<1, 0>(lc, g, s);
+                else if (la == 1) st = launch_eri3_lc<1, 1>(lc, g, s);
+                else if (lb == 0) st = launch_eri3_lc<2, 0>(lc, g, s);
+                else if (lb == 1) st = launch_eri3_lc<2, 1>(lc, g, s);
+                else st = launch_eri3_lc<2, 2>(lc, g, s);
                 if (st != YGPU_OK) return st;
             }
-    YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;
 }

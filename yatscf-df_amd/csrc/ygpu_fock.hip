@@ -342,7 +342,8 @@ int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, 
     return YGPU_OK;
 }
 
-int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms)
+int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,
+              float *t_Jmat_ms)
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
@@ -353,30 +354,30 @@ int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float
     ngroup = std::min<long>(ngroup, 65535);
     YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
     YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
-    YG_HIP(hipEventRecord(c->ev[0], c->stream));
+    YG_HIP(hipEventRecord(c->ev[0], s));
     if (B32)
-        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, c->stream, B32, ld, n_up,
+        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s, B32, ld, n_up,
                            c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
     else
-        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, c->stream,
+        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s,
                            c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf,
                            c->d_tempJpart.as<double>());
     YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, c->stream, c->d_tempJpart.as<double>(),
+    hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, c->d_tempJpart.as<double>(),
                        (int) ngroup, ld, c->d_tempJ.as<double>());
     YG_HIP(hipGetLastError());
-    YG_HIP(hipEventRecord(c->ev[1], c->stream));
-    YG_HIP(hipMemsetAsync(dJ, 0, (size_t) nbf * nbf * sizeof(double), c->stream));
+    YG_HIP(hipEventRecord(c->ev[1], s));
+    YG_HIP(hipMemsetAsync(dJ, 0, (size_t) nbf * nbf * sizeof(double), s));
     const int rpw = 4;
     const long nblk = (n_up + 4L * rpw - 1) / (4L * rpw);
     if (B32)
-        hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(256), ld * sizeof(double), c->stream, B32, ld, n_up,
+        hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(256), ld * sizeof(double), s, B32, ld, n_up,
                            c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
     else
-        hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(256), ld * sizeof(double), c->stream, c->d_B.as<double>(),
+        hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(256), ld * sizeof(double), s, c->d_B.as<double>(),
                            ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
     YG_HIP(hipGetLastError());
-    YG_HIP(hipEventRecord(c->ev[2], c->stream));
+    YG_HIP(hipEventRecord(c->ev[2], s));
     if (t_tempJ_ms || t_Jmat_ms) {
         YG_HIP(hipEventSynchronize(c->ev[2]));
         if (t_tempJ_ms) YG_HIP(hipEventElapsedTime(t_tempJ_ms, c->ev[0], c->ev[1]));

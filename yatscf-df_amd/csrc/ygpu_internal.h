@@ -63,8 +63,10 @@ struct BasisDev {
 struct ygpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;   // independent work beside the main stream (the J build)
+    hipEvent_t fork = nullptr, join = nullptr;
     rocblas_handle rb = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     DBuf boys;                    // Boys table
     BasisDev bas, aux;
     int natoms = 0;
@@ -116,7 +118,8 @@ struct ygpu_ctx {
 int yg_launch_ovlp_hcore(ygpu_ctx *c, double *dS, double *dH);
 int yg_launch_schwarz(ygpu_ctx *c, int quirk_a5, double *d_bf_scr, double *d_df_scr);
 int yg_launch_eri2(ygpu_ctx *c, double *dJpq, int ld);
-int yg_launch_eri3(ygpu_ctx *c, double *dpqA);
+// on stream s; returns once the kernels are queued (the host pair lists are copied before)
+int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s);
 
 // C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
@@ -129,8 +132,10 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
 // upper-triangle K = X X^T with split-K partials in Kpart, result in K (full, mirrored)
 int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part);
-// J from density dD (upper triangle of dJ; the rest zero); B32 != nullptr reads the FP32 copy
-int yg_jbuild(ygpu_ctx *c, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms, float *t_Jmat_ms);
+// J from density dD (upper triangle of dJ; the rest zero) on stream s, timed by ev[0..2];
+// B32 != nullptr reads the FP32 copy
+int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,
+              float *t_Jmat_ms);
 int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
 int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                      // A = (A + A^T) / 2
 // tridiagonal reduction of a row-major n x n symmetric matrix (upper triangle read) on one
