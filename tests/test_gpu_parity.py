@@ -253,11 +253,12 @@ def spectrum_case(n, kind, seed):
 
 
 # 272 is the largest size of the one-workgroup register-resident tridiagonal reduction;
-# 273 .. 2048 take the blocked two-kernel reduction with rocSOLVER dstedc and dormtr
+# 273 .. 2048 take the blocked reduction with rocSOLVER dstedc; the register back-transform
+# has variants for n <= 512, 1024, 1536 and 2048 (1600 and 2048 reach the last)
 @pytest.mark.parametrize("n,kind", [(1, "dense"), (2, "dense"), (3, "dense"), (17, "cluster"), (31, "dense"),
                                     (32, "dense"), (33, "diag"), (64, "cluster"), (200, "diag"), (255, "dense"),
                                     (271, "cluster"), (272, "dense"), (273, "dense"), (300, "cluster"),
-                                    (600, "cluster")])
+                                    (600, "cluster"), (1600, "cluster"), (2048, "dense")])
 def test_density_eigensolver_sizes(ygpu, n, kind):
     log("case: density eigensolver n=%d %s" % (n, kind))
     F, nocc = spectrum_case(n, kind, 1000 + n)
@@ -281,8 +282,8 @@ def test_density_eigensolver_sizes(ygpu, n, kind):
 
 
 # S^-1/2 and metrics: one-workgroup reduction + dstedc up to 272, the blocked reduction +
-# dstedc + dormtr up to 2048, dsyevd above
-@pytest.mark.parametrize("n", [1, 5, 64, 260, 272, 273, 990, 1500])
+# dstedc + the register back-transform up to 2048, dsyevd above
+@pytest.mark.parametrize("n", [1, 5, 64, 260, 272, 273, 990, 1500, 2048, 2049])
 def test_sym_inv_sqrt_sizes(ygpu, n):
     log("case: symmetric inverse square root n=%d" % n)
     rng = np.random.RandomState(77 + n)
