@@ -121,7 +121,9 @@ int ygpu_set_df_tensor(ygpu_ctx_t *ctx, const double *df_tensor);
 
 /* J, K and F = Hcore + 2J - K from D and C_occ (TinySCF_build_FockMat,
  * build_Fock.c:349-396).  D, Hcore: nbf x nbf (Hcore NULL: the copy cached by
- * ygpu_ovlp_hcore); Cocc: nbf x n_occ.  J, K (nullable) and F receive full
+ * ygpu_ovlp_hcore); Cocc: nbf x n_occ.  D and Cocc both NULL: the D and C_occ
+ * of the last ygpu_build_density on this context, already on the device (no
+ * host copy; an SCF loop's hand-off).  J, K (nullable) and F receive full
  * symmetric matrices; J entries of screened pairs are zero. */
 int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *Cocc,
                     const double *Hcore, double *J, double *K, double *F,
@@ -146,7 +148,8 @@ int ygpu_set_fock_precision(ygpu_ctx_t *ctx, int mode);
 /* Density from the orthogonal-basis Fock matrix F_orth = X^T F X
  * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
  * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.
- * X NULL: the copy cached by ygpu_set_overlap.  ev_idx (nullable) receives
+ * X NULL: the copy cached by ygpu_set_overlap.  F_orth NULL: the output of the
+ * last ygpu_diis_transform / ygpu_diis_extrapolate, already on the device.  ev_idx (nullable) receives
  * the order used (identity for the ascending eigenvalues of the eigensolver). */
 int ygpu_build_density(ygpu_ctx_t *ctx, int n_occ, const double *F_orth, const double *X,
                        double *D, double *Cocc, double *eigval, int *ev_idx);
@@ -160,10 +163,13 @@ int ygpu_set_overlap(ygpu_ctx_t *ctx, const double *S, const double *X);
 /* Commutator DIIS with the history on the device (DIIS.c:12-123); the caller
  * keeps the reference's bookkeeping (DIIS_len, the replaced slot, B_mat,
  * DIIS_bmax) and solves the small system.
- *   ygpu_diis_transform: F_orth = X^T F X (iterations 0 and 1, DIIS.c:30-40).
+ *   ygpu_diis_transform: F_orth = X^T F X (iterations 0 and 1, DIIS.c:30-40);
+ *     F NULL: the last ygpu_build_fock's F on the device.
  *   ygpu_diis_update: residual R = X^T (F D S - S D F) X stored in history
  *     slot `slot`, dots[j] = <R, R_j> for j < len, F_orth = X^T F X stored as
- *     the slot's Fock matrix and returned (DIIS.c:51-110).
+ *     the slot's Fock matrix and returned (DIIS.c:51-110); F and D NULL: the
+ *     last ygpu_build_fock's F and the D it was built from; F_orth NULL: not
+ *     returned.  max |R| is formed in the same pass (ygpu_diis_error).
  *   ygpu_diis_extrapolate: F_out = sum_{i<len} coef[i] F_orth_i (DIIS.c:119-122).
  * len <= YGPU_MAX_DIIS. */
 #define YGPU_MAX_DIIS 10

@@ -367,6 +367,45 @@ def test_device_diis(ygpu, name):
         assert np.max(np.abs(Fx - ref)) < 1e-11
 
 
+def test_device_resident_handoffs(ygpu):
+    """The SCF loop's device-resident calls (NULL host inputs: D and C_occ of the last density
+    build, F of the last Fock build, F_orth of the last DIIS step) give bit-identical results to
+    the same sequence with every input copied from the host."""
+    log("case: device-resident hand-offs c10h22")
+    st = oracle_state("c10h22")
+    o = st.o
+    S, X, H = o.mat("S_mat").copy(), o.mat("X_mat").copy(), o.mat("Hcore_mat").copy()
+    nocc = st.C0.shape[1]
+    out = []
+    for resident in (False, True):
+        seq = []
+        with make_engine(ygpu, st) as e:
+            set_oracle_sparsity(e, st)
+            e.set_df_tensor(o.d("df_tensor"))
+            e.set_overlap(S, X)
+            r = e.build_fock(st.D0, st.C0, H)                       # iteration 0: the host guess
+            Fo = e.diis_transform(None if resident else r["F"])
+            D, C, ev, _ = e.build_density(None if resident else Fo, None, nocc)
+            seq += [r["F"], r["J"], r["K"], Fo, D, C, ev]
+            r = e.build_fock(None if resident else D, None if resident else C, H, n_occ=nocc)
+            Fo = e.diis_transform(None if resident else r["F"])
+            D1, C1, ev, _ = e.build_density(None if resident else Fo, None, nocc)
+            seq += [r["F"], r["J"], r["K"], Fo, D1, C1, ev]
+            r = e.build_fock(None if resident else D1, None if resident else C1, H, n_occ=nocc)
+            dots, Fo = e.diis_update(0, 1, None if resident else r["F"], None if resident else D1,
+                                     want_forth=not resident)
+            err = e.diis_error(0)
+            Fx = e.diis_extrapolate(np.array([1.0]))
+            D2, C2, ev, _ = e.build_density(None if resident else Fx, None, nocc)
+            seq += [r["F"], r["J"], r["K"], dots, np.array([err]), Fx, D2, C2, ev]
+        out.append(seq)
+    for k, (a, b) in enumerate(zip(*out)):
+        assert np.array_equal(a, b), k
+    # and the DIIS error is max |R| of the residual just stored
+    R = X.T @ (out[0][-9] @ out[0][11] @ S - S @ out[0][11] @ out[0][-9]) @ X
+    assert abs(out[0][-5][0] - np.abs(R).max()) < 1e-12 * np.abs(R).max() + 1e-15
+
+
 @pytest.mark.parametrize("n_occ", [5, 100, 171, 250, 300, 330])
 def test_k_build_tile_variants(ygpu, n_occ):
     """K = sum_P (B_P C)(B_P C)^T for synthetic occupied blocks of every width the X-build

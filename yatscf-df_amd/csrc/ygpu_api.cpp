@@ -689,8 +689,10 @@ int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Coc
                     double *K, double *F, ygpu_fock_timing_t *t)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_df || n_occ <= 0 || !D || !Cocc || !F || (!H && !c->have_H)) {
-        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built or bad arguments");
+    const bool resident = !D && !Cocc;         // the last density build's D and C_occ, on the device
+    if (!c->have_df || n_occ <= 0 || (!resident && (!D || !Cocc)) || !F || (!H && !c->have_H) ||
+        (resident && !c->density_valid)) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built, no device density or bad arguments");
         return YGPU_ERR_INVALID;
     }
     const int nbf = c->nbf;
@@ -702,12 +704,15 @@ try {
     YG_TRY(c->d_F.ensure(ms));
     YG_TRY(c->d_Cocc.ensure((size_t) nbf * n_occ * sizeof(double)));
     YG_TRY(c->d_X.ensure((size_t) nbf * n_occ * c->naux_pad * sizeof(double)));
-    YG_HIP(hipMemcpyAsync(c->d_D.p, D, ms, hipMemcpyHostToDevice, c->stream));
+    if (!resident) YG_HIP(hipMemcpyAsync(c->d_D.p, D, ms, hipMemcpyHostToDevice, c->stream));
     if (H) {
         YG_HIP(hipMemcpyAsync(c->d_H.p, H, ms, hipMemcpyHostToDevice, c->stream));
         c->have_H = 1;
     }
-    YG_HIP(hipMemcpyAsync(c->d_Cocc.p, Cocc, (size_t) nbf * n_occ * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!resident)
+        YG_HIP(hipMemcpyAsync(c->d_Cocc.p, Cocc, (size_t) nbf * n_occ * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    c->fock_valid = 0;
     float tj1 = 0.f, tj2 = 0.f;
     const long nn = (long) nbf * nbf;
     if (c->fock_mode == YGPU_FOCK_MIXED) {
@@ -774,6 +779,7 @@ try {
     if (J) YG_HIP(hipMemcpyAsync(J, c->d_J.p, ms, hipMemcpyDeviceToHost, c->stream));
     if (K) YG_HIP(hipMemcpyAsync(K, c->d_K.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    c->fock_valid = 1;
     if (t) {
         tj1 = elapsed(c->ev[0], c->ev[1]);
         tj2 = elapsed(c->ev[1], c->ev[2]);
@@ -814,8 +820,9 @@ int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double 
 try {
     YG_TRY(check_ctx(c));
     const int nbf = c->nbf;
-    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || !Fo || (!X && !c->have_S) || !D || !Cocc || !eigval) {
-        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_density: bad arguments");
+    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || (!Fo && !c->fcur_valid) || (!X && !c->have_S) || !D || !Cocc ||
+        !eigval) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_density: bad arguments / no device F_orth");
         return YGPU_ERR_INVALID;
     }
     const size_t ms = (size_t) nbf * nbf * sizeof(double);
@@ -827,7 +834,11 @@ try {
     YG_TRY(c->d_eig.ensure(nbf * sizeof(double)));
     YG_TRY(c->d_evE.ensure(nbf * sizeof(double)));
     YG_TRY(c->d_info.ensure(sizeof(int)));
-    YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
+    // Fo NULL: the last DIIS transform or extrapolation, already on the device
+    if (Fo) YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
+    else YG_HIP(hipMemcpyAsync(c->d_mat0.p, c->d_Fcur.p, ms, hipMemcpyDeviceToDevice, c->stream));
+    c->fock_valid = 0;                          // d_D and d_Cocc are overwritten below
+    c->density_valid = 0;
     const double *dX = c->d_Xo.as<double>();
     if (X) {
         YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
@@ -888,6 +899,7 @@ try {
     }
     if (ev_idx)
         for (int i = 0; i < nbf; ++i) ev_idx[i] = i;
+    c->density_valid = 1;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -926,6 +938,7 @@ static int diis_buffers(ygpu_ctx *c)
     YG_TRY(c->d_Din.ensure(ms));
     YG_TRY(c->d_T1.ensure(ms));
     YG_TRY(c->d_T2.ensure(ms));
+    YG_TRY(c->d_Fcur.ensure(ms));
     YG_TRY(c->d_diisR.ensure(ms * YGPU_MAX_DIIS));
     YG_TRY(c->d_diisF.ensure(ms * YGPU_MAX_DIIS));
     return YGPU_OK;
@@ -934,16 +947,21 @@ static int diis_buffers(ygpu_ctx *c)
 int ygpu_diis_transform(ygpu_ctx_t *c, const double *F, double *F_orth)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_S || !F || !F_orth) {
-        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_transform: call ygpu_set_overlap first / null argument");
+    if (!c->have_S || (!F && !c->fock_valid) || !F_orth) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_transform: call ygpu_set_overlap first / no device Fock / null argument");
         return YGPU_ERR_INVALID;
     }
     const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
     YG_TRY(diis_buffers(c));
-    YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
-    YG_TRY(orth_transform(c, c->d_Fin.as<double>(), c->d_T2.as<double>()));
-    YG_HIP(hipMemcpyAsync(F_orth, c->d_T2.p, ms, hipMemcpyDeviceToHost, c->stream));
+    const double *dF = c->d_F.as<double>();     // F NULL: the last Fock build's
+    if (F) {
+        YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
+        dF = c->d_Fin.as<double>();
+    }
+    YG_TRY(orth_transform(c, dF, c->d_Fcur.as<double>()));
+    YG_HIP(hipMemcpyAsync(F_orth, c->d_Fcur.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    c->fcur_valid = 1;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -953,8 +971,8 @@ int ygpu_diis_update(ygpu_ctx_t *c, int slot, int len, const double *F, const do
                      double *F_orth)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_S || !F || !D || !dots || !F_orth || slot < 0 || slot >= YGPU_MAX_DIIS || len <= slot ||
-        len > YGPU_MAX_DIIS) {
+    if (!c->have_S || ((!F || !D) && !c->fock_valid) || !dots || slot < 0 || slot >= YGPU_MAX_DIIS ||
+        len <= slot || len > YGPU_MAX_DIIS) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_update: bad arguments (slot %d, len %d)", slot, len);
         return YGPU_ERR_INVALID;
     }
@@ -963,29 +981,45 @@ try {
     const size_t ms = (size_t) nn * sizeof(double);
     YG_TRY(diis_buffers(c));
     const int nchunk = 64;
-    YG_TRY(c->d_dots.ensure((size_t) YGPU_MAX_DIIS * nchunk * sizeof(double)));
-    YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
-    YG_HIP(hipMemcpyAsync(c->d_Din.p, D, ms, hipMemcpyHostToDevice, c->stream));
+    YG_TRY(c->d_dots.ensure(((size_t) YGPU_MAX_DIIS * nchunk + 1) * sizeof(double)));
+    // F / D NULL: the last Fock build's F and the density it was built from
+    const double *dF = c->d_F.as<double>(), *dD = c->d_D.as<double>();
+    if (F) {
+        YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
+        dF = c->d_Fin.as<double>();
+    }
+    if (D) {
+        YG_HIP(hipMemcpyAsync(c->d_Din.p, D, ms, hipMemcpyHostToDevice, c->stream));
+        dD = c->d_Din.as<double>();
+    }
     double *T1 = c->d_T1.as<double>(), *T2 = c->d_T2.as<double>();
     double *R = c->d_diisR.as<double>() + (size_t) slot * nn;
     double *F0 = c->d_diisF.as<double>() + (size_t) slot * nn;
     // FDS = F D S (DIIS.c:51-54); R = X^T (FDS - FDS^T) X (DIIS.c:57-65)
-    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_Fin.as<double>(), n, c->d_Din.as<double>(), n, 0.0, T1, n));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, dF, n, dD, n, 0.0, T1, n));
     YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T1, n, c->d_S.as<double>(), n, 0.0, T2, n));
     YG_TRY(yg_antisym(c->stream, n, T2, T1));
     YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, T1, n, 0.0, T2, n));
     YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T2, n, c->d_Xo.as<double>(), n, 0.0, R, n));
     YG_TRY(yg_dots(c->stream, nn, R, c->d_diisR.as<double>(), nn, len, c->d_dots.as<double>(), nchunk));
-    YG_TRY(orth_transform(c, c->d_Fin.as<double>(), F0));
-    std::vector<double> part((size_t) len * nchunk);
-    YG_HIP(hipMemcpyAsync(part.data(), c->d_dots.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    YG_HIP(hipMemcpyAsync(F_orth, F0, ms, hipMemcpyDeviceToHost, c->stream));
+    // max |R| of this slot (ygpu_diis_error) in the same round trip
+    double *dmax = c->d_dots.as<double>() + (size_t) YGPU_MAX_DIIS * nchunk;
+    YG_TRY(yg_maxabs(c->stream, nn, R, dmax));
+    YG_TRY(orth_transform(c, dF, F0));
+    std::vector<double> part((size_t) len * nchunk + 1);
+    YG_HIP(hipMemcpyAsync(part.data(), c->d_dots.p, (size_t) len * nchunk * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    YG_HIP(hipMemcpyAsync(part.data() + (size_t) len * nchunk, dmax, sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    if (F_orth) YG_HIP(hipMemcpyAsync(F_orth, F0, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     for (int j = 0; j < len; ++j) {
         double sacc = 0.0;
         for (int k = 0; k < nchunk; ++k) sacc += part[(size_t) j * nchunk + k];
         dots[j] = sacc;
     }
+    c->diis_err[slot] = part[(size_t) len * nchunk];
+    c->diis_err_ok[slot] = 1;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -997,6 +1031,10 @@ try {
     if (slot < 0 || slot >= YGPU_MAX_DIIS || !max_abs || !c->d_diisR.p) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_error: bad arguments (slot %d)", slot);
         return YGPU_ERR_INVALID;
+    }
+    if (c->diis_err_ok[slot]) {               // formed by the slot's update
+        *max_abs = c->diis_err[slot];
+        return YGPU_OK;
     }
     const long nn = (long) c->nbf * c->nbf;
     YG_TRY(c->d_dots.ensure(sizeof(double)));
@@ -1016,9 +1054,11 @@ try {
         return YGPU_ERR_INVALID;
     }
     const long nn = (long) c->nbf * c->nbf;
-    YG_TRY(yg_lincomb(c->stream, nn, c->d_diisF.as<double>(), nn, len, coef, c->d_T1.as<double>()));
-    YG_HIP(hipMemcpyAsync(F_out, c->d_T1.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    YG_TRY(c->d_Fcur.ensure(nn * sizeof(double)));
+    YG_TRY(yg_lincomb(c->stream, nn, c->d_diisF.as<double>(), nn, len, coef, c->d_Fcur.as<double>()));
+    YG_HIP(hipMemcpyAsync(F_out, c->d_Fcur.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    c->fcur_valid = 1;
     return YGPU_OK;
 } catch (...) {
     return exception_status();

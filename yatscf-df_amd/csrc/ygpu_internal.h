@@ -111,6 +111,15 @@ struct ygpu_ctx {
 
     // own eigensolver paths: reflector scalars and tridiagonal workspace
     DBuf d_tau, d_tri;
+
+    // device-resident hand-offs between the calls of one SCF iteration (NULL host arguments):
+    // d_F, d_D, d_Cocc hold the last Fock build's F and its inputs (fock_valid); after a
+    // density build d_D, d_Cocc hold its outputs (density_valid); d_Fcur holds the last
+    // DIIS transform or extrapolation (fcur_valid)
+    int fock_valid = 0, density_valid = 0, fcur_valid = 0;
+    DBuf d_Fcur;
+    double diis_err[YGPU_MAX_DIIS] = {};   // max |R| per history slot, from its update
+    int diis_err_ok[YGPU_MAX_DIIS] = {};
 };
 
 // --------------------------------------------------------------------------

@@ -242,6 +242,7 @@ void TinySCF_do_SCF(TinySCF_t T)
         printf("Nuclear energy = %.10lf\n", T->nuc_energy);
     }
     T->iter = 0;
+    T->dev_resident = 0;                        /* iteration 0 starts from the host's guess */
     T->converged = 0;
     T->diis_error = -1.0;
     T->mixed_phase = T->mixed;
@@ -261,6 +262,7 @@ void TinySCF_do_SCF(TinySCF_t T)
         TinySCF_DIIS(T);
         double st2 = get_wtime_sec();
         TinySCF_build_DenMat(T);
+        T->dev_resident = 1;                    /* D, C_occ now on the device for the next build */
         double st3 = get_wtime_sec();
         T->t_diis = st2 - st1;
         T->t_density = st3 - st2;
@@ -314,6 +316,7 @@ void TinySCF_do_SCF(TinySCF_t T)
         }
         T->iter++;
     }
+    T->dev_resident = 0;                        /* calls outside the loop copy their host inputs */
     if (T->mixed) YS_GPU(ygpu_set_fock_precision((ygpu_ctx_t *) T->gpu, YGPU_FOCK_FP64));
     T->energy_delta = delta;
     T->converged = delta < T->ene_tol;

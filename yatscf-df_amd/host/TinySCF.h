@@ -122,6 +122,10 @@ struct TinySCF_struct
 #define YS_MAX_HIST 512
     int    n_hist;          /* per-iteration history of the phase timers (s) */
     double t_hist[YS_MAX_HIST][6];
+    int    dev_resident;    /* set by TinySCF_do_SCF while the device holds the current D, C_occ
+                               and F: the Fock, DIIS and density calls then pass NULL for them
+                               and skip the host->device copies (the host matrices are still
+                               written as the reference's are) */
 };
 
 typedef struct TinySCF_struct* TinySCF_t;

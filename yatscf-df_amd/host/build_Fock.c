@@ -17,8 +17,9 @@ void TinySCF_build_FockMat(TinySCF_t T)
     ygpu_fock_timing_t t;
     double st = get_wtime_sec();
     /* Hcore: the device copy made by TinySCF_compute_Hcore_Ovlp_mat */
-    YS_GPU(ygpu_build_fock((ygpu_ctx_t *) T->gpu, T->n_occ, T->D_mat, T->Cocc_mat, NULL, T->J_mat, T->K_mat,
-                           T->F_mat, &t));
+    const int res = T->dev_resident;            /* D, C_occ of the last density build on the device */
+    YS_GPU(ygpu_build_fock((ygpu_ctx_t *) T->gpu, T->n_occ, res ? NULL : T->D_mat, res ? NULL : T->Cocc_mat, NULL,
+                           T->J_mat, T->K_mat, T->F_mat, &t));
     T->t_fock[0] = t.t_tempJ;
     T->t_fock[1] = t.t_Jmat;
     T->t_fock[2] = t.t_tempK;

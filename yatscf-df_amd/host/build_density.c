@@ -33,7 +33,8 @@ void quickSort_eigval(double *eigval, int *ev_idx, int l, int r)
 
 void TinySCF_build_DenMat(TinySCF_t T)
 {
-    /* X: the device copy cached by TinySCF_compute_Hcore_Ovlp_mat */
-    YS_GPU(ygpu_build_density((ygpu_ctx_t *) T->gpu, T->n_occ, T->F_mat, NULL, T->D_mat, T->Cocc_mat, T->eigval,
-                              T->ev_idx));
+    /* X: the device copy cached by TinySCF_compute_Hcore_Ovlp_mat; F: inside the SCF loop the
+     * DIIS step's output, already on the device */
+    YS_GPU(ygpu_build_density((ygpu_ctx_t *) T->gpu, T->n_occ, T->dev_resident ? NULL : T->F_mat, NULL, T->D_mat,
+                              T->Cocc_mat, T->eigval, T->ev_idx));
 }

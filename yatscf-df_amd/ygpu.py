@@ -274,11 +274,15 @@ class Engine:
         B = np.ascontiguousarray(B, dtype=np.float64)
         check(self.L.ygpu_set_df_tensor(self.h, _d(B)))
 
-    def build_fock(self, D, Cocc, H, want_JK=True):
-        n_occ = Cocc.shape[1]
-        D = np.ascontiguousarray(D, dtype=np.float64)
-        Cocc = np.ascontiguousarray(Cocc, dtype=np.float64)
-        H = np.ascontiguousarray(H, dtype=np.float64)
+    def build_fock(self, D, Cocc, H, want_JK=True, n_occ=None):
+        """D = Cocc = None: the last density build's D and C_occ, already on the device (give n_occ);
+        H = None: the copy cached by ovlp_hcore."""
+        if Cocc is not None:
+            n_occ = Cocc.shape[1]
+            D = np.ascontiguousarray(D, dtype=np.float64)
+            Cocc = np.ascontiguousarray(Cocc, dtype=np.float64)
+        if H is not None:
+            H = np.ascontiguousarray(H, dtype=np.float64)
         F = np.zeros((self.nbf, self.nbf))
         J = np.zeros((self.nbf, self.nbf)) if want_JK else None
         K = np.zeros((self.nbf, self.nbf)) if want_JK else None
@@ -291,9 +295,12 @@ class Engine:
         check(self.L.ygpu_set_fock_precision(self.h, int(mode)))
 
     def build_density(self, Fo, X, n_occ):
-        Fo = np.ascontiguousarray(Fo, dtype=np.float64)
-        X = np.ascontiguousarray(X, dtype=np.float64)
-        n = Fo.shape[0]
+        """Fo = None: the last DIIS transform / extrapolation on the device; X = None: the cached X."""
+        if Fo is not None:
+            Fo = np.ascontiguousarray(Fo, dtype=np.float64)
+        if X is not None:
+            X = np.ascontiguousarray(X, dtype=np.float64)
+        n = Fo.shape[0] if Fo is not None else self.nbf
         D = np.zeros((n, n))
         C = np.zeros((n, n_occ))
         ev = np.zeros(n)
@@ -319,16 +326,23 @@ class Engine:
         check(self.L.ygpu_set_overlap(self.h, _d(S), _d(X)))
 
     def diis_transform(self, F):
-        F = np.ascontiguousarray(F, dtype=np.float64)
-        out = np.zeros_like(F)
+        """F = None: the last Fock build's F on the device."""
+        if F is not None:
+            F = np.ascontiguousarray(F, dtype=np.float64)
+        n = F.shape[0] if F is not None else self.nbf
+        out = np.zeros((n, n))
         check(self.L.ygpu_diis_transform(self.h, _d(F), _d(out)))
         return out
 
-    def diis_update(self, slot, length, F, D):
-        F = np.ascontiguousarray(F, dtype=np.float64)
-        D = np.ascontiguousarray(D, dtype=np.float64)
+    def diis_update(self, slot, length, F, D, want_forth=True):
+        """F = D = None: the last Fock build's F and its density on the device."""
+        if F is not None:
+            F = np.ascontiguousarray(F, dtype=np.float64)
+        if D is not None:
+            D = np.ascontiguousarray(D, dtype=np.float64)
         dots = np.zeros(length)
-        out = np.zeros_like(F)
+        n = F.shape[0] if F is not None else self.nbf
+        out = np.zeros((n, n)) if want_forth else None
         check(self.L.ygpu_diis_update(self.h, slot, length, _d(F), _d(D), _d(dots), _d(out)))
         return dots, out
 
