@@ -12,7 +12,8 @@
 // One kernel per column (k_latrd_step): each of its workgroups repeats steps 1-3 and the
 // small products W^T v, V^T v for all rows (loads issued up front, values shared through
 // LDS only), then takes its share of the step-4 GEMV rows, one wavefront per row.  After a panel the
-// trailing block takes the rank-2 NB update in one pass over it (k_syr2k_panel).  V and W are stored
+// trailing block takes the rank-2 NB update in one pass over it (k_panel_update, which also
+// finishes the panel's last W column and writes the reflectors back).  V and W are stored
 // column-major (column l contiguous over the rows), so that the column kernel's per-l loads
 // across a wavefront are coalesced.  The reflectors are
 // left where LAPACK leaves them for the lower variant (v(i+2:n) in A(i+2:n, i) of the
@@ -26,7 +27,6 @@
 namespace {
 
 constexpr int NB = 8;            // panel width (4, 6, 12 and 16 measured slower at n = 300..1560)
-constexpr int CT = 1024;         // column kernel: one workgroup
 constexpr int ST = 512;          // column step: threads per workgroup (four rows each at n = 2048)
 constexpr int LGRID = 128;       // workgroups of the column step (each repeats the column work, then a
                                  // 1/128 share of the trailing GEMV; 64 and 256 measured slower at n = 1560)
@@ -91,30 +91,6 @@ __global__ void k_mirror_lower(int n, long lda, double *__restrict__ A)
     }
 }
 
-// End of a panel p .. p + nbe - 1, one workgroup: W(:, nbe - 1) = tau y - (tau^2 / 2)(y . v) v
-// for the panel's last column (v = vin, y = yin over rows >= p + nbe), and every column's
-// reflector into its row of A (e on the subdiagonal, v below it).  The column kernels leave
-// row i of A alone: their other workgroups may still be reading it.
-__global__ __launch_bounds__(CT) void k_latrd_finish(int n, long lda, double *__restrict__ A, int p, int nbe,
-                                                     const double *__restrict__ V, double *__restrict__ W,
-                                                     const double *__restrict__ vin, const double *__restrict__ yin,
-                                                     const double *__restrict__ e, const double *__restrict__ tau)
-{
-    __shared__ double red[CT / 64];
-    const int t = threadIdx.x, i = p + nbe;
-    const double tp = tau[i - 1];
-    double s = 0.0;
-    for (int r = i + t; r < n; r += CT) s += yin[r] * vin[r];
-    s = block_sum<CT>(s, red);
-    const double c = -0.5 * tp * tp * s;
-    for (int r = i + t; r < n; r += CT) W[(size_t) (nbe - 1) * n + r] = tp * yin[r] + c * vin[r];
-    for (int j = 0; j < nbe; ++j) {
-        const int ic = p + j;
-        for (int r = ic + 1 + t; r < n; r += CT)
-            A[(size_t) ic * lda + r] = (r == ic + 1) ? e[ic] : V[(size_t) j * n + r];
-    }
-}
-
 // One column i = p + J of a panel, by every workgroup of the grid (specialised on the panel
 // position J, so that only the J - 1 stored columns of V and W are loaded).  Each workgroup computes
 // the column's steps 1-3 and the products W^T v, V^T v for all rows (the same operations in
@@ -122,7 +98,7 @@ __global__ __launch_bounds__(CT) void k_latrd_finish(int n, long lda, double *__
 // GEMV y = A0 v - V (W^T v) - W (V^T v).  Workgroup 0 also writes W(:, J - 1), V(:, J) and
 // d, e, tau.  vin / yin are the previous column's v and y, vout / yout this column's
 // (separate buffers: other workgroups still read the inputs).  Row i of A receives the
-// reflector only at the end of the panel (k_latrd_finish).  Thread t owns rows
+// reflector only at the end of the panel (k_panel_update).  Thread t owns rows
 // i + t + ST k (k < 4); V and W are read once per row: with v = 1 at row i + 1 and
 // v = x scale below, W^T v = scale (W^T x)_{>= i+2} + W(i + 1, :), accumulated in the same
 // pass that forms x.
@@ -313,42 +289,64 @@ __global__ __launch_bounds__(ST) void k_latrd_step(int n, long lda, const double
     }
 }
 
-// Trailing update after a panel: C (m x m, row-major, ldc) -= V W^T + W V^T over the panel's
-// nbe columns (V, W column-major with leading dimension ldv).  One 64 x 64 tile of C per
-// workgroup of 16 x 16 threads, a 4 x 4 block per thread; the tile's rows and columns of V and
-// W staged in LDS.
-__global__ __launch_bounds__(256) void k_syr2k_panel(int m, int nbe, const double *__restrict__ V,
-                                                     const double *__restrict__ W, long ldv, double *__restrict__ C,
-                                                     long ldc)
+// End of a panel p .. p + nbe - 1 and its trailing update, in one launch.  Every workgroup
+//   * forms W(:, nbe - 1) = tau y - (tau^2 / 2)(y . v) v of the panel's last column (v = vin,
+//     y = yin, rows >= q = p + nbe; the dot product repeated per workgroup) where its tile needs it,
+//   * updates its 64 x 64 tile of the trailing block C = A(q:, q:) -= V W^T + W V^T (V, W
+//     column-major with leading dimension n, 16 x 16 threads, a 4 x 4 block per thread, the
+//     tile's rows and columns of V and W staged in LDS),
+//   * writes its share of the panel's reflectors into their rows of A (e on the subdiagonal, v
+//     beyond; rows p .. q - 1, which the update does not touch and which the column kernels left
+//     alone while other workgroups still read them).
+__global__ __launch_bounds__(256) void k_panel_update(int n, long lda, double *__restrict__ A, int p, int nbe,
+                                                      const double *__restrict__ V, const double *__restrict__ W,
+                                                      const double *__restrict__ vin, const double *__restrict__ yin,
+                                                      const double *__restrict__ e, const double *__restrict__ tau)
 {
     __shared__ double vr[NB][UT], wr[NB][UT], vc[NB][UT], wc[NB][UT];
+    __shared__ double red[4];
     const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int q = p + nbe, m = n - q;
     const int r0 = blockIdx.y * UT, c0 = blockIdx.x * UT;
     const int rb = r0 + 4 * ty, cb = c0 + 4 * tx;
+    double *C = A + (size_t) q * lda + q;
     double acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-            acc[a][b] = (rb + a < m && cb + b < m) ? C[(size_t) (rb + a) * ldc + cb + b] : 0.0;
-    for (int q = t; q < nbe * UT; q += 256) {
-        const int l = q / UT, k = q % UT;
+            acc[a][b] = (rb + a < m && cb + b < m) ? C[(size_t) (rb + a) * lda + cb + b] : 0.0;
+    // W(:, nbe - 1) from the last column's y and v
+    const double tp = tau[q - 1];
+    double sdot = 0.0;
+    for (int r = q + t; r < n; r += 256) sdot += yin[r] * vin[r];
+    sdot = block_sum<256>(sdot, red);
+    const double cl = -0.5 * tp * tp * sdot;
+    const int lw = nbe - 1;
+    for (int idx = t; idx < nbe * UT; idx += 256) {
+        const int l = idx / UT, k = idx % UT;
+        const int rr = q + r0 + k, rc = q + c0 + k;
         const bool rin = r0 + k < m, cin = c0 + k < m;
-        vr[l][k] = rin ? V[(size_t) l * ldv + r0 + k] : 0.0;
-        wr[l][k] = rin ? W[(size_t) l * ldv + r0 + k] : 0.0;
-        vc[l][k] = cin ? V[(size_t) l * ldv + c0 + k] : 0.0;
-        wc[l][k] = cin ? W[(size_t) l * ldv + c0 + k] : 0.0;
+        vr[l][k] = rin ? V[(size_t) l * n + rr] : 0.0;
+        vc[l][k] = cin ? V[(size_t) l * n + rc] : 0.0;
+        if (l == lw) {
+            wr[l][k] = rin ? tp * yin[rr] + cl * vin[rr] : 0.0;
+            wc[l][k] = cin ? tp * yin[rc] + cl * vin[rc] : 0.0;
+        } else {
+            wr[l][k] = rin ? W[(size_t) l * n + rr] : 0.0;
+            wc[l][k] = cin ? W[(size_t) l * n + rc] : 0.0;
+        }
     }
     __syncthreads();
 #pragma unroll 4
     for (int l = 0; l < nbe; ++l) {
         double va[4], wa[4], vb[4], wb[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            va[q] = vr[l][4 * ty + q];
-            wa[q] = wr[l][4 * ty + q];
-            vb[q] = vc[l][4 * tx + q];
-            wb[q] = wc[l][4 * tx + q];
+        for (int u = 0; u < 4; ++u) {
+            va[u] = vr[l][4 * ty + u];
+            wa[u] = wr[l][4 * ty + u];
+            vb[u] = vc[l][4 * tx + u];
+            wb[u] = wc[l][4 * tx + u];
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -359,7 +357,14 @@ __global__ __launch_bounds__(256) void k_syr2k_panel(int m, int nbe, const doubl
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-            if (rb + a < m && cb + b < m) C[(size_t) (rb + a) * ldc + cb + b] = acc[a][b];
+            if (rb + a < m && cb + b < m) C[(size_t) (rb + a) * lda + cb + b] = acc[a][b];
+    // this workgroup's share of the reflector rows
+    const long G = (long) gridDim.x * gridDim.y, wg = (long) blockIdx.y * gridDim.x + blockIdx.x;
+    const long len = n - p, tot = (long) nbe * len;
+    for (long idx = wg * 256 + t; idx < tot; idx += G * 256) {
+        const int j = (int) (idx / len), r = p + (int) (idx % len), ic = p + j;
+        if (r > ic) A[(size_t) ic * lda + r] = (r == ic + 1) ? e[ic] : V[(size_t) j * n + r];
+    }
 }
 
 // the trailing 2 x 2 block after the last reflector
@@ -415,16 +420,12 @@ int yg_sytrd_blocked(hipStream_t s, int n, long lda, double *A, double *d, doubl
             YG_HIP(hipGetLastError());
             cur ^= 1;
         }
-        hipLaunchKernelGGL(k_latrd_finish, dim3(1), dim3(CT), 0, s, n, lda, A, p, nbe, (const double *) V, W,
-                           (const double *) vb[cur], (const double *) yb[cur], (const double *) e,
-                           (const double *) tau);
-        YG_HIP(hipGetLastError());
-        // trailing block rows / columns >= q: A -= V W^T + W V^T
+        // the panel's last W column, trailing update of rows / columns >= q and the reflector rows
         const int q = p + nbe, m = n - q;
-        double *C = A + (size_t) q * lda + q;
         const int nt = (m + UT - 1) / UT;
-        hipLaunchKernelGGL(k_syr2k_panel, dim3(nt, nt), dim3(256), 0, s, m, nbe, (const double *) V + q,
-                           (const double *) W + q, (long) n, C, lda);
+        hipLaunchKernelGGL(k_panel_update, dim3(nt, nt), dim3(256), 0, s, n, lda, A, p, nbe, (const double *) V,
+                           (const double *) W, (const double *) vb[cur], (const double *) yb[cur],
+                           (const double *) e, (const double *) tau);
         YG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_latrd_tail, dim3(1), dim3(64), 0, s, n, lda, A, d, e, tau);
