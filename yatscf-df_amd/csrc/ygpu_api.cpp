@@ -340,6 +340,13 @@ try {
     return exception_status();
 }
 
+// a new basis or pair layout: nothing left on the device belongs to the next call's system
+static void invalidate_handoffs(ygpu_ctx *c)
+{
+    c->fock_valid = c->density_valid = c->fcur_valid = 0;
+    for (int &ok : c->diis_err_ok) ok = 0;
+}
+
 int ygpu_set_basis(ygpu_ctx_t *c, int which, int nshells, const int *am, const int *nprim, const int *prim_off,
                    const double *xyz, const double *alpha, const double *coef)
 try {
@@ -383,6 +390,7 @@ try {
     YG_HIP(hipStreamSynchronize(c->stream));
     if (which == 0) c->nbf = nbf;
     else c->naux = nbf;
+    invalidate_handoffs(c);
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -546,6 +554,7 @@ try {
     YG_HIP(hipStreamSynchronize(c->stream));
     c->have_sparsity = 1;
     c->have_df = 0;
+    invalidate_handoffs(c);
     return YGPU_OK;
 } catch (...) {
     return exception_status();
