@@ -437,15 +437,17 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
 int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part)
 {
     if (nbf >= 256) {
-        // 128x128 tiles; about 2048 workgroups (four rounds of two per CU), chunks of at least
-        // 1024 along k
+        // 128x128 tiles at two workgroups per CU (512 resident on 256 CUs); at most 2048
+        // workgroups, i.e. four full rounds, with chunks of at least 1024 along k.  The split
+        // count is rounded down: rounding up lands just past a multiple of 512 at nbf = 760, 900
+        // and 1560 (2058, 2052, 2093 workgroups) and adds a nearly empty fifth round.
         if (kdim % 8) {
             ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
             return YGPU_ERR_INVALID;
         }
         const int ntb = (nbf + S_T - 1) / S_T;
         const int ntiles = ntb * (ntb + 1) / 2;
-        int nsplit = (int) std::max<long>(1, std::min<long>((2048 + ntiles - 1) / ntiles, (kdim + 1023) / 1024));
+        int nsplit = (int) std::max<long>(1, std::min<long>(2048 / ntiles, (kdim + 1023) / 1024));
         long kchunk = (kdim + nsplit - 1) / nsplit;
         kchunk = (kchunk + S_BK - 1) / S_BK * S_BK;
         nsplit = (int) ((kdim + kchunk - 1) / kchunk);
@@ -462,8 +464,8 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
     }
     const int ntb = (nbf + BM - 1) / BM;
     const int ntiles = ntb * (ntb + 1) / 2;
-    // enough workgroups to fill the chip, chunks of at least 512 along k
-    int nsplit = (int) std::max<long>(1, std::min<long>((2048 + ntiles - 1) / ntiles, (kdim + 511) / 512));
+    // at most 2048 workgroups (two rounds at four per CU), chunks of at least 512 along k
+    int nsplit = (int) std::max<long>(1, std::min<long>(2048 / ntiles, (kdim + 511) / 512));
     long kchunk = (kdim + nsplit - 1) / nsplit;
     kchunk = (kchunk + BK - 1) / BK * BK;
     nsplit = (int) ((kdim + kchunk - 1) / kchunk);
