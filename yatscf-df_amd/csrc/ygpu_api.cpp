@@ -200,7 +200,7 @@ int inv_sqrt_device(ygpu_ctx *c, int n, int ld, double *dA, double *dW, double *
     YG_HIP(hipMemcpy2DAsync(dW, (size_t) ld * sizeof(double), dA, (size_t) ld * sizeof(double), n * sizeof(double),
                             n, hipMemcpyDeviceToDevice, c->stream));
     YG_TRY(yg_scale_rows(c->stream, n, n, dW, ld, c->d_eig.as<double>(), 1));
-    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, dA, ld, dW, ld, 0.0, dOut, ld));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, dA, ld, dW, ld, 0.0, dOut, ld, &c->d_gemm_ws));
     // the two triangles round the scaled products differently: make the result exactly symmetric
     YG_TRY(yg_symmetrize(c->stream, n, dOut, ld));
     return YGPU_OK;
@@ -870,7 +870,7 @@ try {
                                      c->d_mat2.as<double>(), nbf));
         // C_occ = X V_occ
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
-                        c->d_Cocc.as<double>(), n_occ));
+                        c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
     } else if (nbf <= YG_SYTRD_BLOCKED_NMAX) {
         // blocked reduction, all eigenpairs of T, Q applied to the n_occ lowest only: their
         // columns of d_mat2 (column-major) are the rows of V_occ^T
@@ -880,7 +880,7 @@ try {
                                 c->d_evE.as<double>(), c->d_tau.as<double>(), c->d_tri.as<double>()));
         YG_TRY(solver_tridiag_to_full(c, nbf, nbf, c->d_mat0.as<double>(), c->d_mat2.as<double>(), n_occ));
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
-                        c->d_Cocc.as<double>(), n_occ));
+                        c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
     } else {
         rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf,
                                              c->d_mat0.as<double>(), nbf, c->d_eig.as<double>(),
@@ -891,11 +891,11 @@ try {
         }
         // C = X V: with V column-major in d_mat0, V(k,n) = d_mat0[n*nbf+k] = op_T access
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, dX, nbf, c->d_mat0.as<double>(), nbf, 0.0,
-                        c->d_mat2.as<double>(), nbf));
+                        c->d_mat2.as<double>(), nbf, &c->d_gemm_ws));
         YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
     }
     YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, c->d_Cocc.as<double>(), n_occ,
-                    0.0, c->d_D.as<double>(), nbf));
+                    0.0, c->d_D.as<double>(), nbf, &c->d_gemm_ws));
     int info = 0;
     YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipMemcpyAsync(eigval, c->d_eig.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -935,8 +935,10 @@ try {
 static int orth_transform(ygpu_ctx *c, const double *dF, double *dOut)
 {
     const int n = c->nbf;
-    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, dF, n, 0.0, c->d_T1.as<double>(), n));
-    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_T1.as<double>(), n, c->d_Xo.as<double>(), n, 0.0, dOut, n));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, dF, n, 0.0, c->d_T1.as<double>(), n,
+                    &c->d_gemm_ws));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_T1.as<double>(), n, c->d_Xo.as<double>(), n, 0.0, dOut, n,
+                    &c->d_gemm_ws));
     return YGPU_OK;
 }
 
@@ -1005,11 +1007,13 @@ try {
     double *R = c->d_diisR.as<double>() + (size_t) slot * nn;
     double *F0 = c->d_diisF.as<double>() + (size_t) slot * nn;
     // FDS = F D S (DIIS.c:51-54); R = X^T (FDS - FDS^T) X (DIIS.c:57-65)
-    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, dF, n, dD, n, 0.0, T1, n));
-    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T1, n, c->d_S.as<double>(), n, 0.0, T2, n));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, dF, n, dD, n, 0.0, T1, n, &c->d_gemm_ws));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T1, n, c->d_S.as<double>(), n, 0.0, T2, n, &c->d_gemm_ws));
     YG_TRY(yg_antisym(c->stream, n, T2, T1));
-    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, T1, n, 0.0, T2, n));
-    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T2, n, c->d_Xo.as<double>(), n, 0.0, R, n));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, 1.0, c->d_Xo.as<double>(), n, T1, n, 0.0, T2, n,
+                    &c->d_gemm_ws));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T2, n, c->d_Xo.as<double>(), n, 0.0, R, n,
+                    &c->d_gemm_ws));
     YG_TRY(yg_dots(c->stream, nn, R, c->d_diisR.as<double>(), nn, len, c->d_dots.as<double>(), nchunk));
     // max |R| of this slot (ygpu_diis_error) in the same round trip
     double *dmax = c->d_dots.as<double>() + (size_t) YGPU_MAX_DIIS * nchunk;
@@ -1091,7 +1095,7 @@ try {
     YG_HIP(hipMemcpyAsync(c->d_gB.p, B, bsz, hipMemcpyHostToDevice, c->stream));
     if (beta != 0.0) YG_HIP(hipMemcpyAsync(c->d_gC.p, C, csz, hipMemcpyHostToDevice, c->stream));
     YG_TRY(yg_dgemm(c->stream, ta, tb, m, n, k, alpha, c->d_gA.as<double>(), lda, c->d_gB.as<double>(), ldb, beta,
-                    c->d_gC.as<double>(), ldc));
+                    c->d_gC.as<double>(), ldc, &c->d_gemm_ws));
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     return YGPU_OK;

@@ -175,6 +175,36 @@ __global__ __launch_bounds__(NTHR) void k_gemm(Operand A, Operand B, int M, int 
     });
 }
 
+// Split-K form for GEMMs with few output tiles (the n x n products of the SCF at n of a few
+// hundred, 25 tiles at n = 260): workgroup (x, y, z) accumulates its tile over the k range
+// [z kchunk, (z + 1) kchunk) and stores it to the partial P[z] (m x n, leading dimension n);
+// k_gemm_reduce adds the partials in z order, so the result does not depend on scheduling.
+template <int MA, int MB>
+__global__ __launch_bounds__(NTHR) void k_gemm_part(Operand A, Operand B, int M, int N, int K, int kchunk,
+                                                    double *__restrict__ P)
+{
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    v4d acc[2][2];
+    tile_mma<MA, MB>(A, B, M, N, K, m0, n0, kb, ke, acc);
+    double *Pz = P + (size_t) blockIdx.z * M * N;
+    tile_epilogue(m0, n0, acc, [&](int m, int n, double v) {
+        if (m < M && n < N) Pz[(size_t) m * N + n] = v;
+    });
+}
+
+__global__ void k_gemm_reduce(const double *__restrict__ P, int nsplit, int M, int N, double alpha, double beta,
+                              double *__restrict__ C, long ldc)
+{
+    const long mn = (long) M * N;
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= mn) return;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += P[(size_t) z * mn + idx];
+    double *c = C + (size_t) (idx / N) * ldc + idx % N;
+    *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+}
+
 constexpr int XKCH64 = 256;      // gather indices staged in LDS per k chunk
 
 // X_i = Cocc[partners]^T * B[up rows]: workgroup t takes column tile t % ncolt, row tile
@@ -383,12 +413,43 @@ __global__ void k_syrk_reduce(const double *__restrict__ part, int nsplit, int n
 }  // namespace
 
 // ===========================================================================
+template <int MA, int MB>
+static void launch_gemm_part(hipStream_t s, dim3 grid, const Operand &a, const Operand &b, int m, int n, int k,
+                             int kchunk, double *P)
+{
+    hipLaunchKernelGGL((k_gemm_part<MA, MB>), grid, dim3(NTHR), 0, s, a, b, m, n, k, kchunk, P);
+}
+
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
-             const double *B, int ldb, double beta, double *C, int ldc)
+             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws)
 {
     if (m <= 0 || n <= 0) return YGPU_OK;
     dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
     Operand a{A, lda, nullptr}, b{B, ldb, nullptr};
+    // fewer than 64 output tiles: split k into chunks of at least two stages, up to about 256
+    // workgroups in all, when a workspace is given
+    const int tiles = (int) (grid.x * grid.y);
+    if (ws && tiles < 64 && k >= 4 * BK) {
+        int nsplit = std::min(k / (2 * BK), (256 + tiles - 1) / tiles);
+        int kchunk = (k + nsplit - 1) / nsplit;
+        kchunk = (kchunk + BK - 1) / BK * BK;
+        nsplit = (k + kchunk - 1) / kchunk;
+        if (nsplit > 1) {
+            const long mn = (long) m * n;
+            YG_TRY(ws->ensure((size_t) nsplit * mn * sizeof(double)));
+            double *P = ws->as<double>();
+            dim3 g3(grid.x, grid.y, nsplit);
+            if (!ta && !tb) launch_gemm_part<MODE_N, MODE_N>(s, g3, a, b, m, n, k, kchunk, P);
+            else if (!ta && tb) launch_gemm_part<MODE_N, MODE_T>(s, g3, a, b, m, n, k, kchunk, P);
+            else if (ta && !tb) launch_gemm_part<MODE_T, MODE_N>(s, g3, a, b, m, n, k, kchunk, P);
+            else launch_gemm_part<MODE_T, MODE_T>(s, g3, a, b, m, n, k, kchunk, P);
+            YG_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned) ((mn + 255) / 256)), dim3(256), 0, s, P, nsplit, m, n,
+                               alpha, beta, C, (long) ldc);
+            YG_HIP(hipGetLastError());
+            return YGPU_OK;
+        }
+    }
     if (!ta && !tb) hipLaunchKernelGGL((k_gemm<MODE_N, MODE_N>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
     else if (!ta && tb) hipLaunchKernelGGL((k_gemm<MODE_N, MODE_T>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);
     else if (ta && !tb) hipLaunchKernelGGL((k_gemm<MODE_T, MODE_N>), grid, dim3(NTHR), 0, s, a, b, m, n, k, alpha, beta, C, (long) ldc);

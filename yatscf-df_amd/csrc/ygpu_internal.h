@@ -101,6 +101,7 @@ struct ygpu_ctx {
     int have_S = 0, have_H = 0;
     DBuf d_Fin, d_Din, d_T1, d_T2, d_diisR, d_diisF, d_dots;   // DIIS work and history
     DBuf d_solver_ws;             // rocBLAS/rocSOLVER workspace owned by the library
+    DBuf d_gemm_ws;               // split-K partials of the small GEMMs (main stream only)
     size_t solver_ws_bytes = 0;
 
     // mixed-precision Fock builds (ygpu_set_fock_precision)
@@ -130,9 +131,11 @@ int yg_launch_eri2(ygpu_ctx *c, double *dJpq, int ld);
 // on stream s; returns once the kernels are queued (the host pair lists are copied before)
 int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s);
 
-// C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA
+// C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA; with a
+// workspace ws, products of fewer than 64 output tiles split k over workgroups (partials in ws,
+// added in a fixed order; C must not overlap A, B or ws)
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
-             const double *B, int ldb, double beta, double *C, int ldc);
+             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws = nullptr);
 // C [m][n] = A [m][k] B [n][k]^T with k-contiguous rows (k, lda, ldb multiples of 8), 128x128 tiles
 int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
                    double *C, long ldc);
