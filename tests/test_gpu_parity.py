@@ -73,7 +73,11 @@ def test_dgemm_all_transposes(ygpu):
     log("case: dgemm transposes")
     rng = np.random.RandomState(7)
     with make_engine(ygpu) as e:
-        for (m, n, k) in [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 129, 67)]:
+        # fewer than 64 output tiles with k >= 64 take the split-K path: (130, 70, 200), (257, 129, 67)
+        # (two chunks, the second short), (260, 260, 260) (the SCF's n x n products), (300, 41, 1000)
+        # (a C_occ product with deep k); (520, 520, 100) has 81 tiles and stays in one pass
+        for (m, n, k) in [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 200), (257, 129, 67), (260, 260, 260),
+                          (300, 41, 1000), (520, 520, 100)]:
             for ta in (0, 1):
                 for tb in (0, 1):
                     A = rng.randn(*((k, m) if ta else (m, k)))
@@ -82,6 +86,11 @@ def test_dgemm_all_transposes(ygpu):
                     ref = 1.5 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.5 * C0
                     got = e.dgemm(ta, tb, A, B, alpha=1.5, beta=-0.5, C=C0)
                     assert np.max(np.abs(got - ref)) < 1e-12 * max(1, k), (m, n, k, ta, tb)
+                    ref0 = 2.0 * ((A.T if ta else A) @ (B.T if tb else B))
+                    got0 = e.dgemm(ta, tb, A, B, alpha=2.0, beta=0.0)
+                    assert np.max(np.abs(got0 - ref0)) < 1e-12 * max(1, k), (m, n, k, ta, tb, "beta 0")
+                    # the split partials are added in a fixed order: a repeat is bit-identical
+                    assert np.array_equal(e.dgemm(ta, tb, A, B, alpha=2.0, beta=0.0), got0)
 
 
 @pytest.mark.parametrize("name", ["water", "c10h22"])
