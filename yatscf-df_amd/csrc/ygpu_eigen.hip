@@ -752,9 +752,9 @@ int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long 
     const dim3 grid((nvec + BLV - 1) / BLV), blk(64 * BLV);
     auto lds = [](int mb, int bk) { return (size_t) bk * 64 * mb * sizeof(double); };
     if (mbn <= 8)
-        hipLaunchKernelGGL((k_backtransform_cols<8, 8>), grid, blk, lds(8, 8), s, n, nvec, A, lda, tau, Z, ldz);
+        hipLaunchKernelGGL((k_backtransform_cols<8, 16>), grid, blk, lds(8, 16), s, n, nvec, A, lda, tau, Z, ldz);
     else if (mbn <= 16)
-        hipLaunchKernelGGL((k_backtransform_cols<16, 8>), grid, blk, lds(16, 8), s, n, nvec, A, lda, tau, Z, ldz);
+        hipLaunchKernelGGL((k_backtransform_cols<16, 16>), grid, blk, lds(16, 16), s, n, nvec, A, lda, tau, Z, ldz);
     else if (mbn <= 24)
         hipLaunchKernelGGL((k_backtransform_cols<24, 4>), grid, blk, lds(24, 4), s, n, nvec, A, lda, tau, Z, ldz);
     else
