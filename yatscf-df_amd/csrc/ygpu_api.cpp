@@ -992,7 +992,7 @@ try {
     const size_t ms = (size_t) nn * sizeof(double);
     YG_TRY(diis_buffers(c));
     const int nchunk = 64;
-    YG_TRY(c->d_dots.ensure(((size_t) YGPU_MAX_DIIS * nchunk + 1) * sizeof(double)));
+    YG_TRY(c->d_dots.ensure((size_t) (YGPU_MAX_DIIS + 1) * nchunk * sizeof(double)));
     // F / D NULL: the last Fock build's F and the density it was built from
     const double *dF = c->d_F.as<double>(), *dD = c->d_D.as<double>();
     if (F) {
@@ -1014,15 +1014,14 @@ try {
                     &c->d_gemm_ws));
     YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, T2, n, c->d_Xo.as<double>(), n, 0.0, R, n,
                     &c->d_gemm_ws));
-    YG_TRY(yg_dots(c->stream, nn, R, c->d_diisR.as<double>(), nn, len, c->d_dots.as<double>(), nchunk));
-    // max |R| of this slot (ygpu_diis_error) in the same round trip
+    // with the chunk maxima of |R| (this slot's max |R|, for ygpu_diis_error) from the same pass
     double *dmax = c->d_dots.as<double>() + (size_t) YGPU_MAX_DIIS * nchunk;
-    YG_TRY(yg_maxabs(c->stream, nn, R, dmax));
+    YG_TRY(yg_dots(c->stream, nn, R, c->d_diisR.as<double>(), nn, len, c->d_dots.as<double>(), nchunk, dmax));
     YG_TRY(orth_transform(c, dF, F0));
-    std::vector<double> part((size_t) len * nchunk + 1);
+    std::vector<double> part((size_t) (len + 1) * nchunk);
     YG_HIP(hipMemcpyAsync(part.data(), c->d_dots.p, (size_t) len * nchunk * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
-    YG_HIP(hipMemcpyAsync(part.data() + (size_t) len * nchunk, dmax, sizeof(double), hipMemcpyDeviceToHost,
+    YG_HIP(hipMemcpyAsync(part.data() + (size_t) len * nchunk, dmax, nchunk * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     if (F_orth) YG_HIP(hipMemcpyAsync(F_orth, F0, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
@@ -1031,7 +1030,9 @@ try {
         for (int k = 0; k < nchunk; ++k) sacc += part[(size_t) j * nchunk + k];
         dots[j] = sacc;
     }
-    c->diis_err[slot] = part[(size_t) len * nchunk];
+    double emax = 0.0;
+    for (int k = 0; k < nchunk; ++k) emax = std::max(emax, part[(size_t) len * nchunk + k]);
+    c->diis_err[slot] = emax;
     c->diis_err_ok[slot] = 1;
     return YGPU_OK;
 } catch (...) {

@@ -250,22 +250,37 @@ __global__ __launch_bounds__(1024) void k_maxabs(long n, const double *__restric
     }
 }
 
-// part[j][chunk] = sum over the chunk of x[k] * y_j[k]
+// part[j][chunk] = sum over the chunk of x[k] * y_j[k]; with pmax, also pmax[chunk] = max over
+// the chunk of |x[k]| (from the j = 0 workgroups, which read x anyway)
 __global__ __launch_bounds__(256) void k_dots(long len, const double *__restrict__ x, const double *__restrict__ ys,
-                                              long ystride, double *__restrict__ part)
+                                              long ystride, double *__restrict__ part, double *__restrict__ pmax)
 {
-    __shared__ double red[4];
+    __shared__ double red[4], redm[4];
     const int j = blockIdx.y, nchunk = gridDim.x;
     const long per = (len + nchunk - 1) / nchunk;
     const long b = (long) blockIdx.x * per, e = b + per < len ? b + per : len;
     const double *y = ys + (size_t) j * ystride;
-    double s = 0.0;
-    for (long k = b + threadIdx.x; k < e; k += blockDim.x) s += x[k] * y[k];
+    const bool wantmax = pmax != nullptr && j == 0;   // uniform over the workgroup
+    double s = 0.0, m = 0.0;
+    for (long k = b + threadIdx.x; k < e; k += blockDim.x) {
+        const double xv = x[k];
+        s += xv * y[k];
+        if (wantmax) m = fmax(m, fabs(xv));
+    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        m = fmax(m, __shfl_xor(m, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = s;
+        redm[threadIdx.x >> 6] = m;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) part[(size_t) j * nchunk + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) {
+        part[(size_t) j * nchunk + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (wantmax) pmax[blockIdx.x] = fmax(fmax(redm[0], redm[1]), fmax(redm[2], redm[3]));
+    }
 }
 
 struct Coefs {
@@ -321,10 +336,11 @@ int yg_maxabs(hipStream_t s, long n, const double *x, double *out)
     return YGPU_OK;
 }
 
-int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part, int nchunk)
+int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part, int nchunk,
+            double *pmax)
 {
     if (ny <= 0) return YGPU_OK;
-    hipLaunchKernelGGL(k_dots, dim3(nchunk, ny), dim3(256), 0, s, len, x, ys, ystride, part);
+    hipLaunchKernelGGL(k_dots, dim3(nchunk, ny), dim3(256), 0, s, len, x, ys, ystride, part, pmax);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -183,6 +183,7 @@ int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const doub
 int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);
 int yg_transpose(hipStream_t s, int rows, int cols, const double *A, double *AT);
 int yg_antisym(hipStream_t s, int n, const double *A, double *out);                  // out = A - A^T
+// partial <x, y_j> per chunk into part[j][chunk]; with pmax, the chunk maxima of |x| into pmax[chunk]
 int yg_dots(hipStream_t s, long len, const double *x, const double *ys, long ystride, int ny, double *part,
-            int nchunk);                                                               // partial <x, y_j>
+            int nchunk, double *pmax = nullptr);
 int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, const double *coef, double *out);
