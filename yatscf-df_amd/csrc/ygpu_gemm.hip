@@ -318,6 +318,10 @@ __device__ __forceinline__ void tile128_nt(const double *__restrict__ A, long ld
             for (int q = 0; q < 8; ++q) rb[q] = 0.0;
         }
     };
+    // 16-row / 16-column blocks of this wavefront's 64 x 64 part that reach inside the matrix
+    const int ilim = min(4, max(0, (arows - m0 - wm + 15) / 16));
+    const int jlim = min(4, max(0, (brows - n0 - wn + 15) / 16));
+    const bool wfull = ilim == 4 && jlim == 4;
     if (kb < ke) load(kb);
     for (long k0 = kb; k0 < ke; k0 += S_BK) {
         __syncthreads();
@@ -328,20 +332,41 @@ __device__ __forceinline__ void tile128_nt(const double *__restrict__ A, long ld
         }
         __syncthreads();
         if (k0 + S_BK < ke) load(k0 + S_BK);
+        if (wfull) {
 #pragma unroll
-        for (int kk = 0; kk < S_BK; kk += 4) {
-            const int kr = kk + (lane >> 4), c = lane & 15;
-            double a[4], b[4];
+            for (int kk = 0; kk < S_BK; kk += 4) {
+                const int kr = kk + (lane >> 4), c = lane & 15;
+                double a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = As[kr][wm + 16 * i + c];
-                b[i] = Bs[kr][wn + 16 * i + c];
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = As[kr][wm + 16 * i + c];
+                    b[i] = Bs[kr][wn + 16 * i + c];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
             }
+        } else if (ilim > 0 && jlim > 0) {
+            // a wavefront on the ragged edge: only its 16 x 16 blocks that hold a row < arows and
+            // a column < brows (the others stay zero and are never stored)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int kk = 0; kk < S_BK; kk += 4) {
+                const int kr = kk + (lane >> 4), c = lane & 15;
+                double a[4], b[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = As[kr][wm + 16 * i + c];
+                    b[i] = Bs[kr][wn + 16 * i + c];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (i < ilim && j < jlim)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
         }
     }
 }
@@ -508,7 +533,7 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
         }
         const int ntb = (nbf + S_T - 1) / S_T;
         const int ntiles = ntb * (ntb + 1) / 2;
-        int nsplit = (int) std::max<long>(1, std::min<long>(2048 / ntiles, (kdim + 1023) / 1024));
+        int nsplit = (int) std::max<long>(1, std::min<long>(8192 / ntiles, (kdim + 1023) / 1024));
         long kchunk = (kdim + nsplit - 1) / nsplit;
         kchunk = (kchunk + S_BK - 1) / S_BK * S_BK;
         nsplit = (int) ((kdim + kchunk - 1) / kchunk);
