@@ -59,9 +59,12 @@ __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 // NBUF = 2: two LDS stages; stage s + 1 is split into one buffer while the MFMAs of
 // stage s read the other, with one barrier per stage, so the split of some wavefronts
 // overlaps the matrix-core work of others.
-template <int WM, int WN, int BK, int NBUF, class LA, class LB>
+// mrows / ncols: rows and columns of the tile inside the matrix; with SKIP a wavefront whose
+// 64 x 64 part lies wholly outside issues no MFMAs (its accumulators stay zero and are never
+// stored).  The SYRK keeps SKIP off: the test costs it the registers of its second workgroup per CU.
+template <int WM, int WN, int BK, int NBUF, bool SKIP, class LA, class LB>
 __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2],
-                                         bool zero = true)
+                                         int mrows, int ncols, bool zero = true)
 {
     constexpr int KV = Stage<BK>::KV, KVP = Stage<BK>::KVP;
     constexpr int NT = 64 * WM * WN;
@@ -134,7 +137,10 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
                 }
             }
     };
+    // a wavefront with no row or no column inside the matrix skips its matrix-core work
+    const bool wskip = SKIP && (mrows <= wm || ncols <= wn);
     auto mfma_stage = [&](int buf) {
+        if (wskip) return;
 #pragma unroll
         for (int s = 0; s < BK / 16; ++s) {
             const int kv = 2 * s + (lane >> 5), r = lane & 31;
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
         __syncthreads();
         XLoadA<64 * WM> la{Cocc32, s_pa, kc, n_occ, m0};
         XLoadB<64 * WN> lb{B32, s_up, kc, naux_pad, n0};
-        bf3_tile<WM, WN, XBK, XNBUF>(la, lb, kc, kce, acc, kc == 0);
+        bf3_tile<WM, WN, XBK, XNBUF, true>(la, lb, kc, kce, acc, n_occ - m0, naux_pad - n0, kc == 0);
         if (kce >= klen) break;
     }
     // X stored pre-split for the SYRK: hi = bf16(v), lo = bf16(v - hi), exactly what the
@@ -325,10 +331,10 @@ __global__ __launch_bounds__(NTHR) void k_syrk_bf3(const __bf16 *__restrict__ Xh
     const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
     SLoad<SBK> la{Xh + kb, Xl + kb, kdim, nbf, bi * ST};
     SLoad<SBK> lb{Xh + kb, Xl + kb, kdim, nbf, bj * ST};
-    v16f acc[2][2];
-    bf3_tile<2, 2, SBK, 1>(la, lb, 0, (int) (ke - kb), acc);
-    float *P = part + (size_t) z * nbf * nbf;
     const int m0 = bi * ST, n0 = bj * ST;
+    v16f acc[2][2];
+    bf3_tile<2, 2, SBK, 1, false>(la, lb, 0, (int) (ke - kb), acc, nbf - m0, nbf - n0);
+    float *P = part + (size_t) z * nbf * nbf;
     bf3_epilogue<2>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
         if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = v;
