@@ -283,6 +283,12 @@ __global__ __launch_bounds__(256) void k_dots(long len, const double *__restrict
     }
 }
 
+// x[0 .. n) = v (grid-stride)
+__global__ void k_fill(long n, double *__restrict__ x, double v)
+{
+    for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x) x[i] = v;
+}
+
 struct Coefs {
     double c[YGPU_MAX_DIIS];
 };
@@ -358,47 +364,70 @@ int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, 
     return YGPU_OK;
 }
 
-int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,
-              float *t_Jmat_ms)
+long yg_jbuild_groups(long n_up, long naux_pad, bool fp32)
+{
+    const int C = fp32 ? cpl<float>() : cpl<double>();
+    const int ncolt = (int) ((naux_pad + 64 * C - 1) / (64 * C));
+    // row groups: about 2048 workgroups in total, at least 16 rows per wavefront
+    long ngroup = std::max<long>(1, std::min<long>(2048 / ncolt, n_up / (16L * JWAVES)));
+    return std::min<long>(ngroup, 65535);
+}
+
+int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, double *part,
+                 double *tempJ, long ngroup)
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
     const int C = B32 ? cpl<float>() : cpl<double>();
     const int ncolt = (int) ((ld + 64 * C - 1) / (64 * C));
-    // row groups: about 2048 workgroups in total, at least 16 rows per wavefront
-    long ngroup = std::max<long>(1, std::min<long>(2048 / ncolt, n_up / (16L * JWAVES)));
-    ngroup = std::min<long>(ngroup, 65535);
-    YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
-    YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
     YG_HIP(hipEventRecord(c->ev[0], s));
     if (B32)
         hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s, B32, ld, n_up,
-                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, c->d_tempJpart.as<double>());
+                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, part);
     else
         hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s,
-                           c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf,
-                           c->d_tempJpart.as<double>());
+                           c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, part);
     YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, c->d_tempJpart.as<double>(),
-                       (int) ngroup, ld, c->d_tempJ.as<double>());
+    hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, part, (int) ngroup, ld, tempJ);
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[1], s));
-    YG_HIP(hipMemsetAsync(dJ, 0, (size_t) nbf * nbf * sizeof(double), s));
+    // J entries of screened-out pairs stay zero (a fill kernel, not a memset node)
+    YG_TRY(yg_fill(s, (long) nbf * nbf, dJ, 0.0));
     const int rpw = 4;
     const long nblk = (n_up + 4L * rpw - 1) / (4L * rpw);
     if (B32)
         hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(256), ld * sizeof(double), s, B32, ld, n_up,
-                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
+                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, rpw, dJ);
     else
         hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(256), ld * sizeof(double), s, c->d_B.as<double>(),
-                           ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), c->d_tempJ.as<double>(), nbf, rpw, dJ);
+                           ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, rpw, dJ);
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[2], s));
+    return YGPU_OK;
+}
+
+int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,
+              float *t_Jmat_ms)
+{
+    const long ld = c->naux_pad;
+    const long ngroup = yg_jbuild_groups(c->n_up, ld, B32 != nullptr);
+    YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
+    YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
+    YG_TRY(yg_jbuild_ws(c, s, dD, dJ, B32, c->d_tempJpart.as<double>(), c->d_tempJ.as<double>(), ngroup));
     if (t_tempJ_ms || t_Jmat_ms) {
         YG_HIP(hipEventSynchronize(c->ev[2]));
         if (t_tempJ_ms) YG_HIP(hipEventElapsedTime(t_tempJ_ms, c->ev[0], c->ev[1]));
         if (t_Jmat_ms) YG_HIP(hipEventElapsedTime(t_Jmat_ms, c->ev[1], c->ev[2]));
     }
+    return YGPU_OK;
+}
+
+int yg_fill(hipStream_t s, long n, double *x, double v)
+{
+    if (n <= 0) return YGPU_OK;
+    const long nb = std::min<long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, s, n, x, v);
+    YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
 

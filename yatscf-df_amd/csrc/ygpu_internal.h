@@ -144,10 +144,21 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
 // upper-triangle K = X X^T with split-K partials in Kpart, result in K (full, mirrored)
 int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K, DBuf &part);
+// the split of yg_syrk_upper: returns the number of k chunks (kchunk_out: their length); the
+// partials take nsplit * nbf * nbf doubles
+int yg_syrk_upper_split(int nbf, long kdim, long *kchunk_out);
+// yg_syrk_upper with the partials in a caller-provided buffer
+int yg_syrk_upper_ws(hipStream_t s, int nbf, long kdim, const double *X, double *K, double *part, long kchunk);
 // J from density dD (upper triangle of dJ; the rest zero) on stream s, timed by ev[0..2];
 // B32 != nullptr reads the FP32 copy
 int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,
               float *t_Jmat_ms);
+// row groups of the J build's first pass (its partials take ngroup * naux_pad doubles)
+long yg_jbuild_groups(long n_up, long naux_pad, bool fp32);
+// yg_jbuild with the pass-1 partials and temp_J in caller-provided buffers (nothing allocated)
+int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, double *part,
+                 double *tempJ, long ngroup);
+int yg_fill(hipStream_t s, long n, double *x, double v);                          // x[0..n) = v
 int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
 int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                      // A = (A + A^T) / 2
 // tridiagonal reduction of a row-major n x n symmetric matrix (upper triangle read) on one

@@ -447,6 +447,57 @@ class TinySCF:
         return self.d(name, (n, n))
 
 
+def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, fock_check=False,
+          final_fock=False):
+    """One complete SCF of ``xyz`` through the TinySCF engine, timed from after input parsing
+    (basis shells and nuclei uploaded) to the passing convergence test: S/H/X, Schwarz
+    screening, pair layout, initial guess (SAD, plus the learned pair correction when
+    ``weights`` names a network file), DF tensor, SCF iterations.  ``mixed`` starts the SCF in
+    the reduced-precision Fock mode.  After the timer stops (untimed): with ``final_fock`` the
+    FP64 Fock matrix of the converged density (AO basis) as "F" (the engine's F_mat holds the
+    DIIS output in the orthogonal basis); with ``fock_check`` also one mixed Fock build on the
+    same density, the reduced-precision arithmetic's own error in F and in the orbital energies
+    of X^T F X.  Returns a dict; the engine is closed."""
+    import time
+    T = TinySCF(xyz, niters=niters)
+    try:
+        T.L.tinyscf_set_guess_weights(T.T, (weights or "").encode())
+        T.setd("ene_tol", ene_tol)
+        T.seti("quirk_a2", int(quirk_a2))
+        T.seti("mixed", int(mixed))
+        t0 = time.perf_counter()
+        T.setup()
+        t1 = time.perf_counter()
+        T.build_df_tensor()
+        t2 = time.perf_counter()
+        T.do_scf()
+        t3 = time.perf_counter()
+        nbf = T.geti("nbasfuncs")
+        res = dict(seconds=t3 - t0, setup_s=t1 - t0, df_s=t2 - t1, scf_s=t3 - t2, iters=T.geti("iter"),
+                   energy=T.getd("total_energy"), converged=bool(T.geti("converged")),
+                   hist=T.d("t_hist").reshape(-1, 6).copy(), t_df=T.d("t_df").copy(), nbf=nbf,
+                   naux=T.geti("df_nbf"), n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"),
+                   switch=T.geti("mixed_switch_iter"), guess_ml=T.geti("guess_ml"),
+                   D=T.mat("D_mat").copy(), S=T.mat("S_mat").copy(),
+                   X=T.mat("X_mat").copy(), H=T.mat("Hcore_mat").copy(), eigval=T.d("eigval").copy())
+        mask = T.i("bf_pair_mask").reshape(nbf, nbf)
+        res["n_up"] = int(np.sum(mask[np.triu_indices(nbf)] >= 0))
+        if final_fock or fock_check:
+            res["F"] = T.build_fock(FOCK_FP64)
+        if fock_check:
+            X, F64 = res["X"], res["F"]
+            Fmx = T.build_fock(FOCK_MIXED)
+            e64 = np.linalg.eigvalsh(X.T @ F64 @ X)
+            emx = np.linalg.eigvalsh(X.T @ Fmx @ X)
+            no = res["n_occ"]
+            res["fock_check"] = {"max_abs_dF": float(np.max(np.abs(Fmx - F64))),
+                                 "max_abs_d_orbital_energy": float(np.max(np.abs(emx - e64))),
+                                 "max_abs_d_occupied_orbital_energy": float(np.max(np.abs(emx[:no] - e64[:no])))}
+        return res
+    finally:
+        T.close()
+
+
 # ---------------------------------------------------------------------------
 # learned-guess pair model (host/guess_ml.c) and its network (C-ABI)
 
