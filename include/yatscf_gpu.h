@@ -145,6 +145,24 @@ int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *C
 #define YGPU_FOCK_MIXED 1
 int ygpu_set_fock_precision(ygpu_ctx_t *ctx, int mode);
 
+/* ---- Fock build without device allocation.  No counterpart in the reference
+ * (whose Fock build, build_Fock.c:349-396, allocates nothing per call because
+ * TinySCF_init_free.c preallocates host buffers): it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.
+ * As ygpu_build_fock in FP64 arithmetic (YGPU_FOCK_FP64 must be set), with D,
+ * Cocc and Hcore required host inputs and J, K (nullable) and F host outputs;
+ * the device work buffers (copies of the matrices, X = C_occ^T B, the split-K
+ * and J-pass partials) are carved from one static device array of
+ * YGPU_NOALLOC_SCRATCH_BYTES per GPU, taken at the library's load and held until
+ * the process ends.  It calls no hipMalloc / hipFree.  A call whose buffers
+ * need more than that array (about nbf * n_occ * naux * 8 bytes for X; C10H22
+ * in cc-pVDZ / cc-pVDZ-RI fits) returns YGPU_ERR_INVALID.  Every engine on a
+ * GPU shares that GPU's array: calls that overlap in time on one GPU race. */
+#define YGPU_NOALLOC_SCRATCH_BYTES ((size_t) 128 << 20)
+int ygpu_build_fock_noalloc(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *Cocc,
+                            const double *Hcore, double *J, double *K, double *F);
+
 /* Density from the orthogonal-basis Fock matrix F_orth = X^T F X
  * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
  * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.

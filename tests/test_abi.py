@@ -50,6 +50,8 @@ def test_errors_without_handle():
     assert L.ygpu_build_fock(None, 1, None, None, None, None, None, None, None) == 1
     assert b"null context" in L.ygpu_last_error()
     assert L.ygpu_set_fock_precision(None, 1) == 1
+    assert L.ygpu_build_fock_noalloc(None, 1, None, None, None, None, None, None) == 1
+    assert b"null context" in L.ygpu_last_error()
     assert L.ygpu_diis_error(None, 0, None) == 1
     assert L.ygpu_destroy(None) == 0
     # page-locked host memory: a null result pointer is rejected, freeing NULL is a no-op

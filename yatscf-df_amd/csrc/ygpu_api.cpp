@@ -257,9 +257,10 @@ try {
         ygpu_set_error(YGPU_ERR_INVALID, "device %d out of range (%d visible)", device, n);
         return YGPU_ERR_INVALID;
     }
+    YG_HIP(hipSetDevice(device));
+    YG_TRY(yg_scratch_init(device));
     ygpu_ctx *c = new ygpu_ctx();
     c->device = device;
-    YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreate(&c->stream));
     YG_HIP(hipStreamCreate(&c->side));
     YG_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));

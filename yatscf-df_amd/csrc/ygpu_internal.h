@@ -159,6 +159,9 @@ long yg_jbuild_groups(long n_up, long naux_pad, bool fp32);
 int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, double *part,
                  double *tempJ, long ngroup);
 int yg_fill(hipStream_t s, long n, double *x, double v);                          // x[0..n) = v
+// takes the device address of the no-allocation entry point's static scratch array on `device`
+// (which must be current); once per device, later calls return at once
+int yg_scratch_init(int device);
 int yg_delta(hipStream_t s, long n, const double *D, double *Dprev, double *dD);   // dD = D - Dprev; Dprev = D
 int yg_symmetrize(hipStream_t s, int n, double *A, long lda);                      // A = (A + A^T) / 2
 // tridiagonal reduction of a row-major n x n symmetric matrix (upper triangle read) on one

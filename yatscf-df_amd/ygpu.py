@@ -59,6 +59,7 @@ SIGNATURES = {
     "ygpu_build_df_tensor_ex": [vp, dp, dp, dp, ctypes.POINTER(DfTiming)],
     "ygpu_set_df_tensor": [vp, dp],
     "ygpu_build_fock": [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, ctypes.POINTER(FockTiming)],
+    "ygpu_build_fock_noalloc": [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp],
     "ygpu_build_density": [vp, ctypes.c_int, dp, dp, dp, dp, dp, ip],
     "ygpu_dgemm": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                    dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int],
@@ -289,6 +290,18 @@ class Engine:
         t = FockTiming()
         check(self.L.ygpu_build_fock(self.h, n_occ, _d(D), _d(Cocc), _d(H), _d(J), _d(K), _d(F), ctypes.byref(t)))
         return dict(F=F, J=J, K=K, timing=t)
+
+    def build_fock_noalloc(self, D, Cocc, H, want_JK=True):
+        """ygpu_build_fock_noalloc: the FP64 Fock build with its device buffers in the library's
+        static scratch array (no device allocation); host D, C_occ and Hcore are required."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        Cocc = np.ascontiguousarray(Cocc, dtype=np.float64)
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        F = np.zeros((self.nbf, self.nbf))
+        J = np.zeros((self.nbf, self.nbf)) if want_JK else None
+        K = np.zeros((self.nbf, self.nbf)) if want_JK else None
+        check(self.L.ygpu_build_fock_noalloc(self.h, Cocc.shape[1], _d(D), _d(Cocc), _d(H), _d(J), _d(K), _d(F)))
+        return dict(F=F, J=J, K=K)
 
     def set_fock_precision(self, mode):
         """FOCK_FP64 (reference arithmetic) or FOCK_MIXED (BF16-split K, incremental FP32-tensor J)."""
