@@ -129,8 +129,13 @@ def kernel_table(runs, mixed):
                      "achieved": rate, "peak": peak, "unit": unit, "frac": rate / peak}
 
     df_ms = [r["t_df"][3] * 1e3 for r in runs]
-    add("k_gemm128_nt (B = pqA J^-1/2, DF tensor)", "mfma", df_ms, 1, 2.0 * n_up * naux * naux,
-        FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
+    if os.environ.get("YATSCF_METRIC", "cholesky") == "invsqrt":
+        add("k_gemm128_nt (B = pqA J^-1/2, DF tensor)", "mfma", df_ms, 1, 2.0 * n_up * naux * naux,
+            FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
+    else:
+        # B = pqA L^-T with L^-1 lower triangular: column q sums q + 1 products
+        add("k_gemm128_nt (B = pqA L^-T, triangular, DF tensor)", "mfma", df_ms, 1,
+            float(n_up) * naux * (naux + 1), FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
     for want_mixed in ((True, False) if mixed else (False,)):
         rows = [phase_rows(r, want_mixed) for r in runs]
         n = int(round(np.mean([len(x) for x in rows]))) + (2 if want_mixed or not mixed else 0)

@@ -102,10 +102,26 @@ int ygpu_set_sparsity(ygpu_ctx_t *ctx, const int *bf_pair_mask, int num_uniq_sp,
                       const int *uniq_sp_lid, const int *uniq_sp_rid, const double *sp_scrval,
                       const double *df_sp_scrval, double shell_scrtol2);
 
-/* The DF tensor B = (mn|P) J^{-1/2} on the device (TinySCF_build_DF_tensor,
- * build_DF_tensor.c:256-287).  Jpq_invsqrt (nullable, naux x naux) and
- * df_tensor (nullable, the reference's host layout nnz x naux with both
- * triangles, TinySCF.c:268-273) receive host copies. */
+/* Factor W of the Coulomb metric (W^T W = J_PQ^{-1}) that forms the DF tensor
+ * B = (mn|P) W^T.  The reference takes W = J^{-1/2} from an eigendecomposition
+ * of J_PQ (calc_inverse_sqrt_Jpq, build_DF_tensor.c:205-237):
+ * YGPU_METRIC_INVSQRT.  YGPU_METRIC_CHOLESKY (the default) takes W = L^{-1}
+ * with J_PQ = L L^T: the fitted integrals B B^T = (mn|P) J^{-1} (Q|kl), hence
+ * J, K and F, are the same up to rounding, the factor costs a Cholesky
+ * instead of an eigensolve and, W being lower triangular, the contraction
+ * half the flops; B's columns and the Jpq output below differ (L^{-1} instead
+ * of J^{-1/2}).  A metric that is not positive definite is YGPU_ERR_NUMERIC
+ * in both modes. */
+#define YGPU_METRIC_CHOLESKY 0
+#define YGPU_METRIC_INVSQRT 1
+int ygpu_set_metric_factor(ygpu_ctx_t *ctx, int mode);
+
+/* The DF tensor B = (mn|P) W^T on the device (TinySCF_build_DF_tensor,
+ * build_DF_tensor.c:256-287), W as set by ygpu_set_metric_factor.
+ * Jpq_invsqrt (nullable, naux x naux) receives W (J^{-1/2} as the reference's
+ * Jpq ends, :236, in the YGPU_METRIC_INVSQRT mode; L^{-1} in the Cholesky
+ * mode) and df_tensor (nullable, the reference's host layout nnz x naux with
+ * both triangles, TinySCF.c:268-273) a host copy of B. */
 int ygpu_build_df_tensor(ygpu_ctx_t *ctx, double *Jpq_invsqrt, double *df_tensor,
                          ygpu_df_timing_t *timing);
 

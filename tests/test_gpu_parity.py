@@ -154,35 +154,87 @@ def oracle_spread(st):
 
 
 @pytest.mark.parametrize("name", ["water", "c10h22"])
-def test_df_tensor(ygpu, name):
-    log("case: DF tensor", name)
+@pytest.mark.parametrize("mode", ["cholesky", "invsqrt"])
+def test_df_tensor(ygpu, name, mode):
+    """3-centre integrals, the metric factor W and the fitted operator B B^T against the oracle.
+    invsqrt: W = J^-1/2, the reference's factor (symmetric, W J W = I); cholesky: W = L^-1 with
+    J = L L^T (lower triangular, W J W^T = I).  B B^T = (mn|P) J^-1 (Q|kl) either way."""
+    log("case: DF tensor", name, mode)
     st = oracle_state(name)
     with make_engine(ygpu, st) as e:
         set_oracle_sparsity(e, st)
+        e.set_metric_factor(ygpu.METRIC_CHOLESKY if mode == "cholesky" else ygpu.METRIC_INVSQRT)
         r = e.build_df_tensor(want_pqA=True, want_Jpq=True, want_B=True)
     pqA_ref = st.o.d("pqA").reshape(r["pqA"].shape)
     err3 = np.max(np.abs(r["pqA"] - pqA_ref))
     log("  3c max abs err %.3e (max |value| %.3e)" % (err3, np.abs(pqA_ref).max()))
     assert err3 < 1e-11
-    # J^{-1/2} by its defining property, as good as the oracle's own factor
+    # the metric factor by its defining property, as good as the oracle's own J^-1/2
     J = metric(st)
     W, Wref = r["Jpq"], st.o.d("Jpq").reshape(r["Jpq"].shape)
-    res = np.max(np.abs(W @ J @ W - np.eye(len(J))))
     res_ref = np.max(np.abs(Wref @ J @ Wref - np.eye(len(J))))
-    log("  |W J W - I|: gpu %.3e oracle %.3e; rel |dW| %.3e" % (res, res_ref,
-        np.max(np.abs(W - Wref)) / np.abs(Wref).max()))
     ev = np.linalg.eigvalsh(J)
     kappa_eps = ev[-1] / ev[0] * np.finfo(float).eps
     log("  metric condition number %.3e" % (ev[-1] / ev[0]))
+    if mode == "invsqrt":
+        res = np.max(np.abs(W @ J @ W - np.eye(len(J))))
+        log("  |W J W - I|: gpu %.3e oracle %.3e; rel |dW| %.3e" % (res, res_ref,
+            np.max(np.abs(W - Wref)) / np.abs(Wref).max()))
+        assert np.max(np.abs(W - W.T)) < 1e-9 * np.abs(W).max()
+    else:
+        res = np.max(np.abs(W @ J @ W.T - np.eye(len(J))))
+        log("  |W J W^T - I|: gpu %.3e (oracle's J^-1/2: %.3e)" % (res, res_ref))
+        assert np.all(np.triu(W, 1) == 0.0)
+        # the factor is unique: W = L^-1 of numpy's Cholesky factor
+        Wnp = np.linalg.inv(np.linalg.cholesky(J))
+        log("  rel |W - L^-1 (numpy)| %.3e" % (np.max(np.abs(W - Wnp)) / np.abs(Wnp).max()))
+        assert np.max(np.abs(W - Wnp)) < 10 * kappa_eps * np.abs(Wnp).max()
     assert res < 10 * res_ref + 10 * kappa_eps
-    assert np.max(np.abs(W - W.T)) < 1e-9 * np.abs(W).max()
     # the fitted Coulomb operator B B^T on random vectors
     B_ref = st.o.d("df_tensor").reshape(r["B"].shape)
     x = np.random.RandomState(3).randn(B_ref.shape[0], 4)
     g, gref = r["B"] @ (r["B"].T @ x), B_ref @ (B_ref.T @ x)
     errG = np.max(np.abs(g - gref)) / np.abs(gref).max()
-    log("  fitted operator rel err %.3e, max|dB| %.3e" % (errG, np.max(np.abs(r["B"] - B_ref))))
+    log("  fitted operator rel err %.3e" % errG)
     assert errG < 1e-10
+
+
+def test_metric_factors_agree_beyond_2048(ygpu):
+    """C30H62 (naux 2910 > 2048, padded row length 2944): the reference's J^-1/2 (rocSOLVER
+    dsyevd on the padded matrix) and the Cholesky factor describe the same J^-1 (W_is W_is =
+    W_ch^T W_ch) and the same fitted operator B B^T."""
+    log("case: metric factors at naux 2910")
+    T = ygpu.TinySCF(xyz_path("c30h62"), niters=1)
+    try:
+        T.setup()
+        mask, lid, rid = T.i("bf_pair_mask").copy(), T.i("uniq_sp_lid").copy(), T.i("uniq_sp_rid").copy()
+        sp, df = T.d("sp_scrval").copy(), T.d("df_sp_scrval").copy()
+    finally:
+        T.close()
+    s = system("c30h62")
+    out = {}
+    for mode in (ygpu.METRIC_INVSQRT, ygpu.METRIC_CHOLESKY):
+        e = ygpu.Engine(0)
+        try:
+            e.set_system(s)
+            e.set_sparsity(mask, lid, rid, sp, df)
+            e.set_metric_factor(mode)
+            out[mode] = e.build_df_tensor(want_Jpq=True, want_B=True)
+        finally:
+            e.close()
+    Wi, Wc = out[ygpu.METRIC_INVSQRT]["Jpq"], out[ygpu.METRIC_CHOLESKY]["Jpq"]
+    Ji_a, Ji_b = Wi @ Wi, Wc.T @ Wc
+    rel = np.max(np.abs(Ji_a - Ji_b)) / np.abs(Ji_b).max()
+    Bi, Bc = out[ygpu.METRIC_INVSQRT]["B"], out[ygpu.METRIC_CHOLESKY]["B"]
+    x = np.random.RandomState(5).randn(Bi.shape[0], 3)
+    gi, gc = Bi @ (Bi.T @ x), Bc @ (Bc.T @ x)
+    relG = np.max(np.abs(gi - gc)) / np.abs(gc).max()
+    log("  rel |W_is^2 - W_ch^T W_ch| %.3e; fitted operator rel diff %.3e; W_is symmetric %.1e"
+        % (rel, relG, np.max(np.abs(Wi - Wi.T)) / np.abs(Wi).max()))
+    assert np.max(np.abs(Wi - Wi.T)) < 1e-9 * np.abs(Wi).max()
+    assert np.all(np.triu(Wc, 1) == 0.0)
+    assert rel < 1e-5
+    assert relG < 1e-10
 
 
 @pytest.mark.parametrize("name", ["water", "c10h22"])

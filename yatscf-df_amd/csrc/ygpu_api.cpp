@@ -624,17 +624,41 @@ try {
     YG_HIP(hipMemsetAsync(c->d_mat0.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
     YG_TRY(yg_launch_eri2(c, c->d_mat0.as<double>(), (int) ld));
     YG_HIP(hipEventRecord(c->ev[2], c->stream));
-    YG_HIP(hipMemsetAsync(c->d_Jpq.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
-    double mine = 0.0;
-    YG_TRY(inv_sqrt_device(c, c->naux, (int) ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
-                           c->d_Jpq.as<double>(), &mine));
-    YG_HIP(hipEventRecord(c->ev[3], c->stream));
-    // B = pqA J^{-1/2}; J^{-1/2} is exactly symmetric (inv_sqrt_device symmetrises it), so its rows
-    // serve as the k-contiguous second operand
-    YG_TRY(yg_dgemm_nt128(c->stream, (int) c->n_up, (int) ld, ld, c->d_pqA.as<double>(), ld,
-                          c->d_Jpq.as<double>(), ld, c->d_B.as<double>(), ld));
+    const bool chol = c->metric_mode == YGPU_METRIC_CHOLESKY;
+    if (chol) {
+        // J = L L^T on the device, W = L^{-1} (lower triangular, identity on the padding), then
+        // B = pqA W^T with the zeros of W above its diagonal skipped
+        YG_TRY(c->d_info.ensure(sizeof(int)));
+        YG_TRY(c->d_mat2.ensure((size_t) ld * 64 * sizeof(double)));
+        YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
+        YG_TRY(yg_potrf_lower(c->stream, c->naux, (int) ld, ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
+                              c->d_info.as<int>(), c->d_mat2.as<double>()));
+        YG_TRY(yg_trtri_lower(c->stream, (int) ld, ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
+                              c->d_Jpq.as<double>(), ld, c->d_mat2.as<double>()));
+        YG_HIP(hipEventRecord(c->ev[3], c->stream));
+        YG_TRY(yg_dgemm_nt128(c->stream, (int) c->n_up, (int) ld, ld, c->d_pqA.as<double>(), ld,
+                              c->d_Jpq.as<double>(), ld, c->d_B.as<double>(), ld, true));
+    } else {
+        YG_HIP(hipMemsetAsync(c->d_Jpq.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
+        double mine = 0.0;
+        YG_TRY(inv_sqrt_device(c, c->naux, (int) ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
+                               c->d_Jpq.as<double>(), &mine));
+        YG_HIP(hipEventRecord(c->ev[3], c->stream));
+        // B = pqA J^{-1/2}; J^{-1/2} is exactly symmetric (inv_sqrt_device symmetrises it), so its
+        // rows serve as the k-contiguous second operand
+        YG_TRY(yg_dgemm_nt128(c->stream, (int) c->n_up, (int) ld, ld, c->d_pqA.as<double>(), ld,
+                              c->d_Jpq.as<double>(), ld, c->d_B.as<double>(), ld));
+    }
     YG_HIP(hipEventRecord(c->ev[4], c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    if (chol) {
+        int info = 0;
+        YG_HIP(hipMemcpy(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (info != 0) {
+            ygpu_set_error(YGPU_ERR_NUMERIC, "metric not positive definite: Cholesky pivot %d is not positive", info);
+            return YGPU_ERR_NUMERIC;
+        }
+    }
     if (t) {
         t->t_3c = elapsed(c->ev[0], c->ev[1]) * 1e-3;
         t->t_2c = elapsed(c->ev[1], c->ev[2]) * 1e-3;
@@ -820,6 +844,19 @@ try {
         c->have_B32 = 0;
         c->d_Kpart32.release();
     }
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_metric_factor(ygpu_ctx_t *c, int mode)
+try {
+    YG_TRY(check_ctx(c));
+    if (mode != YGPU_METRIC_CHOLESKY && mode != YGPU_METRIC_INVSQRT) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_metric_factor: unknown mode %d", mode);
+        return YGPU_ERR_INVALID;
+    }
+    c->metric_mode = mode;
     return YGPU_OK;
 } catch (...) {
     return exception_status();

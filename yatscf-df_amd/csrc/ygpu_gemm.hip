@@ -406,16 +406,19 @@ __global__ __launch_bounds__(NTHR) void k_syrk128(const double *__restrict__ X, 
 }
 
 // C [M][N] (ldc) = A [M][K] B[N][K]^T, 128x128 tiles, column tiles of one row panel
-// consecutive on one XCD
+// consecutive on one XCD.  lower_b: B is lower triangular (B[n][k] = 0 for k > n), so the
+// column tile [n0, n0 + 128) needs only k < n0 + 128
 __global__ __launch_bounds__(NTHR) void k_gemm128_nt(const double *__restrict__ A, long lda,
                                                      const double *__restrict__ B, long ldb, int M, int N, long K,
-                                                     int ncolt, long total, double *__restrict__ C, long ldc)
+                                                     int ncolt, long total, int lower_b, double *__restrict__ C,
+                                                     long ldc)
 {
     const long w_id = xcd_remap64(blockIdx.x, total);
     const int ct = (int) (w_id % ncolt), rt = (int) (w_id / ncolt);
     const int m0 = rt * S_T, n0 = ct * S_T;
+    const long ke = lower_b ? std::min<long>(K, n0 + S_T) : K;
     v4d acc[4][4];
-    tile128_nt(A, lda, M, m0, B, ldb, N, n0, 0, K, acc);
+    tile128_nt(A, lda, M, m0, B, ldb, N, n0, 0, ke, acc);
     tile128_epilogue(acc, [&](int r, int c, double v) {
         const int m = m0 + r, n = n0 + c;
         if (m < M && n < N) C[(size_t) m * ldc + n] = v;
@@ -484,7 +487,7 @@ int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, c
 }
 
 int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
-                   double *C, long ldc)
+                   double *C, long ldc, bool lower_b)
 {
     if (m <= 0 || n <= 0) return YGPU_OK;
     if (k % 8 || lda % 8 || ldb % 8) {
@@ -497,8 +500,8 @@ int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long ld
         ygpu_set_error(YGPU_ERR_INVALID, "yg_dgemm_nt128: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_gemm128_nt, dim3((unsigned) total), dim3(NTHR), 0, s, A, lda, B, ldb, m, n, k, ncolt, total, C,
-                       ldc);
+    hipLaunchKernelGGL(k_gemm128_nt, dim3((unsigned) total), dim3(NTHR), 0, s, A, lda, B, ldb, m, n, k, ncolt, total,
+                       (int) lower_b, C, ldc);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

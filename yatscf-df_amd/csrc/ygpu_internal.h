@@ -91,6 +91,7 @@ struct ygpu_ctx {
 
     // DF tensor
     int have_df = 0;
+    int metric_mode = YGPU_METRIC_CHOLESKY;   // factor of J_PQ used for B (ygpu_set_metric_factor)
     DBuf d_pqA, d_B, d_Jpq;
 
     // Fock / density work buffers
@@ -136,9 +137,19 @@ int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s);
 // added in a fixed order; C must not overlap A, B or ws)
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
              const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws = nullptr);
-// C [m][n] = A [m][k] B [n][k]^T with k-contiguous rows (k, lda, ldb multiples of 8), 128x128 tiles
+// C [m][n] = A [m][k] B [n][k]^T with k-contiguous rows (k, lda, ldb multiples of 8), 128x128 tiles;
+// lower_b: B lower triangular (its zeros above the diagonal are skipped, half the flops at n = k)
 int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
-                   double *C, long ldc);
+                   double *C, long ldc, bool lower_b = false);
+// blocked Cholesky A = L L^T (lower, row-major, order n padded to npad, a multiple of 64, with
+// the padding rows / columns zero on entry): L in the lower triangle of A, the inverses of the
+// 64 x 64 diagonal blocks in Linv (npad / 64 blocks of 64 x 64); a non-positive pivot sets
+// *info (device) to its column + 1; T holds npad x 64 doubles of workspace
+int yg_potrf_lower(hipStream_t s, int n, int npad, long lda, double *A, double *Linv, int *info, double *T);
+// W = L^{-1} (npad x npad, lower triangular, zeros above the diagonal) from yg_potrf_lower's L and
+// Linv; T holds 64 x npad doubles of workspace
+int yg_trtri_lower(hipStream_t s, int npad, long lda, const double *L, const double *Linv, double *W, long ldw,
+                   double *T);
 // X[i] = Cocc(partners of row i)^T * Bup(rows of i) for every basis function i
 int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);

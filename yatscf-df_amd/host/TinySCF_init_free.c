@@ -207,6 +207,13 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     if (ygpu_set_basis(g, 1, T->df_nshells, am, np, po, xyz, al, co) != YGPU_OK) goto gpu_fail;
     CMS_getAtoms(T->basis, &chg, &axyz);
     if (ygpu_set_atoms(g, T->natoms, chg, axyz) != YGPU_OK) goto gpu_fail;
+    /* factor of the Coulomb metric: the reference's J^{-1/2} (YATSCF_METRIC=invsqrt) or the
+     * Cholesky factor's inverse (the default: the same J and K, a third of the DF build time) */
+    {
+        const char *m = getenv("YATSCF_METRIC");
+        const int mode = (m && !strcmp(m, "invsqrt")) ? YGPU_METRIC_INVSQRT : YGPU_METRIC_CHOLESKY;
+        if (ygpu_set_metric_factor(g, mode) != YGPU_OK) goto gpu_fail;
+    }
 
     T->init_time = get_wtime_sec() - st;
     return 0;

@@ -70,6 +70,7 @@ SIGNATURES = {
     "ygpu_diis_extrapolate": [vp, ctypes.c_int, dp, dp],
     "ygpu_diis_error": [vp, ctypes.c_int, dp],
     "ygpu_set_fock_precision": [vp, ctypes.c_int],
+    "ygpu_set_metric_factor": [vp, ctypes.c_int],
     "ygpu_mlp_forward": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "ygpu_mlp_trainer_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -82,6 +83,7 @@ SIGNATURES = {
 }
 
 FOCK_FP64, FOCK_MIXED = 0, 1
+METRIC_CHOLESKY, METRIC_INVSQRT = 0, 1
 
 fp = ctypes.POINTER(ctypes.c_float)
 
@@ -270,6 +272,10 @@ class Engine:
         t = DfTiming()
         check(self.L.ygpu_build_df_tensor_ex(self.h, _d(pqA), _d(Jpq), _d(B), ctypes.byref(t)))
         return dict(pqA=pqA, Jpq=Jpq, B=B, timing=t)
+
+    def set_metric_factor(self, mode):
+        """METRIC_CHOLESKY (W = L^-1, the default) or METRIC_INVSQRT (W = J^-1/2, the reference's)."""
+        check(self.L.ygpu_set_metric_factor(self.h, int(mode)))
 
     def set_df_tensor(self, B):
         B = np.ascontiguousarray(B, dtype=np.float64)
