@@ -498,3 +498,21 @@ def test_k_build_tile_variants(ygpu, n_occ):
     log("  FP64 max|dK|/max|K| %.3e; mixed max|dK|/S %.3e" % (e64, emx))
     assert e64 < 1e-12
     assert emx < 2.0 ** -14
+
+
+@pytest.mark.parametrize("name", ["water", "c10h22"])
+def test_initial_guess_matches_oracle(ygpu, name):
+    """The host engine's SAD density (SAD blocks scaled by 0.5 (Z - Q) / Z, TinySCF.c:324-364)
+    and its iteration-0 C_occ (D2Cocc with quirk A1, TinySCF.c:282-322) equal the oracle's."""
+    log("case: initial guess", name)
+    st = oracle_state(name)
+    n, nocc = st.sys.nbf, st.sys.n_occ
+    T = ygpu.TinySCF(xyz_path(name), niters=1)
+    try:
+        T.setup()
+        D, C = T.mat("D_mat").copy(), T.d("Cocc_mat", (n, nocc)).copy()
+    finally:
+        T.close()
+    log("  max|dD| %.2e max|dC_occ| %.2e" % (np.abs(D - st.D0).max(), np.abs(C - st.C0).max()))
+    assert np.array_equal(D, st.D0)
+    assert np.max(np.abs(C - st.C0)) < 1e-15 * max(1.0, np.abs(st.C0).max())

@@ -160,3 +160,29 @@ void tinyscf_set_guess_weights(TinySCF_t T, const char *path)
 {
     snprintf(T->guess_weights, sizeof(T->guess_weights), "%s", path ? path : "");
 }
+
+/* TinySCF_D2Cocc on a caller-given density (nbf x nbf, row-major) without an engine: the
+ * host-only step between the SAD guess and the first Fock build, exposed so that it can be
+ * compared with the oracle's dpstrf-based factor on the CPU. */
+int tinyscf_d2cocc(int nbf, int n_occ, int quirk_a1, const double *D, double *Cocc)
+{
+    struct TinySCF_struct T;
+    memset(&T, 0, sizeof(T));
+    T.nbasfuncs = nbf;
+    T.n_occ = n_occ;
+    T.mat_size = nbf * nbf;
+    T.quirk_a1 = quirk_a1;
+    T.D_mat = (double *) malloc(sizeof(double) * (size_t) nbf * nbf);
+    T.tmp_mat = (double *) malloc(sizeof(double) * (size_t) nbf * nbf);
+    T.Cocc_mat = Cocc;
+    if (!T.D_mat || !T.tmp_mat) {
+        free(T.D_mat);
+        free(T.tmp_mat);
+        return -1;
+    }
+    memcpy(T.D_mat, D, sizeof(double) * (size_t) nbf * nbf);
+    TinySCF_D2Cocc(&T);
+    free(T.D_mat);
+    free(T.tmp_mat);
+    return 0;
+}
