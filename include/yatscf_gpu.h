@@ -44,6 +44,19 @@ typedef struct {
 /* Message of the calling thread's last error ("" if none). */
 const char *ygpu_last_error(void);
 
+/* Failure inside a drop-in TinySCF function.  The reference's hot-path
+ * functions return void and abort on failure (e.g. the asserts of
+ * build_DF_tensor.c:67, :214); the drop-in build_DF_tensor.c, build_Fock.c and
+ * build_density.c call ygpu_fail(status, their name) when a call of this
+ * library fails.  ygpu_fail passes (status, where, ygpu_last_error()) to the
+ * calling thread's handler if one is installed (the handler may leave by
+ * longjmp: the host engine's Python bindings do, to turn the failure into an
+ * error status), else, or if the handler returns, prints the message and
+ * aborts.  ygpu_set_failure_handler(NULL) restores the abort. */
+typedef void (*ygpu_failure_handler_t)(int status, const char *where, const char *message);
+int ygpu_set_failure_handler(ygpu_failure_handler_t handler);
+void ygpu_fail(int status, const char *where);
+
 /* Number of visible HIP devices. */
 int ygpu_device_count(int *count);
 
@@ -95,9 +108,12 @@ int ygpu_schwarz(ygpu_ctx_t *ctx, int quirk_a5, double *sp_scrval, double *bf_pa
                  double *df_sp_scrval);
 
 /* The screened pair layout (TinySCF_prepare_sparsity, TinySCF.c:228-280)
- * and the unique shell-pair list (TinySCF.c:183-221).  bf_pair_mask must be
- * symmetric in which entries are >= 0.  The device stores only rows with
- * i <= j ("up" rows), in the reference's CSR order restricted to j >= i. */
+ * and the unique shell-pair list (TinySCF.c:183-221).  A basis-function pair
+ * is kept when either orientation has bf_pair_mask >= 0: the reference's
+ * Schwarz index (TinySCF.c:153) can keep (i, j) and drop (j, i), a case its
+ * integral copy leaves undefined (a write to row -1, build_DF_tensor.c:36-43).
+ * Diagonal pairs must be kept.  The device stores only rows with i <= j ("up"
+ * rows), in the reference's CSR order restricted to j >= i. */
 int ygpu_set_sparsity(ygpu_ctx_t *ctx, const int *bf_pair_mask, int num_uniq_sp,
                       const int *uniq_sp_lid, const int *uniq_sp_rid, const double *sp_scrval,
                       const double *df_sp_scrval, double shell_scrtol2);
@@ -137,10 +153,12 @@ int ygpu_set_df_tensor(ygpu_ctx_t *ctx, const double *df_tensor);
 
 /* J, K and F = Hcore + 2J - K from D and C_occ (TinySCF_build_FockMat,
  * build_Fock.c:349-396).  D, Hcore: nbf x nbf (Hcore NULL: the copy cached by
- * ygpu_ovlp_hcore); Cocc: nbf x n_occ.  D and Cocc both NULL: the D and C_occ
- * of the last ygpu_build_density on this context, already on the device (no
- * host copy; an SCF loop's hand-off).  J, K (nullable) and F receive full
- * symmetric matrices; J entries of screened pairs are zero. */
+ * ygpu_ovlp_hcore or by an earlier call); Cocc: nbf x n_occ.  D and Cocc both
+ * NULL, or the SCF hand-off mode on (ygpu_set_scf_resident) after a
+ * ygpu_build_density: that build's D and C_occ, already on the device (no
+ * host copy).  J, K (nullable) and F receive full symmetric matrices; J
+ * entries of screened pairs are zero.  The call's phase timers (timing,
+ * nullable) are also kept by the engine (ygpu_get_fock_timings). */
 int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *Cocc,
                     const double *Hcore, double *J, double *K, double *F,
                     ygpu_fock_timing_t *timing);
@@ -183,8 +201,10 @@ int ygpu_build_fock_noalloc(ygpu_ctx_t *ctx, int n_occ, const double *D, const d
  * (TinySCF_build_DenMat, build_density.c:37-69): eigval ascending, C = X C0,
  * Cocc = the n_occ lowest columns (nbf x n_occ row-major), D = Cocc Cocc^T.
  * X NULL: the copy cached by ygpu_set_overlap.  F_orth NULL: the output of the
- * last ygpu_diis_transform / ygpu_diis_extrapolate, already on the device.  ev_idx (nullable) receives
- * the order used (identity for the ascending eigenvalues of the eigensolver). */
+ * last ygpu_diis_transform / ygpu_diis_extrapolate, already on the device.  In
+ * the SCF hand-off mode (ygpu_set_scf_resident) host F_orth and X are not
+ * copied up while those device copies exist.  ev_idx (nullable) receives the
+ * order used (identity for the ascending eigenvalues of the eigensolver). */
 int ygpu_build_density(ygpu_ctx_t *ctx, int n_occ, const double *F_orth, const double *X,
                        double *D, double *Cocc, double *eigval, int *ev_idx);
 
@@ -198,12 +218,13 @@ int ygpu_set_overlap(ygpu_ctx_t *ctx, const double *S, const double *X);
  * keeps the reference's bookkeeping (DIIS_len, the replaced slot, B_mat,
  * DIIS_bmax) and solves the small system.
  *   ygpu_diis_transform: F_orth = X^T F X (iterations 0 and 1, DIIS.c:30-40);
- *     F NULL: the last ygpu_build_fock's F on the device.
+ *     F NULL (or the hand-off mode on): the last ygpu_build_fock's F on the
+ *     device.
  *   ygpu_diis_update: residual R = X^T (F D S - S D F) X stored in history
  *     slot `slot`, dots[j] = <R, R_j> for j < len, F_orth = X^T F X stored as
- *     the slot's Fock matrix and returned (DIIS.c:51-110); F and D NULL: the
- *     last ygpu_build_fock's F and the D it was built from; F_orth NULL: not
- *     returned.  max |R| is formed in the same pass (ygpu_diis_error).
+ *     the slot's Fock matrix and returned (DIIS.c:51-110); F or D NULL (or the
+ *     hand-off mode on): the last ygpu_build_fock's F and the D it was built
+ *     from; F_orth NULL: not returned.  max |R| is formed in the same pass (ygpu_diis_error).
  *   ygpu_diis_extrapolate: F_out = sum_{i<len} coef[i] F_orth_i (DIIS.c:119-122).
  * len <= YGPU_MAX_DIIS. */
 #define YGPU_MAX_DIIS 10
@@ -219,6 +240,23 @@ int ygpu_diis_error(ygpu_ctx_t *ctx, int slot, double *max_abs);
  * (the cblas_dgemm calls of DIIS.c:34-107). */
 int ygpu_dgemm(ygpu_ctx_t *ctx, int transa, int transb, int m, int n, int k, double alpha,
                const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc);
+
+/* SCF hand-off mode (no counterpart in the reference, whose matrices all live
+ * on the host).  With on != 0, host inputs whose current value the device
+ * already holds are not copied up again: D and C_occ after a
+ * ygpu_build_density, F and its D after a ygpu_build_fock, F_orth after a DIIS
+ * transform or extrapolation, and the cached Hcore and X.  The caller promises
+ * not to change those host matrices between the calls of one SCF loop, as the
+ * reference's TinySCF_do_SCF does not (TinySCF.c:378-435); the host outputs
+ * are written as before.  Off (the default) copies every host input. */
+int ygpu_set_scf_resident(ygpu_ctx_t *ctx, int on);
+
+/* Phase timers kept by the engine: of every ygpu_build_fock since the last
+ * ygpu_clear_timings, oldest first (count receives how many, at most max are
+ * copied; the engine keeps at most 4096), and of the last DF tensor build. */
+int ygpu_get_fock_timings(ygpu_ctx_t *ctx, int max, ygpu_fock_timing_t *out, int *count);
+int ygpu_get_df_timing(ygpu_ctx_t *ctx, ygpu_df_timing_t *out);
+int ygpu_clear_timings(ygpu_ctx_t *ctx);
 
 /* Device-resident pair layout sizes: rows of the packed tensor (i <= j),
  * rows of the reference layout (both triangles) and the padded aux width. */

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -229,6 +230,23 @@ int exception_status()
 extern "C" {
 
 const char *ygpu_last_error(void) { return g_err.c_str(); }
+
+static thread_local ygpu_failure_handler_t g_fail_handler = nullptr;
+
+int ygpu_set_failure_handler(ygpu_failure_handler_t handler)
+{
+    g_fail_handler = handler;
+    return YGPU_OK;
+}
+
+void ygpu_fail(int status, const char *where)
+{
+    const char *w = where ? where : "ygpu";
+    if (g_fail_handler) g_fail_handler(status, w, g_err.c_str());   // may not return
+    std::fprintf(stderr, "%s: %s\n", w, g_err.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
 
 int ygpu_device_count(int *count)
 try {
@@ -501,14 +519,13 @@ try {
     c->h_upmask.assign((size_t) nbf * nbf, -1);
     std::vector<int> up_i, up_j;
     long nnz = 0;
+    // a pair is kept when either orientation survived the screening: the reference's Schwarz
+    // index (quirk A5, TinySCF.c:153) can keep (i, j) and drop (j, i), and it then writes the
+    // (j, i) row of pqA at index -1 (build_DF_tensor.c:36-43); the union is the defined reading
+    auto kept = [&](int i, int j) { return mask[(size_t) i * nbf + j] >= 0 || mask[(size_t) j * nbf + i] >= 0; };
     for (int i = 0; i < nbf; ++i)
         for (int j = 0; j < nbf; ++j) {
-            const bool a = mask[(size_t) i * nbf + j] >= 0, b = mask[(size_t) j * nbf + i] >= 0;
-            if (a != b) {
-                ygpu_set_error(YGPU_ERR_INVALID, "pair mask not symmetric at (%d,%d)", i, j);
-                return YGPU_ERR_INVALID;
-            }
-            if (!a) continue;
+            if (!kept(i, j)) continue;
             ++nnz;
             if (j >= i) {
                 c->h_upmask[(size_t) i * nbf + j] = (int) up_i.size();
@@ -530,7 +547,7 @@ try {
     for (int i = 0; i < nbf; ++i) {
         xptr[i] = (int) xup.size();
         for (int j = 0; j < nbf; ++j) {
-            if (mask[(size_t) i * nbf + j] < 0) continue;
+            if (!kept(i, j)) continue;
             const int lo = std::min(i, j), hi = std::max(i, j);
             xup.push_back(c->h_upmask[(size_t) lo * nbf + hi]);
             xpartner.push_back(j);
@@ -659,12 +676,11 @@ try {
             return YGPU_ERR_NUMERIC;
         }
     }
-    if (t) {
-        t->t_3c = elapsed(c->ev[0], c->ev[1]) * 1e-3;
-        t->t_2c = elapsed(c->ev[1], c->ev[2]) * 1e-3;
-        t->t_invsqrt = elapsed(c->ev[2], c->ev[3]) * 1e-3;
-        t->t_dfgemm = elapsed(c->ev[3], c->ev[4]) * 1e-3;
-    }
+    c->df_last.t_3c = elapsed(c->ev[0], c->ev[1]) * 1e-3;
+    c->df_last.t_2c = elapsed(c->ev[1], c->ev[2]) * 1e-3;
+    c->df_last.t_invsqrt = elapsed(c->ev[2], c->ev[3]) * 1e-3;
+    c->df_last.t_dfgemm = elapsed(c->ev[3], c->ev[4]) * 1e-3;
+    if (t) *t = c->df_last;
     if (Jpq_out) {
         // one contiguous copy: the pitched device-to-host copy runs row by row
         double *packed = c->d_mat1.as<double>();
@@ -706,7 +722,9 @@ try {
         for (int j = i; j < nbf; ++j) {
             const int u = c->h_upmask[(size_t) i * nbf + j];
             if (u < 0) continue;
-            const int r = c->h_mask[(size_t) i * nbf + j];
+            // the row of either orientation (one of them may have been screened out)
+            int r = c->h_mask[(size_t) i * nbf + j];
+            if (r < 0) r = c->h_mask[(size_t) j * nbf + i];
             std::memcpy(up.data() + (size_t) u * ld, df + (size_t) r * naux, naux * sizeof(double));
         }
     YG_TRY(upload(c->d_B, up.data(), up.size() * sizeof(double), c->stream));
@@ -723,8 +741,12 @@ int ygpu_build_fock(ygpu_ctx_t *c, int n_occ, const double *D, const double *Coc
                     double *K, double *F, ygpu_fock_timing_t *t)
 try {
     YG_TRY(check_ctx(c));
-    const bool resident = !D && !Cocc;         // the last density build's D and C_occ, on the device
-    if (!c->have_df || n_occ <= 0 || (!resident && (!D || !Cocc)) || !F || (!H && !c->have_H) ||
+    // D and C_occ of the last density build, already on the device: both NULL, or in the SCF
+    // hand-off mode whenever that build's outputs are there
+    const bool resident = (!D && !Cocc) || (c->resident && c->density_valid);
+    // Hcore: NULL, or in the hand-off mode, the device copy once one exists
+    const bool h_cached = !H || (c->resident && c->have_H);
+    if (!c->have_df || n_occ <= 0 || (!resident && (!D || !Cocc)) || !F || (h_cached && !c->have_H) ||
         (resident && !c->density_valid)) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_fock: DF tensor not built, no device density or bad arguments");
         return YGPU_ERR_INVALID;
@@ -739,7 +761,7 @@ try {
     YG_TRY(c->d_Cocc.ensure((size_t) nbf * n_occ * sizeof(double)));
     YG_TRY(c->d_X.ensure((size_t) nbf * n_occ * c->naux_pad * sizeof(double)));
     if (!resident) YG_HIP(hipMemcpyAsync(c->d_D.p, D, ms, hipMemcpyHostToDevice, c->stream));
-    if (H) {
+    if (!h_cached) {
         YG_HIP(hipMemcpyAsync(c->d_H.p, H, ms, hipMemcpyHostToDevice, c->stream));
         c->have_H = 1;
     }
@@ -814,16 +836,17 @@ try {
     if (K) YG_HIP(hipMemcpyAsync(K, c->d_K.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
     c->fock_valid = 1;
-    if (t) {
-        tj1 = elapsed(c->ev[0], c->ev[1]);
-        tj2 = elapsed(c->ev[1], c->ev[2]);
-        t->t_tempJ = tj1 * 1e-3;
-        t->t_Jmat = tj2 * 1e-3;
-        t->t_tempK = elapsed(c->ev[8], c->ev[4]) * 1e-3;   // J (ev 5 .. 3) runs beside the K build
-        t->t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
-        t->t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
-        t->t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
-    }
+    ygpu_fock_timing_t tm;
+    tj1 = elapsed(c->ev[0], c->ev[1]);
+    tj2 = elapsed(c->ev[1], c->ev[2]);
+    tm.t_tempJ = tj1 * 1e-3;
+    tm.t_Jmat = tj2 * 1e-3;
+    tm.t_tempK = elapsed(c->ev[8], c->ev[4]) * 1e-3;   // J (ev 5 .. 3) runs beside the K build
+    tm.t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
+    tm.t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
+    tm.t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
+    if (c->fock_hist.size() < YG_MAX_FOCK_HIST) c->fock_hist.push_back(tm);
+    if (t) *t = tm;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -867,7 +890,11 @@ int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double 
 try {
     YG_TRY(check_ctx(c));
     const int nbf = c->nbf;
-    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || (!Fo && !c->fcur_valid) || (!X && !c->have_S) || !D || !Cocc ||
+    // F_orth of the last DIIS step and the cached X: NULL, or in the SCF hand-off mode whenever
+    // the device holds them
+    const bool fo_dev = !Fo || (c->resident && c->fcur_valid);
+    const bool x_dev = !X || (c->resident && c->have_S);
+    if (nbf <= 0 || n_occ <= 0 || n_occ > nbf || (fo_dev && !c->fcur_valid) || (x_dev && !c->have_S) || !D || !Cocc ||
         !eigval) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_build_density: bad arguments / no device F_orth");
         return YGPU_ERR_INVALID;
@@ -882,12 +909,12 @@ try {
     YG_TRY(c->d_evE.ensure(nbf * sizeof(double)));
     YG_TRY(c->d_info.ensure(sizeof(int)));
     // Fo NULL: the last DIIS transform or extrapolation, already on the device
-    if (Fo) YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
+    if (!fo_dev) YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
     else YG_HIP(hipMemcpyAsync(c->d_mat0.p, c->d_Fcur.p, ms, hipMemcpyDeviceToDevice, c->stream));
     c->fock_valid = 0;                          // d_D and d_Cocc are overwritten below
     c->density_valid = 0;
     const double *dX = c->d_Xo.as<double>();
-    if (X) {
+    if (!x_dev) {
         YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
         dX = c->d_mat1.as<double>();
     }
@@ -996,14 +1023,16 @@ static int diis_buffers(ygpu_ctx *c)
 int ygpu_diis_transform(ygpu_ctx_t *c, const double *F, double *F_orth)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_S || (!F && !c->fock_valid) || !F_orth) {
+    // F NULL, or in the SCF hand-off mode whenever it is there: the last Fock build's F
+    const bool f_dev = !F || (c->resident && c->fock_valid);
+    if (!c->have_S || (f_dev && !c->fock_valid) || !F_orth) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_transform: call ygpu_set_overlap first / no device Fock / null argument");
         return YGPU_ERR_INVALID;
     }
     const size_t ms = (size_t) c->nbf * c->nbf * sizeof(double);
     YG_TRY(diis_buffers(c));
     const double *dF = c->d_F.as<double>();     // F NULL: the last Fock build's
-    if (F) {
+    if (!f_dev) {
         YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
         dF = c->d_Fin.as<double>();
     }
@@ -1020,7 +1049,10 @@ int ygpu_diis_update(ygpu_ctx_t *c, int slot, int len, const double *F, const do
                      double *F_orth)
 try {
     YG_TRY(check_ctx(c));
-    if (!c->have_S || ((!F || !D) && !c->fock_valid) || !dots || slot < 0 || slot >= YGPU_MAX_DIIS ||
+    // F / D NULL, or in the SCF hand-off mode whenever they are there: the last Fock build's F
+    // and the density it was built from
+    const bool fd_dev = !F || !D || (c->resident && c->fock_valid);
+    if (!c->have_S || (fd_dev && !c->fock_valid) || !dots || slot < 0 || slot >= YGPU_MAX_DIIS ||
         len <= slot || len > YGPU_MAX_DIIS) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_diis_update: bad arguments (slot %d, len %d)", slot, len);
         return YGPU_ERR_INVALID;
@@ -1031,13 +1063,10 @@ try {
     YG_TRY(diis_buffers(c));
     const int nchunk = 64;
     YG_TRY(c->d_dots.ensure((size_t) (YGPU_MAX_DIIS + 1) * nchunk * sizeof(double)));
-    // F / D NULL: the last Fock build's F and the density it was built from
     const double *dF = c->d_F.as<double>(), *dD = c->d_D.as<double>();
-    if (F) {
+    if (!fd_dev) {
         YG_HIP(hipMemcpyAsync(c->d_Fin.p, F, ms, hipMemcpyHostToDevice, c->stream));
         dF = c->d_Fin.as<double>();
-    }
-    if (D) {
         YG_HIP(hipMemcpyAsync(c->d_Din.p, D, ms, hipMemcpyHostToDevice, c->stream));
         dD = c->d_Din.as<double>();
     }
@@ -1137,6 +1166,52 @@ try {
                     c->d_gC.as<double>(), ldc, &c->d_gemm_ws));
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_scf_resident(ygpu_ctx_t *c, int on)
+try {
+    YG_TRY(check_ctx(c));
+    c->resident = on ? 1 : 0;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_get_fock_timings(ygpu_ctx_t *c, int max, ygpu_fock_timing_t *out, int *count)
+try {
+    YG_TRY(check_ctx(c));
+    if (max < 0 || (max > 0 && !out)) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_get_fock_timings: bad arguments");
+        return YGPU_ERR_INVALID;
+    }
+    const int n = (int) c->fock_hist.size();
+    for (int k = 0; k < n && k < max; ++k) out[k] = c->fock_hist[k];
+    if (count) *count = n;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_get_df_timing(ygpu_ctx_t *c, ygpu_df_timing_t *out)
+try {
+    YG_TRY(check_ctx(c));
+    if (!out) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_get_df_timing: null output");
+        return YGPU_ERR_INVALID;
+    }
+    *out = c->df_last;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_clear_timings(ygpu_ctx_t *c)
+try {
+    YG_TRY(check_ctx(c));
+    c->fock_hist.clear();
     return YGPU_OK;
 } catch (...) {
     return exception_status();

@@ -30,6 +30,8 @@ void ygpu_set_error(int code, const char *fmt, ...);
         if (s_ != YGPU_OK) return s_; \
     } while (0)
 
+// Fock-build timers kept per engine (ygpu_get_fock_timings)
+#define YG_MAX_FOCK_HIST 4096
 // largest n whose density-build eigenproblem runs on the one-workgroup tridiagonal reduction
 #define YG_EIG_NMAX_SMALL 272
 // largest n of the blocked two-kernel tridiagonal reduction (two rows per thread of its
@@ -120,6 +122,12 @@ struct ygpu_ctx {
     // DIIS transform or extrapolation (fcur_valid)
     int fock_valid = 0, density_valid = 0, fcur_valid = 0;
     DBuf d_Fcur;
+    // SCF hand-off mode (ygpu_set_scf_resident): host inputs that the device already holds are
+    // not uploaded again
+    int resident = 0;
+    // phase timers: every Fock build since the last ygpu_clear_timings, the last DF build
+    std::vector<ygpu_fock_timing_t> fock_hist;
+    ygpu_df_timing_t df_last = {};
     double diis_err[YGPU_MAX_DIIS] = {};   // max |R| per history slot, from its update
     int diis_err_ok[YGPU_MAX_DIIS] = {};
 };

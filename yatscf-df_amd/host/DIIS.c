@@ -53,7 +53,7 @@ void TinySCF_DIIS(TinySCF_t T)
     double st = get_wtime_sec();
     if (T->iter <= 1) {
         /* F = X^T F X only (DIIS.c:30-40) */
-        YS_GPU(ygpu_diis_transform(g, T->dev_resident ? NULL : T->F_mat, T->tmp_mat));
+        YS_GPU(ygpu_diis_transform(g, T->F_mat, T->tmp_mat));
         memcpy(T->F_mat, T->tmp_mat, DBL_SIZE * ms);
         T->t_diis = get_wtime_sec() - st;
         return;
@@ -64,14 +64,9 @@ void TinySCF_DIIS(TinySCF_t T)
     /* device: residual X^T (FDS - FDS^T) X into history slot idx, its dot products with
      * the stored residuals, and X^T F X stored as the slot's Fock matrix */
     double dot[MAX_DIIS + 1];
-    if (T->dev_resident) {
-        /* F and D of this iteration's Fock build are on the device; X^T F X is not copied out
-         * (the extrapolation below overwrites F_mat) */
-        YS_GPU(ygpu_diis_update(g, idx, T->DIIS_len, NULL, NULL, dot, NULL));
-    } else {
-        YS_GPU(ygpu_diis_update(g, idx, T->DIIS_len, T->F_mat, T->D_mat, dot, T->tmp_mat));
-        memcpy(T->F_mat, T->tmp_mat, DBL_SIZE * ms);
-    }
+    /* X^T F X is not copied out: the extrapolation below overwrites F_mat (inside the SCF loop
+     * the engine's hand-off mode also keeps F and D from being copied up again) */
+    YS_GPU(ygpu_diis_update(g, idx, T->DIIS_len, T->F_mat, T->D_mat, dot, NULL));
     YS_GPU(ygpu_diis_error(g, idx, &T->diis_error));
     for (int i = 0; i < T->DIIS_len; i++) {
         T->B_mat[idx * ldB + i] = dot[i];

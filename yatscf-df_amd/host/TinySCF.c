@@ -242,7 +242,9 @@ void TinySCF_do_SCF(TinySCF_t T)
         printf("Nuclear energy = %.10lf\n", T->nuc_energy);
     }
     T->iter = 0;
-    T->dev_resident = 0;                        /* iteration 0 starts from the host's guess */
+    ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
+    YS_GPU(ygpu_set_scf_resident(g, 0));        /* iteration 0 starts from the host's guess */
+    YS_GPU(ygpu_clear_timings(g));
     T->converged = 0;
     T->diis_error = -1.0;
     T->mixed_phase = T->mixed;
@@ -253,7 +255,7 @@ void TinySCF_do_SCF(TinySCF_t T)
         double st0 = get_wtime_sec();
         if (T->verbose) printf("--------------- Iteration %d ---------------\n", T->iter);
         if (T->mixed)
-            YS_GPU(ygpu_set_fock_precision((ygpu_ctx_t *) T->gpu, T->mixed_phase ? YGPU_FOCK_MIXED : YGPU_FOCK_FP64));
+            YS_GPU(ygpu_set_fock_precision(g, T->mixed_phase ? YGPU_FOCK_MIXED : YGPU_FOCK_FP64));
         TinySCF_build_FockMat(T);
         TinySCF_calc_energy(T);
         delta = fabs(T->HF_energy - prev);
@@ -262,7 +264,9 @@ void TinySCF_do_SCF(TinySCF_t T)
         TinySCF_DIIS(T);
         double st2 = get_wtime_sec();
         TinySCF_build_DenMat(T);
-        T->dev_resident = 1;                    /* D, C_occ now on the device for the next build */
+        /* D, C_occ, F now stay on the device between the calls of the loop (the host copies are
+         * still written, as the reference's are) */
+        if (T->iter == 0) YS_GPU(ygpu_set_scf_resident(g, 1));
         double st3 = get_wtime_sec();
         T->t_diis = st2 - st1;
         T->t_density = st3 - st2;
@@ -316,8 +320,8 @@ void TinySCF_do_SCF(TinySCF_t T)
         }
         T->iter++;
     }
-    T->dev_resident = 0;                        /* calls outside the loop copy their host inputs */
-    if (T->mixed) YS_GPU(ygpu_set_fock_precision((ygpu_ctx_t *) T->gpu, YGPU_FOCK_FP64));
+    YS_GPU(ygpu_set_scf_resident(g, 0));        /* calls outside the loop copy their host inputs */
+    if (T->mixed) YS_GPU(ygpu_set_fock_precision(g, YGPU_FOCK_FP64));
     T->energy_delta = delta;
     T->converged = delta < T->ene_tol;
     if (T->verbose) printf("--------------- SCF iterations finished ---------------\n");

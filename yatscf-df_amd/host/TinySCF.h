@@ -11,8 +11,11 @@
  *    (the device keeps the packed tensor; the host copies are 2 x nnz x naux
  *    doubles in the reference layout);
  *  - switches for reference behaviours that a user may want to change (the
- *    Schwarz element read, D2Cocc, the DIIS replacement rule), a convergence
- *    flag and per-phase timers.
+ *    Schwarz element read, D2Cocc, the DIIS replacement rule), the
+ *    mixed-precision and learned-guess options and a convergence flag (the
+ *    drop-in build_DF_tensor.c, build_Fock.c and build_density.c read none of
+ *    these: only the reference's fields and `gpu`; the phase timers live in
+ *    the device engine, ygpu_get_fock_timings).
  */
 #ifndef _YATSCF_TFOCK_H_
 #define _YATSCF_TFOCK_H_
@@ -116,16 +119,7 @@ struct TinySCF_struct
     int    verbose;         /* reference-style printout, default 1 */
     int    converged;
     double energy_delta;
-    double t_df[4];         /* 3c, 2c, inv-sqrt, DF GEMM (s) */
-    double t_fock[6];       /* aux J, J, aux K, K, symm, total (s) of the last Fock build */
     double t_density, t_diis;
-#define YS_MAX_HIST 512
-    int    n_hist;          /* per-iteration history of the phase timers (s) */
-    double t_hist[YS_MAX_HIST][6];
-    int    dev_resident;    /* set by TinySCF_do_SCF while the device holds the current D, C_occ
-                               and F: the Fock, DIIS and density calls then pass NULL for them
-                               and skip the host->device copies (the host matrices are still
-                               written as the reference's are) */
 };
 
 typedef struct TinySCF_struct* TinySCF_t;

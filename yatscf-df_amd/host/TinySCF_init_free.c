@@ -64,7 +64,6 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     }
     T->mixed_phase = 0;
     T->mixed_switch_iter = -1;
-    T->dev_resident = 0;
     T->diis_error = -1.0;
     T->guess_ml = 0;
     T->guess_weights[0] = 0;

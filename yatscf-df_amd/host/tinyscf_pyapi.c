@@ -15,6 +15,7 @@
 #include "build_density.h"
 #include "guess_ml.h"
 #include "utils.h"
+#include "yatscf_gpu.h"
 
 /* Runs one TinySCF API function by name; a failing device call returns its
  * status here (message from tinyscf_error) instead of aborting the caller. */
@@ -65,9 +66,27 @@ double *tinyscf_dptr(TinySCF_t T, const char *n, size_t *count)
     RET("pqA", T->pqA, (size_t) T->num_bf_pair_scr * T->df_nbf)
     RET("df_tensor", T->df_tensor, (size_t) T->num_bf_pair_scr * T->df_nbf)
     RET("B_mat", T->B_mat, (size_t) (MAX_DIIS + 1) * (MAX_DIIS + 1))
-    RET("t_df", T->t_df, 4) RET("t_fock", T->t_fock, 6)
-    RET("t_hist", &T->t_hist[0][0], (size_t) 6 * T->n_hist)
 #undef RET
+    /* phase timers kept by the device engine: the last DF build, every Fock build of the last
+     * SCF (six values each, the fields of ygpu_fock_timing_t) */
+    static double t_df[4];
+    static double t_hist[4096 * 6];
+    if (!strcmp(n, "t_df")) {
+        ygpu_df_timing_t t;
+        if (!T->gpu || ygpu_get_df_timing((ygpu_ctx_t *) T->gpu, &t) != YGPU_OK) { *count = 0; return NULL; }
+        t_df[0] = t.t_3c; t_df[1] = t.t_2c; t_df[2] = t.t_invsqrt; t_df[3] = t.t_dfgemm;
+        *count = 4;
+        return t_df;
+    }
+    if (!strcmp(n, "t_hist")) {
+        int cnt = 0;
+        if (!T->gpu || ygpu_get_fock_timings((ygpu_ctx_t *) T->gpu, 4096, (ygpu_fock_timing_t *) t_hist, &cnt) != YGPU_OK) {
+            *count = 0;
+            return NULL;
+        }
+        *count = (size_t) 6 * (cnt < 4096 ? cnt : 4096);
+        return t_hist;
+    }
     *count = 0;
     return NULL;
 }

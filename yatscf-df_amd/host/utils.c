@@ -41,8 +41,21 @@ void print_mat(double *mat, const int ldm, const int nrows, const int ncols, con
 static jmp_buf *ys_jmp = NULL;
 static char ys_msg[1536];
 
-void ys_set_error_jump(void *jb) { ys_jmp = (jmp_buf *) jb; }
 const char *ys_error_message(void) { return ys_msg; }
+
+/* failure handler of the device library's ygpu_fail (a failed call inside a drop-in TinySCF
+ * function): back to the installed jump target with the status */
+static void ys_failure_jump(int status, const char *where, const char *message)
+{
+    snprintf(ys_msg, sizeof(ys_msg), "%s failed (status %d): %s", where, status, message);
+    if (ys_jmp) longjmp(*ys_jmp, status);
+}
+
+void ys_set_error_jump(void *jb)
+{
+    ys_jmp = (jmp_buf *) jb;
+    (void) ygpu_set_failure_handler(jb ? ys_failure_jump : NULL);
+}
 
 /* A failed device call ends the current TinySCF function: with a jump target
  * installed (callers that handle errors, such as the Python bindings) control
