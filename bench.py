@@ -98,13 +98,25 @@ def profiled_traffic(kernel, workload):
     return None, None
 
 
-def phase_rows(r, want_mixed):
-    """Per-iteration phase timers (aux J, J, aux K, K, symm, J/K wall) of iterations >= 2 in the
-    requested arithmetic."""
+FOCK_FP64, FOCK_MIXED, FOCK_MIXED6, FOCK_INCR = 0, 1, 2, 3
+KIND_NAMES = {FOCK_FP64: "FP64", FOCK_MIXED: "BF16x3", FOCK_MIXED6: "BF16x6", FOCK_INCR: "FP64 reference + difference"}
+
+
+def kinds(r):
+    """Fock arithmetic of each SCF iteration of a solve (FP64 throughout for a plain solve)."""
     import numpy as np
-    h = r["hist"]
-    sw = (r["switch"] if r["switch"] >= 0 else len(h)) if r["mixed"] else 0
-    idx = [i for i in range(2, len(h)) if (i < sw) == want_mixed]
+    n = len(r["hist"])
+    if not r["mixed"]:
+        return np.zeros(n, dtype=int)
+    return np.asarray(r["phases"][:n], dtype=int)
+
+
+def phase_rows(r, kind, first=2):
+    """Per-iteration phase timers (aux J, J, aux K, K, symm, J/K wall) of iterations >= first
+    whose Fock build ran the given arithmetic."""
+    import numpy as np
+    h, k = r["hist"], kinds(r)
+    idx = [i for i in range(first, len(h)) if k[i] == kind]
     return h[idx] if idx else np.zeros((0, 6))
 
 
@@ -136,18 +148,32 @@ def kernel_table(runs, mixed):
         # B = pqA L^-T with L^-1 lower triangular: column q sums q + 1 products
         add("k_gemm128_nt (B = pqA L^-T, triangular, DF tensor)", "mfma", df_ms, 1,
             float(n_up) * naux * (naux + 1), FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
-    for want_mixed in ((True, False) if mixed else (False,)):
-        rows = [phase_rows(r, want_mixed) for r in runs]
-        n = int(round(np.mean([len(x) for x in rows]))) + (2 if want_mixed or not mixed else 0)
+    for kind in (FOCK_MIXED, FOCK_MIXED6, FOCK_INCR, FOCK_FP64):
+        rows = [phase_rows(r, kind) for r in runs]
+        n = int(round(np.mean([int(np.sum(kinds(r) == kind)) for r in runs])))
+        if n == 0:
+            continue
         xb = [v for x in rows for v in x[:, 2] * 1e3]
         sy = [v for x in rows for v in x[:, 3] * 1e3]
         jj = [v for x in rows for v in (x[:, 0] + x[:, 1]) * 1e3]
-        if want_mixed:
+        j32 = 2.0 * n_up * naux * 4 + 16.0 * nbf * nbf
+        if kind in (FOCK_MIXED, FOCK_MIXED6):
+            np_ = 3 if kind == FOCK_MIXED else 6
+            peak = BF16_PEAK_TFLOPS / np_
+            tag = "BF16x%d split" % np_
+            add("k_xbuild_bf (X = C_occ^T B, %s)" % tag, "mfma", xb, n, xb_flops, peak, "TFLOP/s", tag + ", f32 acc")
+            add("k_syrk_bf (K = X X^T, %s)" % tag, "mfma", sy, n, syrk_flops, peak, "TFLOP/s", tag + ", f32 acc")
+            add("J build (%s iterations, FP32 tensor, two passes)" % tag, "hbm", jj, n, j32, HBM_PEAK_GBS, "GB/s",
+                "f32 tensor, f64 sums")
+        elif kind == FOCK_INCR:
             peak = BF16_PEAK_TFLOPS / 3.0
-            add("k_xbuild_bf3 (X = C_occ^T B, BF16x3 split)", "mfma", xb, n, xb_flops, peak, "TFLOP/s", "bf16x3/f32")
-            add("k_syrk_bf3 (K = X X^T, BF16x3 split)", "mfma", sy, n, syrk_flops, peak, "TFLOP/s", "bf16x3/f32")
-            add("J build (FP32 tensor, two passes)", "hbm", jj, n, 2.0 * n_up * naux * 4 + 16.0 * nbf * nbf,
-                HBM_PEAK_GBS, "GB/s", "f32 tensor, f64 sums")
+            # X^P and X^M together: the X build over 2 n_occ columns; G = X^M (X^P)^T: every tile
+            add("k_xbuild_bf (X^[P|M], 2 n_occ columns, BF16x3 split)", "mfma", xb, n, 2.0 * xb_flops, peak, "TFLOP/s",
+                "BF16x3 split, f32 acc")
+            add("k_gemm_bf_nt (G = X^M X^P^T, difference K, BF16x3 split)", "mfma", sy, n,
+                2.0 * nbf * nbf * n_occ * naux, peak, "TFLOP/s", "BF16x3 split, f32 acc")
+            add("J build (difference iterations, FP32 tensor, two passes)", "hbm", jj, n, j32, HBM_PEAK_GBS, "GB/s",
+                "f32 tensor, f64 sums")
         else:
             add("k_xbuild (X = C_occ^T B, FP64)", "mfma", xb, n, xb_flops, FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
             add("k_syrk128 (K = X X^T, FP64)", "mfma", sy, n, syrk_flops, FP64_PEAK_TFLOPS, "TFLOP/s", "fp64")
@@ -244,7 +270,8 @@ def main():
                (", FP64 from iteration %d" % r["switch"]) if mixed else "", r["energy"]))
     secs = [r["seconds"] for r in runs]
     last = runs[-1]
-    hist = np.vstack([phase_rows(r, mixed) for r in runs])
+    # J/K per iteration: of the reduced-precision iterations of a mixed solve, else of all
+    hist = np.vstack([phase_rows(r, FOCK_MIXED if mixed else FOCK_FP64) for r in runs])
     if len(hist) == 0:
         hist = np.vstack([r["hist"][2:] for r in runs])
     # J and K run on two streams at once: the J/K figure is the wall time of the pair (start of
@@ -258,7 +285,8 @@ def main():
     dom = kernels[dom_name]
     traffic, traffic_src = profiled_traffic(dom_name.split()[0], args.workload)
     if mixed:
-        dtype = "bf16x3-split products with f32 accumulation (K), f32 tensor with f64 sums (J), then f64"
+        dtype = ("bf16x3-split products with f32 accumulation (K), f32 tensor with f64 sums (J), then f64 "
+                 "(full builds and difference builds against them)")
     else:
         dtype = "f64"
     out = {
@@ -276,13 +304,19 @@ def main():
         "data": "synthetic (seeded generated geometry; reference basis-set files)",
         "config": {"workload": WORKLOADS[args.workload],
                    "guess": ("SAD + learned pair correction" if last["guess_ml"] else "SAD"),
-                   "precision": ("mixed until the DIIS error is small, then FP64" if mixed else "FP64 (plain path)"),
+                   "precision": ("BF16x3-split K (FP32 accumulation) and FP32-tensor incremental J until the DIIS "
+                                 "error reaches the split's floor; then a full FP64 reference build and "
+                                 "difference-density builds K = K_ref + K(D - D_ref) (BF16x3 products of "
+                                 "aligned orbitals) with FP64 re-references; convergence confirmed by a full "
+                                 "FP64 build" if mixed else "FP64 (plain path)"),
                    "ene_tol": settings["ene_tol"],
                    "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
                                    "largest stored residual",
                    "basis": "cc-pVDZ", "aux_basis": "cc-pVDZ-RI", "nbf": nbf, "naux": naux, "n_occ": n_occ,
                    "pairs_both_triangles": nnz, "pairs_packed": n_up, "schwarz_tol": 1e-11},
         "iterations": last["iters"],
+        "iterations_by_arithmetic": {KIND_NAMES[k]: int(np.sum(kinds(last) == k)) for k in KIND_NAMES
+                                     if np.sum(kinds(last) == k)},
         "mixed_switch_iteration": last["switch"] if mixed else None,
         "energy_Eh": last["energy"],
         "converged": bool(last["converged"]),
@@ -310,7 +344,7 @@ def main():
                "iterations_saved": fp["iters"] - last["iters"],
                "fock_at_converged_density": fp["fock_check"],
                "converged": bool(fp["converged"] and last["converged"])}
-        fph = phase_rows(fp, False)
+        fph = phase_rows(fp, FOCK_FP64)
         if len(fph):
             cmp["fp64_jk_ms_per_iter"] = float(np.median(fph[:, 5]) * 1e3)
             cmp["fp64_k_ms_per_iter"] = float(np.median(fph[:, 2] + fph[:, 3]) * 1e3)

@@ -164,20 +164,46 @@ int ygpu_build_fock(ygpu_ctx_t *ctx, int n_occ, const double *D, const double *C
                     ygpu_fock_timing_t *timing);
 
 /* Arithmetic of the following ygpu_build_fock calls.  The reference builds
- * J and K in FP64 only (build_Fock.c:37-347); YGPU_FOCK_MIXED is this
- * library's reduced-precision mode for the early SCF iterations:
- *   K: X = C_occ^T B and K = X X^T as BF16-split products (hi*hi + hi*lo +
- *      lo*hi of bf16 hi/lo parts) on the matrix cores with FP32
- *      accumulation, FP32 X, FP64 sums across k chunks;
+ * J and K in FP64 only (build_Fock.c:37-347); YGPU_FOCK_MIXED and
+ * YGPU_FOCK_MIXED6 are this library's reduced-precision modes for the early
+ * SCF iterations:
+ *   K: X = C_occ^T B and K = X X^T as BF16-split products on the matrix cores
+ *      with FP32 accumulation, X kept as its BF16 parts, FP64 sums across k
+ *      chunks.  YGPU_FOCK_MIXED ("BF16x3"): each FP32 operand split into a
+ *      hi and a lo part, products hi*hi + hi*lo + lo*hi (~2^-16 relative).
+ *      YGPU_FOCK_MIXED6 ("BF16x6"): three parts, which hold the FP32 value
+ *      exactly, and the six products of part-index sum below 3 (~2^-24
+ *      relative, FP32-accurate products at twice the BF16x3 cost);
  *   J: incremental, J = J_prev + J(D - D_prev), from an FP32 copy of B with
- *      FP64 arithmetic; the first build after entering the mode is a full
- *      FP64 J that starts the increments.
- * YGPU_FOCK_FP64 (the default) is the reference arithmetic; switching modes
- * restarts the incremental J.  The FP32 copy of B costs n_up x naux_pad x 4
- * bytes of device memory while the mode is in use. */
+ *      FP64 arithmetic; the first build after entering a reduced mode from
+ *      FP64 is a full FP64 J that starts the increments.
+ * YGPU_FOCK_FP64 (the default) is the reference arithmetic; switching between
+ * it and a reduced mode restarts the incremental J.  The FP32 copy of B costs
+ * n_up x naux_pad x 4 bytes of device memory from the first reduced-mode build
+ * until the engine is destroyed (it is refreshed when the DF tensor changes). */
 #define YGPU_FOCK_FP64 0
 #define YGPU_FOCK_MIXED 1
+#define YGPU_FOCK_MIXED6 2
+/* YGPU_FOCK_INCR: difference-density build against the last FP64 build on this
+ * engine (the reference: its C_occ = C_ref and K_ref, with the same n_occ).
+ * The new C_occ is first rotated within its occupied space to C1 = C Q, Q
+ * the orthogonal polar factor of C_ref^T S C (Newton-Schulz iteration; S from
+ * ygpu_set_overlap), so that M = C1 - C_ref is as small as the change of the
+ * density; then D - D_ref = (P M^T + M P^T) / 2 with P = C1 + C_ref and
+ *   K = K_ref + (X^M (X^P)^T + X^P (X^M)^T) / 2,  X^Y_i = Y(partners)^T B(rows of i),
+ * the difference in BF16x3 products (FP32 accumulation), whose error scales with
+ * |M| instead of |K|; J incremental from the FP32 tensor as above.  Without a
+ * reference, or when the alignment does not converge (a principal angle near
+ * 90 degrees), the call makes a full FP64 build instead, which becomes the new
+ * reference.  Every FP64 build records its C_occ and K as the reference
+ * (nbf x n_occ + nbf x nbf doubles). */
+#define YGPU_FOCK_INCR 3
 int ygpu_set_fock_precision(ygpu_ctx_t *ctx, int mode);
+/* What the last ygpu_build_fock ran: mode_used (nullable) the YGPU_FOCK_*
+ * arithmetic (a YGPU_FOCK_INCR call that had to make a full FP64 build reports
+ * YGPU_FOCK_FP64), align_err (nullable) max |Q^T Q - I| of a difference build's
+ * alignment (0 otherwise). */
+int ygpu_fock_last_info(ygpu_ctx_t *ctx, int *mode_used, double *align_err);
 
 /* ---- Fock build without device allocation.  No counterpart in the reference
  * (whose Fock build, build_Fock.c:349-396, allocates nothing per call because

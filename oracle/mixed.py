@@ -6,15 +6,19 @@ mixed mode forms the same two products,
     X_i = C_occ(partners of i)^T B(rows of i)      (build_Fock.c:249-285)
     K   = X X^T                                     (build_Fock.c:317-341)
 
-from FP32 operands split as x = hi + lo with hi = bf16(x), lo = bf16(x - hi)
-(round to nearest even), each product taken as hi*hi + hi*lo + lo*hi.  Here
-the split is emulated bit-exactly and the split products are summed in FP64,
-so the result differs from the GPU's only by the GPU's FP32 accumulation;
-the FP64 K of the oracle measures the method's own error.
+from FP32 operands split into np BF16 parts, p_0 = bf16(x), p_1 = bf16(x - p_0),
+p_2 = bf16(x - p_0 - p_1) (round to nearest even), each product taken as the
+sum of the part products p_a q_b with a + b < np: np = 2 ("BF16x3", hi*hi +
+hi*lo + lo*hi) or np = 3 ("BF16x6", six products).  Here the split is
+emulated bit-exactly and the split products are summed in FP64, so the result
+differs from the GPU's only by the GPU's FP32 accumulation; the FP64 K of the
+oracle measures the method's own error.
 
 Pinned by: test_oracle_mixed.py (bf16 rounding against hand-worked bit
-patterns and ties-to-even cases; hi + lo against the float32 value; the
-three-term product against the exact product within its 2^-16 bound).
+patterns and ties-to-even cases; the parts of hand-worked float32 values;
+hi + lo against the float32 value within 2^-17, the three parts equal to it
+exactly; the three- and six-term products against the exact product within
+their 2^-16 and 2^-23 bounds).
 """
 import numpy as np
 
@@ -26,19 +30,28 @@ def bf16_round(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-def split(x):
-    """x (any float) -> (hi, lo) bf16 values as float64 arrays, x32 = float32(x)."""
-    x32 = np.asarray(x, dtype=np.float32)
-    hi = bf16_round(x32)
-    lo = bf16_round((x32 - hi).astype(np.float32))
-    return hi.astype(np.float64), lo.astype(np.float64)
+def split(x, nparts=2):
+    """x (any float) -> its nparts bf16 parts as float64 arrays (x32 = float32(x); each part
+    the bf16 rounding of what the earlier ones leave, the remainders formed in float32)."""
+    r = np.asarray(x, dtype=np.float32)
+    parts = []
+    for _ in range(nparts):
+        h = bf16_round(r)
+        parts.append(h.astype(np.float64))
+        r = (r - h).astype(np.float32)
+    return tuple(parts)
 
 
-def split_matmul(a, b):
-    """a @ b with both operands split: hi_a hi_b + hi_a lo_b + lo_a hi_b, FP64 sums."""
-    ah, al = split(a)
-    bh, bl = split(b)
-    return ah @ bh + ah @ bl + al @ bh
+def split_matmul(a, b, nparts=2):
+    """a @ b with both operands split into nparts parts: the part products p_a q_b with
+    a + b < nparts (hi_a hi_b + hi_a lo_b + lo_a hi_b for two parts), FP64 sums."""
+    pa = split(a, nparts)
+    pb = split(b, nparts)
+    out = 0.0
+    for s in range(nparts):
+        for i in range(s + 1):
+            out = out + pa[i] @ pb[s - i]
+    return out
 
 
 def abs_x(B_rows, mask, Cocc):
@@ -53,8 +66,8 @@ def abs_x(B_rows, mask, Cocc):
     return A
 
 
-def k_build(B_rows, mask, Cocc):
-    """Mixed-mode K from the reference-layout DF tensor.
+def k_build(B_rows, mask, Cocc, nparts=2):
+    """Mixed-mode K from the reference-layout DF tensor (nparts BF16 parts per operand).
 
     B_rows: nnz x naux (rows in the CSR order of bf_pair_mask); mask: nbf x nbf
     row ids (-1 screened); Cocc: nbf x n_occ.  Returns (K, X32) with X32 the
@@ -66,8 +79,8 @@ def k_build(B_rows, mask, Cocc):
         js = np.nonzero(mask[i] >= 0)[0]
         if len(js) == 0:
             continue
-        X[i] = split_matmul(Cocc[js].T, B_rows[mask[i, js]])
+        X[i] = split_matmul(Cocc[js].T, B_rows[mask[i, js]], nparts)
     X32 = X.astype(np.float32)
     Xf = X32.reshape(nbf, -1)
-    K = split_matmul(Xf, Xf.T)
+    K = split_matmul(Xf, Xf.T, nparts)
     return K, X32

@@ -64,9 +64,11 @@ def ref(name):
 
 
 @pytest.mark.parametrize("name", ["water", "c10h22"])
-def test_mixed_fock_against_emulation(ygpu, name):
+@pytest.mark.parametrize("nparts", [2, 3])
+def test_mixed_fock_against_emulation(ygpu, name, nparts):
+    """nparts 2: YGPU_FOCK_MIXED (BF16x3); 3: YGPU_FOCK_MIXED6 (BF16x6)."""
     import mixed
-    log("case: mixed Fock", name)
+    log("case: mixed Fock", name, "BF16 parts", nparts)
     r = ref(name)
     o = r.o
     e = ygpu.Engine(0)
@@ -75,7 +77,7 @@ def test_mixed_fock_against_emulation(ygpu, name):
         e.set_sparsity(o.i("bf_pair_mask"), o.i("uniq_sp_lid"), o.i("uniq_sp_rid"), o.d("sp_scrval"),
                        o.d("df_sp_scrval"))
         e.set_df_tensor(o.d("df_tensor"))
-        e.set_fock_precision(ygpu.FOCK_MIXED)
+        e.set_fock_precision(ygpu.FOCK_MIXED if nparts == 2 else ygpu.FOCK_MIXED6)
         out = []
         for st in r.states:
             out.append(e.build_fock(st["D"], st["C"], r.H))
@@ -84,7 +86,7 @@ def test_mixed_fock_against_emulation(ygpu, name):
     finally:
         e.close()
     for k, (st, got) in enumerate(zip(r.states, out)):
-        Kem, X32 = mixed.k_build(r.B, r.mask, st["C"])
+        Kem, X32 = mixed.k_build(r.B, r.mask, st["C"], nparts)
         # S_ij = sum_k |X_ik| |X_jk| >= |K_ij|: the scale of rounding errors made in the K product;
         # T_ij = sum_k (A_ik |X_jk| + |X_ik| A_jk) with A = |C|^T |B| summed like X: the scale of
         # the FP32 accumulation errors of X carried into K
@@ -104,8 +106,9 @@ def test_mixed_fock_against_emulation(ygpu, name):
         # rounding per MFMA (three per 16-deep step) in X and in each 1024-long chunk of K,
         # ~1.5e-6 T when emulated step by step for c10h22; a dropped split term shows at ~1e-4 T
         assert e_emul.max() < 2.0 ** -17
-        # the method's own error: <= 3 2^-17 per product plus the accumulation
-        assert e_fp64.max() < 2.0 ** -14
+        # the method's own error: BF16x3 <= 3 2^-17 per product plus the accumulation; BF16x6
+        # ~2^-23 per product, the FP32 accumulation (~1e-6 of S) left
+        assert e_fp64.max() < (2.0 ** -14 if nparts == 2 else 2.0 ** -17)
         assert np.allclose(got["K"], got["K"].T, rtol=0, atol=0)
         if k == 0:
             assert d_J < 1e-10            # first mixed build: full FP64 J
@@ -165,3 +168,64 @@ def test_mixed_scf_reaches_fp64_energy(ygpu):
         assert abs(e - e_ref) < 1e-8
     finally:
         T.close()
+
+
+def test_difference_density_fock(ygpu):
+    """YGPU_FOCK_INCR on C10H22: K = K_ref + K(D - D_ref) with the difference in BF16x3 products
+    of the aligned orbitals.  States are consecutive SCF iterations of the oracle (occupied
+    orbitals from its eigensolver, S-orthonormal).  (1) The next state against its FP64 K: the
+    error is a small fraction of the change K - K_ref (it scales with the difference, not with
+    K).  (2) The reference orbitals rotated within their occupied space (the same density): the
+    alignment undoes the rotation and K equals K_ref to rounding.  (3) Without a reference the
+    call makes a full FP64 build."""
+    import oracle
+    log("case: difference-density Fock build c10h22")
+    s = system("c10h22")
+    o = oracle.OracleSCF(s, niters=60)
+    o.setup()
+    o.build_df_tensor()
+    n, nocc = s.nbf, s.n_occ
+    H, S, X = o.mat("Hcore_mat").copy(), o.mat("S_mat").copy(), o.mat("X_mat").copy()
+    states = []
+    for _ in range(4):
+        D, C = o.mat("D_mat").copy(), o.d("Cocc_mat", (n, nocc)).copy()
+        o.build_fock()
+        states.append(dict(D=D, C=C, K=o.mat("K_mat").copy()))
+        o.diis()
+        o.build_density()
+    sa, sb = states[2], states[3]
+    assert np.abs(sa["C"].T @ S @ sa["C"] - np.eye(nocc)).max() < 1e-10
+    R, _ = np.linalg.qr(np.random.RandomState(9).randn(nocc, nocc))
+    e = ygpu.Engine(0)
+    try:
+        e.set_system(s)
+        e.set_sparsity(o.i("bf_pair_mask"), o.i("uniq_sp_lid"), o.i("uniq_sp_rid"), o.d("sp_scrval"),
+                       o.d("df_sp_scrval"))
+        e.set_df_tensor(o.d("df_tensor"))
+        e.set_overlap(S, X)
+        e.set_fock_precision(ygpu.FOCK_INCR)
+        first = e.build_fock(sa["D"], sa["C"], H)               # no reference yet: full FP64
+        m_first = e.fock_last_info()[0]
+        e.set_fock_precision(ygpu.FOCK_FP64)
+        ref0 = e.build_fock(sa["D"], sa["C"], H)                # the reference
+        e.set_fock_precision(ygpu.FOCK_INCR)
+        inc = e.build_fock(sb["D"], sb["C"], H)
+        m_inc, a_inc = e.fock_last_info()
+        e.set_fock_precision(ygpu.FOCK_FP64)
+        e.build_fock(sa["D"], sa["C"], H)                       # back to the reference
+        e.set_fock_precision(ygpu.FOCK_INCR)
+        rot = e.build_fock(sa["D"], sa["C"] @ R, H)
+        m_rot, a_rot = e.fock_last_info()
+    finally:
+        e.close()
+    assert m_first == ygpu.FOCK_FP64 and np.array_equal(first["K"], ref0["K"])
+    assert m_inc == ygpu.FOCK_INCR and m_rot == ygpu.FOCK_INCR
+    dK = np.abs(sb["K"] - sa["K"]).max()
+    err = np.abs(inc["K"] - sb["K"]).max()
+    err_rot = np.abs(rot["K"] - ref0["K"]).max()
+    log("  max|K_b - K_a| %.3e; difference build vs FP64 K_b: %.3e (%.2e of the change; alignment %.1e); "
+        "rotated reference: %.3e (alignment %.1e)" % (dK, err, err / dK, a_inc, err_rot, a_rot))
+    assert a_inc < 1e-12 and a_rot < 1e-12
+    assert err < 2.0 ** -12 * dK
+    assert err_rot < 1e-12 * np.abs(ref0["K"]).max()
+    assert np.allclose(inc["K"], inc["K"].T, rtol=0, atol=0)

@@ -493,11 +493,15 @@ def test_k_build_tile_variants(ygpu, n_occ):
         r64 = e.build_fock(st.D0, C, st.o.mat("Hcore_mat"))
         e.set_fock_precision(ygpu.FOCK_MIXED)
         rmx = e.build_fock(st.D0, C, st.o.mat("Hcore_mat"))
+        e.set_fock_precision(ygpu.FOCK_MIXED6)
+        rm6 = e.build_fock(st.D0, C, st.o.mat("Hcore_mat"))
     e64 = np.abs(r64["K"] - K).max() / np.abs(K).max()
     emx = (np.abs(rmx["K"] - K) / S).max()
-    log("  FP64 max|dK|/max|K| %.3e; mixed max|dK|/S %.3e" % (e64, emx))
+    em6 = (np.abs(rm6["K"] - K) / S).max()
+    log("  FP64 max|dK|/max|K| %.3e; BF16x3 max|dK|/S %.3e; BF16x6 max|dK|/S %.3e" % (e64, emx, em6))
     assert e64 < 1e-12
     assert emx < 2.0 ** -14
+    assert em6 < 2.0 ** -17
 
 
 @pytest.mark.parametrize("name", ["water", "c10h22"])

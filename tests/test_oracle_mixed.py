@@ -71,3 +71,41 @@ def test_k_build_matches_split_products(nbf, nocc, naux):
     assert np.max(np.abs(X32 - X)) <= 1e-4 * np.max(np.abs(X))
     assert np.max(np.abs(K - Kx)) <= 1e-4 * np.max(np.abs(Kx))
     assert np.allclose(K, K.T, rtol=0, atol=1e-12 * np.abs(K).max())
+
+
+def test_three_part_split_known_patterns():
+    """Hand-worked parts: x = 1 + 2^-9 + 2^-18 + 2^-23 splits into 1, 2^-9 and 2^-18 + 2^-23 (each
+    part the bf16 rounding of the remainder); x = 1 + 2^-8 (a tie) gives hi = 1, mid = 2^-8."""
+    x = np.array([1.0 + 2 ** -9 + 2 ** -18 + 2 ** -23, 1.0 + 2 ** -8, -3.0 - 2 ** -12], dtype=np.float32)
+    p0, p1, p2 = mixed.split(x, 3)
+    assert np.array_equal(p0, [1.0, 1.0, -3.0])
+    assert np.array_equal(p1, [2.0 ** -9, 2.0 ** -8, -2.0 ** -12])
+    assert np.array_equal(p2, [2.0 ** -18 + 2.0 ** -23, 0.0, 0.0])
+    # two parts are the first two of three
+    h, lo = mixed.split(x, 2)
+    assert np.array_equal(h, p0) and np.array_equal(lo, p1)
+
+
+def test_three_parts_hold_float32_exactly():
+    x = np.random.RandomState(4).randn(100000) * np.exp(np.random.RandomState(5).randn(100000) * 5)
+    x32 = x.astype(np.float32).astype(np.float64)
+    p0, p1, p2 = mixed.split(x, 3)
+    assert np.array_equal(p0 + p1 + p2, x32)
+    assert np.all(np.abs(p1) <= 2.0 ** -8 * np.abs(p0)) and np.all(np.abs(p2) <= 2.0 ** -8 * np.abs(p1))
+
+
+def test_six_term_product_error_bound():
+    """BF16x6: |ab - sum_{a+b<3} p_a q_b| <= ~3 2^-24 |a||b| per product (the dropped
+    mid*lo, lo*mid, lo*lo terms), 2^7 times tighter than BF16x3's bound; the nine-term sum is
+    the exact float32 product."""
+    rng = np.random.RandomState(6)
+    a, b = rng.randn(40, 300), rng.randn(300, 30)
+    exact = a.astype(np.float32).astype(np.float64) @ b.astype(np.float32).astype(np.float64)
+    got = mixed.split_matmul(a, b, 3)
+    bound = 3 * 2.0 ** -23 * (np.abs(a) @ np.abs(b))
+    assert np.all(np.abs(got - exact) <= bound)
+    got3 = mixed.split_matmul(a, b, 2)
+    assert np.max(np.abs(got - exact)) < np.max(np.abs(got3 - exact)) / 16
+    pa, pb = mixed.split(a, 3), mixed.split(b, 3)
+    nine = sum(pa[i] @ pb[j] for i in range(3) for j in range(3))
+    assert np.max(np.abs(nine - exact)) < 1e-12 * np.max(np.abs(exact))

@@ -363,6 +363,7 @@ try {
 static void invalidate_handoffs(ygpu_ctx *c)
 {
     c->fock_valid = c->density_valid = c->fcur_valid = 0;
+    c->kref_valid = 0;
     for (int &ok : c->diis_err_ok) ok = 0;
 }
 
@@ -695,6 +696,7 @@ try {
     c->d_pqA.release();   // (mn|P) is not needed once B is formed
     c->have_df = 1;
     c->have_B32 = 0;
+    c->kref_valid = 0;
     c->jinc_valid = 0;
     return YGPU_OK;
 } catch (...) {
@@ -731,6 +733,7 @@ try {
     YG_HIP(hipStreamSynchronize(c->stream));
     c->have_df = 1;
     c->have_B32 = 0;
+    c->kref_valid = 0;
     c->jinc_valid = 0;
     return YGPU_OK;
 } catch (...) {
@@ -771,7 +774,57 @@ try {
     c->fock_valid = 0;
     float tj1 = 0.f, tj2 = 0.f;
     const long nn = (long) nbf * nbf;
-    if (c->fock_mode == YGPU_FOCK_MIXED) {
+    int mode = c->fock_mode;
+    c->incr_fallback = 0;
+    if (mode == YGPU_FOCK_INCR) {
+        // the difference build needs the C_occ and K of an FP64 build with the same n_occ
+        if (!c->kref_valid || c->kref_nocc != n_occ || !c->have_S) {
+            mode = YGPU_FOCK_FP64;
+            c->incr_fallback = 1;
+        } else {
+            // orthogonal Q minimising |C Q - C_ref| in the S metric (maximising tr(Q^T A)): the
+            // polar factor of A = C^T S C_ref by Newton-Schulz, Q <- Q (3 I - Q^T Q) / 2 (the
+            // singular values of A are the cosines of the principal angles between the two
+            // occupied spaces)
+            const size_t qs = (size_t) n_occ * n_occ * sizeof(double);
+            YG_TRY(c->d_SC.ensure((size_t) nbf * n_occ * sizeof(double)));
+            YG_TRY(c->d_C1.ensure((size_t) nbf * n_occ * sizeof(double)));
+            YG_TRY(c->d_Qa.ensure(qs));
+            YG_TRY(c->d_Qb.ensure(qs));
+            YG_TRY(c->d_Qt.ensure(qs));
+            YG_TRY(c->d_nserr.ensure(sizeof(double)));
+            double *Qa = c->d_Qa.as<double>(), *Qb = c->d_Qb.as<double>(), *Qt = c->d_Qt.as<double>();
+            YG_TRY(yg_dgemm(c->stream, 0, 0, nbf, n_occ, nbf, 1.0, c->d_S.as<double>(), nbf, c->d_Cocc.as<double>(),
+                            n_occ, 0.0, c->d_SC.as<double>(), n_occ, &c->d_gemm_ws));
+            YG_TRY(yg_dgemm(c->stream, 1, 0, n_occ, n_occ, nbf, 1.0, c->d_SC.as<double>(), n_occ,
+                            c->d_Cref.as<double>(), n_occ, 0.0, Qa, n_occ, &c->d_gemm_ws));
+            for (int it = 0; it < YG_NS_ITERS; ++it) {
+                YG_TRY(yg_dgemm(c->stream, 1, 0, n_occ, n_occ, n_occ, 1.0, Qa, n_occ, Qa, n_occ, 0.0, Qt, n_occ,
+                                &c->d_gemm_ws));
+                YG_TRY(yg_ns_shift(c->stream, n_occ, Qt));
+                YG_TRY(yg_dgemm(c->stream, 0, 0, n_occ, n_occ, n_occ, 0.5, Qa, n_occ, Qt, n_occ, 0.0, Qb, n_occ,
+                                &c->d_gemm_ws));
+                std::swap(Qa, Qb);
+            }
+            YG_TRY(yg_dgemm(c->stream, 1, 0, n_occ, n_occ, n_occ, 1.0, Qa, n_occ, Qa, n_occ, 0.0, Qt, n_occ,
+                            &c->d_gemm_ws));
+            YG_TRY(yg_dev_identity(c->stream, n_occ, Qt, c->d_nserr.as<double>()));
+            YG_HIP(hipMemcpyAsync(&c->incr_align_err, c->d_nserr.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            YG_HIP(hipStreamSynchronize(c->stream));
+            if (!(c->incr_align_err < 1e-12)) {
+                // the occupied spaces are too far apart for the iteration (a principal angle near
+                // 90 degrees): a full FP64 build, which becomes the new reference
+                mode = YGPU_FOCK_FP64;
+                c->incr_fallback = 1;
+            } else {
+                // C_occ aligned to the reference: C1 = C Q
+                YG_TRY(yg_dgemm(c->stream, 0, 0, nbf, n_occ, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, Qa, n_occ,
+                                0.0, c->d_C1.as<double>(), n_occ, &c->d_gemm_ws));
+            }
+        }
+    }
+    if (mode == YGPU_FOCK_MIXED || mode == YGPU_FOCK_MIXED6 || mode == YGPU_FOCK_INCR) {
+        const int nparts = mode == YGPU_FOCK_MIXED6 ? 3 : 2;
         if (!c->have_B32) {
             YG_TRY(c->d_B32.ensure((size_t) c->n_up * c->naux_pad * sizeof(float)));
             YG_TRY(yg_to_f32(c->stream, c->d_B.as<double>(), c->d_B32.as<float>(), c->n_up * c->naux_pad));
@@ -800,15 +853,29 @@ try {
             YG_TRY(yg_accum(c->side, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
         }
         YG_HIP(hipEventRecord(c->ev[3], c->side));
-        // K from BF16-split products; X stored pre-split in the X buffer
         YG_HIP(hipEventRecord(c->ev[8], c->stream));
-        YG_TRY(yg_to_f32(c->stream, c->d_Cocc.as<double>(), c->d_Cocc32.as<float>(), (long) nbf * n_occ));
-        YG_TRY(yg_xbuild_bf3(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
-                             c->d_xpartner.as<int>(), c->d_Cocc32.as<float>(), c->d_B32.as<float>(),
-                             c->d_X.as<float>()));
-        YG_HIP(hipEventRecord(c->ev[4], c->stream));
-        YG_TRY(yg_syrk_bf3(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(), c->d_K.as<double>(),
-                           c->d_Kpart32));
+        if (mode == YGPU_FOCK_INCR) {
+            // D - D_ref = (P M^T + M P^T) / 2 with P = C1 + C_ref, M = C1 - C_ref: K(D - D_ref) =
+            // (X^M (X^P)^T + X^P (X^M)^T) / 2, BF16-split products whose error scales with |M|;
+            // K = K_ref (FP64) + that difference
+            YG_TRY(c->d_PM32.ensure((size_t) nbf * 2 * n_occ * sizeof(float)));
+            YG_TRY(yg_pm32(c->stream, nbf, n_occ, c->d_C1.as<double>(), c->d_Cref.as<double>(), c->d_PM32.as<float>()));
+            YG_TRY(yg_xbuild_bf(c->stream, nparts, nbf, 2 * n_occ, (int) c->naux_pad, c->d_xptr.as<int>(),
+                                c->d_xup.as<int>(), c->d_xpartner.as<int>(), c->d_PM32.as<float>(),
+                                c->d_B32.as<float>(), c->d_X.as<float>()));
+            YG_HIP(hipEventRecord(c->ev[4], c->stream));
+            YG_TRY(yg_incr_k_bf(c->stream, nparts, nbf, n_occ, (int) c->naux_pad, c->d_X.as<float>(),
+                                c->d_Kref.as<double>(), c->d_K.as<double>(), c->d_Kpart32));
+        } else {
+            // K from BF16-split products; X stored pre-split in the X buffer
+            YG_TRY(yg_to_f32(c->stream, c->d_Cocc.as<double>(), c->d_Cocc32.as<float>(), (long) nbf * n_occ));
+            YG_TRY(yg_xbuild_bf(c->stream, nparts, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(),
+                                c->d_xup.as<int>(), c->d_xpartner.as<int>(), c->d_Cocc32.as<float>(),
+                                c->d_B32.as<float>(), c->d_X.as<float>()));
+            YG_HIP(hipEventRecord(c->ev[4], c->stream));
+            YG_TRY(yg_syrk_bf(c->stream, nparts, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(),
+                              c->d_K.as<double>(), c->d_Kpart32));
+        }
         YG_HIP(hipEventRecord(c->ev[6], c->stream));
         YG_TRY(join_side(c, c->side));
     } else {
@@ -826,6 +893,14 @@ try {
         YG_TRY(yg_syrk_upper(c->stream, nbf, (long) n_occ * c->naux_pad, c->d_X.as<double>(), c->d_K.as<double>(),
                              c->d_Kpart));
         YG_HIP(hipEventRecord(c->ev[6], c->stream));
+        // this FP64 build is the reference of later difference builds (YGPU_FOCK_INCR)
+        YG_TRY(c->d_Cref.ensure((size_t) nbf * n_occ * sizeof(double)));
+        YG_TRY(c->d_Kref.ensure(ms));
+        YG_HIP(hipMemcpyAsync(c->d_Cref.p, c->d_Cocc.p, (size_t) nbf * n_occ * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+        YG_HIP(hipMemcpyAsync(c->d_Kref.p, c->d_K.p, ms, hipMemcpyDeviceToDevice, c->stream));
+        c->kref_valid = 1;
+        c->kref_nocc = n_occ;
         YG_TRY(join_side(c, c->side));
     }
     YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
@@ -847,6 +922,7 @@ try {
     tm.t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;
     if (c->fock_hist.size() < YG_MAX_FOCK_HIST) c->fock_hist.push_back(tm);
     if (t) *t = tm;
+    c->last_mode = mode;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -855,18 +931,13 @@ try {
 int ygpu_set_fock_precision(ygpu_ctx_t *c, int mode)
 try {
     YG_TRY(check_ctx(c));
-    if (mode != YGPU_FOCK_FP64 && mode != YGPU_FOCK_MIXED) {
+    if (mode != YGPU_FOCK_FP64 && mode != YGPU_FOCK_MIXED && mode != YGPU_FOCK_MIXED6 && mode != YGPU_FOCK_INCR) {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_fock_precision: unknown mode %d", mode);
         return YGPU_ERR_INVALID;
     }
-    if (mode != c->fock_mode) c->jinc_valid = 0;
+    // the incremental J restarts when the arithmetic of J changes (FP64 <-> an FP32-tensor mode)
+    if ((mode == YGPU_FOCK_FP64) != (c->fock_mode == YGPU_FOCK_FP64)) c->jinc_valid = 0;
     c->fock_mode = mode;
-    if (mode == YGPU_FOCK_FP64) {
-        // the FP32 tensor copy and the incremental state are only used by the mixed mode
-        c->d_B32.release();
-        c->have_B32 = 0;
-        c->d_Kpart32.release();
-    }
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -1166,6 +1237,16 @@ try {
                     c->d_gC.as<double>(), ldc, &c->d_gemm_ws));
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_fock_last_info(ygpu_ctx_t *c, int *mode_used, double *align_err)
+try {
+    YG_TRY(check_ctx(c));
+    if (mode_used) *mode_used = c->last_mode;
+    if (align_err) *align_err = c->last_mode == YGPU_FOCK_INCR ? c->incr_align_err : 0.0;
     return YGPU_OK;
 } catch (...) {
     return exception_status();

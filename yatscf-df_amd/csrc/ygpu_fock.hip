@@ -289,6 +289,46 @@ __global__ void k_fill(long n, double *__restrict__ x, double v)
     for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x) x[i] = v;
 }
 
+// T = 3 I - T (n x n): the Newton-Schulz step Q <- Q (3 I - Q^T Q) / 2 of the orthogonal polar factor
+__global__ void k_ns_shift(int n, double *__restrict__ T)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) n * n) return;
+    const int i = (int) (idx / n), j = (int) (idx % n);
+    T[idx] = (i == j ? 3.0 : 0.0) - T[idx];
+}
+
+// max |T - I| (n x n), one workgroup
+__global__ __launch_bounds__(1024) void k_dev_identity(int n, const double *__restrict__ T, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double m = 0.0;
+    for (long e = threadIdx.x; e < (long) n * n; e += blockDim.x) {
+        const int i = (int) (e / n), j = (int) (e % n);
+        m = fmax(m, fabs(T[e] - (i == j ? 1.0 : 0.0)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int) (blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+        out[0] = m;
+    }
+}
+
+// PM[i][s] = C1[i][s] + C0[i][s], PM[i][n_occ + s] = C1[i][s] - C0[i][s], in FP32 (nbf x 2 n_occ)
+__global__ void k_pm32(int nbf, int n_occ, const double *__restrict__ C1, const double *__restrict__ C0,
+                       float *__restrict__ PM)
+{
+    const long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long) nbf * n_occ) return;
+    const int i = (int) (e / n_occ), s = (int) (e % n_occ);
+    const double a = C1[e], b = C0[e];
+    PM[(size_t) i * 2 * n_occ + s] = (float) (a + b);
+    PM[(size_t) i * 2 * n_occ + n_occ + s] = (float) (a - b);
+}
+
 struct Coefs {
     double c[YGPU_MAX_DIIS];
 };
@@ -461,6 +501,30 @@ int yg_transpose(hipStream_t s, int rows, int cols, const double *A, double *AT)
 {
     dim3 grid((cols + 31) / 32, (rows + 31) / 32);
     hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, rows, cols, A, AT);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_ns_shift(hipStream_t s, int n, double *T)
+{
+    const long tot = (long) n * n;
+    hipLaunchKernelGGL(k_ns_shift, dim3((tot + 255) / 256), dim3(256), 0, s, n, T);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_dev_identity(hipStream_t s, int n, const double *T, double *out)
+{
+    hipLaunchKernelGGL(k_dev_identity, dim3(1), dim3(1024), 0, s, n, T, out);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_pm32(hipStream_t s, int nbf, int n_occ, const double *C1, const double *C0, float *PM)
+{
+    const long tot = (long) nbf * n_occ;
+    if (tot == 0) return YGPU_OK;
+    hipLaunchKernelGGL(k_pm32, dim3((tot + 255) / 256), dim3(256), 0, s, nbf, n_occ, C1, C0, PM);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

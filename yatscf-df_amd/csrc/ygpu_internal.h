@@ -30,6 +30,9 @@ void ygpu_set_error(int code, const char *fmt, ...);
         if (s_ != YGPU_OK) return s_; \
     } while (0)
 
+// Newton-Schulz steps of the orthogonal alignment of a difference-density build (from principal
+// angles of up to 60 degrees the error falls below 1e-15 in 9 steps)
+#define YG_NS_ITERS 10
 // Fock-build timers kept per engine (ygpu_get_fock_timings)
 #define YG_MAX_FOCK_HIST 4096
 // largest n whose density-build eigenproblem runs on the one-workgroup tridiagonal reduction
@@ -112,6 +115,13 @@ struct ygpu_ctx {
     int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
     int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
+    // difference-density exchange (YGPU_FOCK_INCR): the C_occ and FP64 K of the last FP64 build,
+    // the aligned C_occ, the polar factor and its Newton-Schulz work, the FP32 [P | M] block
+    int kref_valid = 0, kref_nocc = 0;
+    DBuf d_Cref, d_Kref, d_C1, d_SC, d_Qa, d_Qb, d_Qt, d_PM32, d_nserr;
+    double incr_align_err = 0.0;  // max |Q^T Q - I| of the last difference build
+    int incr_fallback = 0;        // the last YGPU_FOCK_INCR call fell back to a full FP64 build
+    int last_mode = YGPU_FOCK_FP64;   // arithmetic the last Fock build ran
 
     // own eigensolver paths: reflector scalars and tridiagonal workspace
     DBuf d_tau, d_tri;
@@ -206,12 +216,21 @@ int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                  
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)
 int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
-// X (nbf x n_occ naux_pad) is stored pre-split: the BF16 hi values, then the BF16 lo values
-// (the bytes of an FP32 X); the X build writes it, the SYRK reads it
-int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+// X (nbf x n_occ naux_pad) is stored pre-split into nparts (2 or 3) BF16 arrays, one after the
+// other (part q of element e at q * nbf * n_occ * naux_pad + e); the X build writes it, the
+// SYRK reads it
+int yg_xbuild_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
                   const int *xpartner, const float *Cocc32, const float *B32, void *X);
-int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part);
+int yg_syrk_bf(hipStream_t s, int nparts, int nbf, long kdim, const void *X, double *K, DBuf &part);
 int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const double *K, double *F);
+// difference-density exchange: K = Kref + (G + G^T) / 2 with G = X^M (X^P)^T, X the pre-split X
+// build of the combined [P | M] occupied block (2 n_occ columns per basis function)
+int yg_incr_k_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, const void *X, const double *Kref,
+                 double *K, DBuf &part);
+int yg_ns_shift(hipStream_t s, int n, double *T);                                    // T = 3 I - T
+int yg_dev_identity(hipStream_t s, int n, const double *T, double *out);             // out[0] = max |T - I|
+// PM = [C1 + C0 | C1 - C0] in FP32 (nbf x 2 n_occ)
+int yg_pm32(hipStream_t s, int nbf, int n_occ, const double *C1, const double *C0, float *PM);
 int yg_scale_rows(hipStream_t s, int n, int ncol, double *A, int lda, const double *w, int mode);
 int yg_gather_cols(hipStream_t s, int nbf, int n_occ, const double *C, const int *idx, double *Cocc);
 int yg_transpose(hipStream_t s, int rows, int cols, const double *A, double *AT);

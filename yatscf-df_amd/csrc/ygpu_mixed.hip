@@ -4,11 +4,14 @@
 //   X_i = C_occ(partners of i)^T B(rows of i)   ("aux. K tensor", :249-285)
 //   K   = X X^T                                  ("K matrix", :317-341)
 // run as BF16-split products on the gfx950 matrix cores: every FP32 operand
-// x is split into hi = bf16(x) and lo = bf16(x - hi), and a*b is formed as
-// hi_a*hi_b + hi_a*lo_b + lo_a*hi_b (the lo*lo term, ~2^-18 relative, is
-// dropped) with v_mfma_f32_32x32x16_bf16 and FP32 accumulation.  The split
-// keeps ~16 significant bits per operand at three BF16 MFMAs per product,
-// against one FP32 MFMA at a sixteenth of the BF16 rate.
+// x is split into NP BF16 parts, p_0 = bf16(x), p_1 = bf16(x - p_0), ..., and
+// a*b is formed from the part products p_a * q_b with a + b < NP on
+// v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  NP = 2 ("BF16x3"): three
+// products, ~16 significant bits per operand (the dropped lo*lo term is
+// ~2^-18 relative).  NP = 3 ("BF16x6"): six products; the three parts hold an
+// FP32 value exactly and the dropped terms are ~2^-24 relative, so the
+// products are as accurate as FP32 ones, at 2.5 PF/s / 6 against the 157
+// TF/s of the FP32 matrix instructions.
 //
 // Tile engine: WM x WN wavefronts (up to 16), each owning a 64x64 output
 // sub-tile as 2x2 MFMA blocks of 32x32; the workgroup tile is (64*WM) x
@@ -37,24 +40,34 @@ template <int BK> struct Stage {
     static constexpr int KV = BK / 8;
     static constexpr int KVP = KV + 1;
 };
-constexpr int SBK = 64;   // SYRK stage depth (4 wavefronts have the registers for it)
+// SYRK stage depth: 64 for two parts (4 wavefronts have the registers for it); 32 for three,
+// which keeps the stage's LDS (6 arrays) small enough for two workgroups per CU
+template <int NP> constexpr int syrk_bk() { return NP == 2 ? 64 : 32; }
 constexpr int XBK = 32;   // X-build stage depth (12-16 wavefronts per workgroup)
 constexpr int XNBUF = 1;  // X-build LDS stages (two measured no faster: the loads, not the split, set the pace)
 
-__device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
+// x = p[0] + p[1] + ... (+ the remainder below the last part), each part the bf16 rounding
+// (round to nearest even) of what the earlier parts leave; the subtractions are exact in FP32
+template <int NP>
+__device__ __forceinline__ void splitN(const float (&x)[8], v8bf (&p)[NP])
 {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16) x[e];
-        hi[e] = h;
-        lo[e] = (__bf16) (x[e] - (float) h);
+        float r = x[e];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const __bf16 h = (__bf16) r;
+            p[q][e] = h;
+            r -= (float) h;
+        }
     }
 }
 
 // acc[i][j] += A(64 rows of this wave) * B(64 cols of this wave) over k in [kbeg, kend).
 // Loader contract: coords(idx, row, kv) names the tile row and 8-group of the
 // idx-th (row, kv) slot (idx < tile rows * KV); load(row, k8, kend, v) fills the
-// 8 values k8 .. k8+7 of that row (zero beyond kend or outside the matrix).
+// 8 values k8 .. k8+7 of that row (zero beyond kend or outside the matrix), or for loaders
+// whose data is stored pre-split (kPresplit) loadp(row, k8, kend, parts) its NP BF16 parts.
 // NBUF = 1: one LDS stage, split and MFMA phases separated by two barriers per stage.
 // NBUF = 2: two LDS stages; stage s + 1 is split into one buffer while the MFMAs of
 // stage s read the other, with one barrier per stage, so the split of some wavefronts
@@ -62,8 +75,10 @@ __device__ __forceinline__ void split8(const float (&x)[8], v8bf &hi, v8bf &lo)
 // mrows / ncols: rows and columns of the tile inside the matrix; with SKIP a wavefront whose
 // 64 x 64 part lies wholly outside issues no MFMAs (its accumulators stay zero and are never
 // stored).  The SYRK keeps SKIP off: the test costs it the registers of its second workgroup per CU.
-template <int WM, int WN, int BK, int NBUF, bool SKIP, class LA, class LB>
-__device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2],
+// Part products are issued smallest first: for NP = 2 lo*hi, hi*lo, hi*hi; for NP = 3 the
+// products of part-index sum 2, then 1, then 0.
+template <int NP, int WM, int WN, int BK, int NBUF, bool SKIP, class LA, class LB>
+__device__ __forceinline__ void bfs_tile(const LA &la, const LB &lb, int kbeg, int kend, v16f (&acc)[2][2],
                                          int mrows, int ncols, bool zero = true)
 {
     constexpr int KV = Stage<BK>::KV, KVP = Stage<BK>::KVP;
@@ -71,7 +86,7 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
     constexpr int BMt = 64 * WM, BNt = 64 * WN;
     constexpr int SA = BMt * KV, SB = BNt * KV;                    // (row, 8-group) slots per stage
     constexpr int GA = (SA + NT - 1) / NT, GB = (SB + NT - 1) / NT;
-    __shared__ v8bf Ah[NBUF][BMt][KVP], Al[NBUF][BMt][KVP], Bh[NBUF][BNt][KVP], Bl[NBUF][BNt][KVP];
+    __shared__ v8bf As[NBUF][NP][BMt][KVP], Bs[NBUF][NP][BNt][KVP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w / WN) * 64, wn = (w % WN) * 64;
     if (zero) {
@@ -97,21 +112,21 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
         inb[g] = idx < SB;
         lb.coords(inb[g] ? idx : 0, rowb[g], kvb[g]);
     }
-    // FP32 staging registers (split into hi / lo at the LDS store), or for loaders whose
-    // data is stored pre-split (kPresplit), the BF16 hi / lo groups themselves
+    // FP32 staging registers (split into parts at the LDS store), or for pre-split data the
+    // BF16 parts themselves
     float ra[GA][8], rb[GB][8];
-    v8bf rah[GA], ral[GA], rbh[GB], rbl[GB];
+    v8bf rap[GA][NP], rbp[GB][NP];
     auto load_stage = [&](int k0) {
 #pragma unroll
         for (int g = 0; g < GA; ++g)
             if (ina[g]) {
-                if constexpr (LA::kPresplit) la.load2(rowa[g], k0 + 8 * kva[g], kend, rah[g], ral[g]);
+                if constexpr (LA::kPresplit) la.loadp(rowa[g], k0 + 8 * kva[g], kend, rap[g]);
                 else la.load(rowa[g], k0 + 8 * kva[g], kend, ra[g]);
             }
 #pragma unroll
         for (int g = 0; g < GB; ++g)
             if (inb[g]) {
-                if constexpr (LB::kPresplit) lb.load2(rowb[g], k0 + 8 * kvb[g], kend, rbh[g], rbl[g]);
+                if constexpr (LB::kPresplit) lb.loadp(rowb[g], k0 + 8 * kvb[g], kend, rbp[g]);
                 else lb.load(rowb[g], k0 + 8 * kvb[g], kend, rb[g]);
             }
     };
@@ -119,22 +134,16 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
 #pragma unroll
         for (int g = 0; g < GA; ++g)
             if (ina[g]) {
-                if constexpr (LA::kPresplit) {
-                    Ah[buf][rowa[g]][kva[g]] = rah[g];
-                    Al[buf][rowa[g]][kva[g]] = ral[g];
-                } else {
-                    split8(ra[g], Ah[buf][rowa[g]][kva[g]], Al[buf][rowa[g]][kva[g]]);
-                }
+                if constexpr (!LA::kPresplit) splitN<NP>(ra[g], rap[g]);
+#pragma unroll
+                for (int q = 0; q < NP; ++q) As[buf][q][rowa[g]][kva[g]] = rap[g][q];
             }
 #pragma unroll
         for (int g = 0; g < GB; ++g)
             if (inb[g]) {
-                if constexpr (LB::kPresplit) {
-                    Bh[buf][rowb[g]][kvb[g]] = rbh[g];
-                    Bl[buf][rowb[g]][kvb[g]] = rbl[g];
-                } else {
-                    split8(rb[g], Bh[buf][rowb[g]][kvb[g]], Bl[buf][rowb[g]][kvb[g]]);
-                }
+                if constexpr (!LB::kPresplit) splitN<NP>(rb[g], rbp[g]);
+#pragma unroll
+                for (int q = 0; q < NP; ++q) Bs[buf][q][rowb[g]][kvb[g]] = rbp[g][q];
             }
     };
     // a wavefront with no row or no column inside the matrix skips its matrix-core work
@@ -144,22 +153,24 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
 #pragma unroll
         for (int s = 0; s < BK / 16; ++s) {
             const int kv = 2 * s + (lane >> 5), r = lane & 31;
-            v8bf ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ah[i] = Ah[buf][wm + 32 * i + r][kv];
-                al[i] = Al[buf][wm + 32 * i + r][kv];
-                bh[i] = Bh[buf][wn + 32 * i + r][kv];
-                bl[i] = Bl[buf][wn + 32 * i + r][kv];
-            }
+            v8bf a[2][NP], b[2][NP];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                for (int q = 0; q < NP; ++q) {
+                    a[i][q] = As[buf][q][wm + 32 * i + r][kv];
+                    b[i][q] = Bs[buf][q][wn + 32 * i + r][kv];
                 }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int sum = NP - 1; sum >= 0; --sum)
+#pragma unroll
+                        for (int pa = sum; pa >= 0; --pa)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][pa], b[j][sum - pa], acc[i][j],
+                                                                               0, 0, 0);
         }
     };
     if (kbeg >= kend) return;
@@ -191,7 +202,7 @@ __device__ __forceinline__ void bf3_tile(const LA &la, const LB &lb, int kbeg, i
 }
 
 template <int WN, class F>
-__device__ __forceinline__ void bf3_epilogue(const v16f (&acc)[2][2], F &&store)
+__device__ __forceinline__ void bfs_epilogue(const v16f (&acc)[2][2], F &&store)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w / WN) * 64, wn = (w % WN) * 64;
@@ -248,15 +259,16 @@ struct XLoadB {
 
 constexpr int XKCH = 256;    // gather indices staged in LDS per k chunk (one index latency per 8 stages)
 
-// X build: (64 WM occupied rows) x (64 WN aux columns) per workgroup of WM x WN wavefronts
-template <int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restrict__ xptr,
-                                                             const int *__restrict__ xup,
-                                                             const int *__restrict__ xpartner,
-                                                             const float *__restrict__ Cocc32,
-                                                             const float *__restrict__ B32, __bf16 *__restrict__ Xh,
-                                                             __bf16 *__restrict__ Xl, int n_occ, int naux_pad,
-                                                             int ncolt, int nrowt, long total)
+// X build: (64 WM occupied rows) x (64 WN aux columns) per workgroup of WM x WN wavefronts;
+// X stored pre-split, part q at Xp + q * xpart
+template <int NP, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf(const int *__restrict__ xptr,
+                                                            const int *__restrict__ xup,
+                                                            const int *__restrict__ xpartner,
+                                                            const float *__restrict__ Cocc32,
+                                                            const float *__restrict__ B32, __bf16 *__restrict__ Xp,
+                                                            size_t xpart, int n_occ, int naux_pad, int ncolt,
+                                                            int nrowt, long total)
 {
     // column tiles fastest: consecutive workgroups read neighbouring segments of the same B
     // rows (measured faster than grouping the row tiles of one panel on an XCD)
@@ -276,41 +288,48 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf3(const int *__restri
         __syncthreads();
         XLoadA<64 * WM> la{Cocc32, s_pa, kc, n_occ, m0};
         XLoadB<64 * WN> lb{B32, s_up, kc, naux_pad, n0};
-        bf3_tile<WM, WN, XBK, XNBUF, true>(la, lb, kc, kce, acc, n_occ - m0, naux_pad - n0, kc == 0);
+        bfs_tile<NP, WM, WN, XBK, XNBUF, true>(la, lb, kc, kce, acc, n_occ - m0, naux_pad - n0, kc == 0);
         if (kce >= klen) break;
     }
-    // X stored pre-split for the SYRK: hi = bf16(v), lo = bf16(v - hi), exactly what the
-    // SYRK's own split of the FP32 value would give
+    // X stored pre-split for the SYRK, exactly the parts the SYRK's own split of the FP32
+    // value would give
     const size_t xo = (size_t) i * n_occ * naux_pad;
-    bf3_epilogue<WN>(acc, [&](int r, int c, float v) {
+    bfs_epilogue<WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
         if (m < n_occ && n < naux_pad) {
-            const __bf16 h = (__bf16) v;
-            Xh[xo + (size_t) m * naux_pad + n] = h;
-            Xl[xo + (size_t) m * naux_pad + n] = (__bf16) (v - (float) h);
+            float rem = v;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const __bf16 h = (__bf16) rem;
+                Xp[(size_t) q * xpart + xo + (size_t) m * naux_pad + n] = h;
+                rem -= (float) h;
+            }
         }
     });
 }
 
-// ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous, X pre-split into BF16
-// hi / lo arrays by the X build: a stage moves 16-byte groups straight into LDS
-template <int BK>
+// ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous, X pre-split into NP BF16
+// arrays by the X build: a stage moves 16-byte groups straight into LDS.  Rows are ld apart
+// (ld = kdim for the SYRK's X)
+template <int NP, int BK>
 struct SLoad {
     static constexpr bool kPresplit = true;
-    const __bf16 *xh, *xl; long kdim; int nbf, r0;
+    const __bf16 *xp; size_t xpart; long ld; int nbf, r0;
     __device__ void coords(int idx, int &row, int &kv) const
     {
         kv = idx % Stage<BK>::KV;
         row = idx / Stage<BK>::KV;
     }
-    __device__ void load2(int row, int k8, int kend, v8bf &h, v8bf &l) const {
+    __device__ void loadp(int row, int k8, int kend, v8bf (&p)[NP]) const {
         const int r = r0 + row;
         if (r < nbf && k8 < kend) {
-            h = *(const v8bf *) (xh + (size_t) r * kdim + k8);
-            l = *(const v8bf *) (xl + (size_t) r * kdim + k8);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) p[q] = *(const v8bf *) (xp + (size_t) q * xpart + (size_t) r * ld + k8);
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = l[e] = (__bf16) 0.f;
+            for (int q = 0; q < NP; ++q)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) p[q][e] = (__bf16) 0.f;
         }
     }
 };
@@ -318,9 +337,10 @@ struct SLoad {
 constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
 
 // upper 128x128 tiles of X X^T over k chunks; part[z] is FP32 per chunk
-__global__ __launch_bounds__(NTHR) void k_syrk_bf3(const __bf16 *__restrict__ Xh, const __bf16 *__restrict__ Xl,
-                                                   int nbf, long kdim, int ntb, int ntiles, long kchunk, long total,
-                                                   float *__restrict__ part)
+template <int NP>
+__global__ __launch_bounds__(NTHR) void k_syrk_bf(const __bf16 *__restrict__ Xp, size_t xpart, int nbf, long kdim,
+                                                  int ntb, int ntiles, long kchunk, long total,
+                                                  float *__restrict__ part)
 {
     const long t = xcd_remap(blockIdx.x, total);
     int tile = (int) (t % ntiles);
@@ -329,20 +349,21 @@ __global__ __launch_bounds__(NTHR) void k_syrk_bf3(const __bf16 *__restrict__ Xh
     while (tile >= ntb - bi) { tile -= ntb - bi; ++bi; }
     const int bj = bi + tile;
     const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
-    SLoad<SBK> la{Xh + kb, Xl + kb, kdim, nbf, bi * ST};
-    SLoad<SBK> lb{Xh + kb, Xl + kb, kdim, nbf, bj * ST};
+    constexpr int BK = syrk_bk<NP>();
+    SLoad<NP, BK> la{Xp + kb, xpart, kdim, nbf, bi * ST};
+    SLoad<NP, BK> lb{Xp + kb, xpart, kdim, nbf, bj * ST};   // X rows are kdim apart
     const int m0 = bi * ST, n0 = bj * ST;
     v16f acc[2][2];
-    bf3_tile<2, 2, SBK, 1, false>(la, lb, 0, (int) (ke - kb), acc, nbf - m0, nbf - n0);
+    bfs_tile<NP, 2, 2, BK, 1, false>(la, lb, 0, (int) (ke - kb), acc, nbf - m0, nbf - n0);
     float *P = part + (size_t) z * nbf * nbf;
-    bf3_epilogue<2>(acc, [&](int r, int c, float v) {
+    bfs_epilogue<2>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
         if (m < nbf && n < nbf) P[(size_t) m * nbf + n] = v;
     });
 }
 
 // K[m][n] = K[n][m] = sum_z part[z][m][n] (in FP64) for m <= n inside upper tiles
-__global__ void k_syrk_bf3_reduce(const float *__restrict__ part, int nsplit, int nbf, double *__restrict__ K)
+__global__ void k_syrk_bf_reduce(const float *__restrict__ part, int nsplit, int nbf, double *__restrict__ K)
 {
     const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long) nbf * nbf) return;
@@ -357,6 +378,49 @@ __global__ void k_syrk_bf3_reduce(const float *__restrict__ part, int nsplit, in
     if (z < nsplit) s0 += (double) part[(size_t) z * nbf * nbf + idx];
     K[(size_t) m * nbf + n] = s0 + s1;
     K[(size_t) n * nbf + m] = s0 + s1;
+}
+
+// ---- difference-density exchange: G = A B^T over all 128x128 tiles of the n x n result, A and
+// B rows of pre-split BF16 matrices (rows ld apart, parts xpart apart), k chunks of FP32
+// partials in part[z]
+template <int NP>
+__global__ __launch_bounds__(NTHR) void k_gemm_bf_nt(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
+                                                     size_t xpart, long ld, int n, long kdim, int ntb, long kchunk,
+                                                     long total, float *__restrict__ part)
+{
+    const long t = xcd_remap(blockIdx.x, total);
+    const int ntiles = ntb * ntb;
+    const int tile = (int) (t % ntiles);
+    const long z = t / ntiles;
+    const int bi = tile / ntb, bj = tile % ntb;
+    const long kb = z * kchunk, ke = std::min<long>(kdim, kb + kchunk);
+    constexpr int BK = syrk_bk<NP>();
+    SLoad<NP, BK> la{A + kb, xpart, ld, n, bi * ST};
+    SLoad<NP, BK> lb{B + kb, xpart, ld, n, bj * ST};
+    const int m0 = bi * ST, n0 = bj * ST;
+    v16f acc[2][2];
+    bfs_tile<NP, 2, 2, BK, 1, false>(la, lb, 0, (int) (ke - kb), acc, n - m0, n - n0);
+    float *P = part + (size_t) z * n * n;
+    bfs_epilogue<2>(acc, [&](int r, int c, float v) {
+        const int m = m0 + r, q = n0 + c;
+        if (m < n && q < n) P[(size_t) m * n + q] = v;
+    });
+}
+
+// K = Kref + (G + G^T) / 2 with G = sum_z part[z] (FP64 sums), one thread per upper element
+__global__ void k_incr_reduce(const float *__restrict__ part, int nsplit, int n, const double *__restrict__ Kref,
+                              double *__restrict__ K)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long) n * n) return;
+    const int m = (int) (idx / n), q = (int) (idx % n);
+    if (m > q) return;
+    const size_t a = (size_t) m * n + q, b = (size_t) q * n + m;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += (double) part[(size_t) z * n * n + a] + (double) part[(size_t) z * n * n + b];
+    const double v = Kref[a] + 0.5 * s;
+    K[a] = v;
+    K[b] = v;
 }
 
 __global__ void k_to_f32(const double *__restrict__ in, float *__restrict__ out, long n)
@@ -377,58 +441,117 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n)
     return YGPU_OK;
 }
 
-template <int WM, int WN>
-static int launch_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                             const int *xpartner, const float *Cocc32, const float *B32, void *X)
+template <int NP, int WM, int WN>
+static int launch_xbuild_bf(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                            const int *xpartner, const float *Cocc32, const float *B32, void *X)
 {
     const int ncolt = (naux_pad + 64 * WN - 1) / (64 * WN), nrowt = (n_occ + 64 * WM - 1) / (64 * WM);
-    __bf16 *Xh = (__bf16 *) X, *Xl = Xh + (size_t) nbf * n_occ * naux_pad;
+    const size_t xpart = (size_t) nbf * n_occ * naux_pad;
     const long total = (long) ncolt * nrowt * nbf;
     if (total > 0x7fffffffL) {
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL((k_xbuild_bf3<WM, WN>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, xup, xpartner,
-                       Cocc32, B32, Xh, Xl, n_occ, naux_pad, ncolt, nrowt, total);
+    hipLaunchKernelGGL((k_xbuild_bf<NP, WM, WN>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, xup,
+                       xpartner, Cocc32, B32, (__bf16 *) X, xpart, n_occ, naux_pad, ncolt, nrowt, total);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
 
-int yg_xbuild_bf3(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
-                  const int *xpartner, const float *Cocc32, const float *B32, void *X)
+template <int NP>
+static int xbuild_np(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                     const int *xpartner, const float *Cocc32, const float *B32, void *X)
 {
-    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
     // all occupied rows in one workgroup up to 256 (row tiles of 256 beyond)
-    if (n_occ <= 64) return launch_xbuild_bf3<1, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
-    if (n_occ <= 128) return launch_xbuild_bf3<2, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
-    if (n_occ <= 192) return launch_xbuild_bf3<3, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (n_occ <= 64) return launch_xbuild_bf<NP, 1, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (n_occ <= 128) return launch_xbuild_bf<NP, 2, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (n_occ <= 192) return launch_xbuild_bf<NP, 3, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
     // (a 320 x 128 tile for 256 < n_occ <= 320 wastes no rows but measured slower than two 256-row tiles)
-    return launch_xbuild_bf3<4, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    return launch_xbuild_bf<NP, 4, 4>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
 }
 
-int yg_syrk_bf3(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part)
+int yg_xbuild_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
+                 const int *xpartner, const float *Cocc32, const float *B32, void *X)
 {
-    if (nbf <= 0) return YGPU_OK;
-    if (kdim % 8) {
-        ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
-        return YGPU_ERR_INVALID;
-    }
+    if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
+    if (nparts == 2) return xbuild_np<2>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    if (nparts == 3) return xbuild_np<3>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
+    ygpu_set_error(YGPU_ERR_INVALID, "X build: %d BF16 parts (2 or 3)", nparts);
+    return YGPU_ERR_INVALID;
+}
+
+template <int NP>
+static int syrk_np(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part)
+{
     const int ntb = (nbf + ST - 1) / ST;
     const int ntiles = ntb * (ntb + 1) / 2;
     // chunks of 1024..8192 along k (FP32 accumulation inside a chunk, FP64 across
     // chunks), enough of them for about 1024 workgroups
     long kchunk = kdim * ntiles / 1024;
     kchunk = std::max<long>(1024, std::min<long>(8192, kchunk));
-    kchunk = (kchunk + SBK - 1) / SBK * SBK;
+    constexpr int BK = syrk_bk<NP>();
+    kchunk = (kchunk + BK - 1) / BK * BK;
     const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
     YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
     const long total = (long) ntiles * nsplit;
-    const __bf16 *Xh = (const __bf16 *) X, *Xl = Xh + (size_t) nbf * kdim;
-    hipLaunchKernelGGL(k_syrk_bf3, dim3((unsigned) total), dim3(NTHR), 0, s, Xh, Xl, nbf, kdim, ntb, ntiles, kchunk,
-                       total, part.as<float>());
+    const size_t xpart = (size_t) nbf * kdim;
+    hipLaunchKernelGGL(k_syrk_bf<NP>, dim3((unsigned) total), dim3(NTHR), 0, s, (const __bf16 *) X, xpart, nbf, kdim,
+                       ntb, ntiles, kchunk, total, part.as<float>());
     YG_HIP(hipGetLastError());
     const long tot = (long) nbf * nbf;
-    hipLaunchKernelGGL(k_syrk_bf3_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, K);
+    hipLaunchKernelGGL(k_syrk_bf_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, K);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_syrk_bf(hipStream_t s, int nparts, int nbf, long kdim, const void *X, double *K, DBuf &part)
+{
+    if (nbf <= 0) return YGPU_OK;
+    if (kdim % 8) {
+        ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
+        return YGPU_ERR_INVALID;
+    }
+    if (nparts == 2) return syrk_np<2>(s, nbf, kdim, X, K, part);
+    if (nparts == 3) return syrk_np<3>(s, nbf, kdim, X, K, part);
+    ygpu_set_error(YGPU_ERR_INVALID, "SYRK: %d BF16 parts (2 or 3)", nparts);
+    return YGPU_ERR_INVALID;
+}
+
+int yg_incr_k_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, const void *X, const double *Kref,
+                 double *K, DBuf &part)
+{
+    if (nbf <= 0) return YGPU_OK;
+    const long kdim = (long) n_occ * naux_pad, ld = 2 * kdim;
+    if (kdim % 8) {
+        ygpu_set_error(YGPU_ERR_INVALID, "difference K: k extent %ld is not a multiple of 8", kdim);
+        return YGPU_ERR_INVALID;
+    }
+    const int ntb = (nbf + ST - 1) / ST;
+    const int ntiles = ntb * ntb;
+    long kchunk = kdim * ntiles / 1024;
+    kchunk = std::max<long>(1024, std::min<long>(8192, kchunk));
+    const int bk = nparts == 2 ? syrk_bk<2>() : syrk_bk<3>();
+    kchunk = (kchunk + bk - 1) / bk * bk;
+    const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
+    YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
+    const long total = (long) ntiles * nsplit;
+    // X of the combined [P | M] block: rows 0 .. n_occ-1 of basis function i hold X^P_i, rows
+    // n_occ .. 2 n_occ-1 X^M_i, so X^M_i starts kdim after X^P_i and rows are 2 kdim apart
+    const size_t xpart = (size_t) nbf * ld;
+    const __bf16 *XP = (const __bf16 *) X, *XM = XP + kdim;
+    if (nparts == 2)
+        hipLaunchKernelGGL(k_gemm_bf_nt<2>, dim3((unsigned) total), dim3(NTHR), 0, s, XM, XP, xpart, ld, nbf, kdim, ntb,
+                           kchunk, total, part.as<float>());
+    else if (nparts == 3)
+        hipLaunchKernelGGL(k_gemm_bf_nt<3>, dim3((unsigned) total), dim3(NTHR), 0, s, XM, XP, xpart, ld, nbf, kdim, ntb,
+                           kchunk, total, part.as<float>());
+    else {
+        ygpu_set_error(YGPU_ERR_INVALID, "difference K: %d BF16 parts (2 or 3)", nparts);
+        return YGPU_ERR_INVALID;
+    }
+    YG_HIP(hipGetLastError());
+    const long tot = (long) nbf * nbf;
+    hipLaunchKernelGGL(k_incr_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, Kref, K);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

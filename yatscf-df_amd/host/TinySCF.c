@@ -235,6 +235,26 @@ static void TinySCF_calc_energy(TinySCF_t T)
     T->HF_energy = e;
 }
 
+/* the DIIS history as TinySCF_init_free.c:173-180 leaves it */
+static void diis_restart(TinySCF_t T)
+{
+    T->DIIS_len = 0;
+    T->DIIS_bmax_id = 0;
+    T->DIIS_bmax = -DBL_MAX;
+    /* B: -1 off the diagonal (the constraint row and column), 0 on it */
+    for (int i = 0; i < (MAX_DIIS + 1) * (MAX_DIIS + 1); i++) T->B_mat[i] = -1.0;
+    for (int i = 0; i < MAX_DIIS + 1; i++) T->B_mat[i * (MAX_DIIS + 1) + i] = 0.0;
+}
+
+/* the reduced-precision phase of a mixed run after `phase` (FP64 when none is left):
+ * BF16x3 products, then BF16x6 products, as mixed_schedule selects */
+static int next_reduced_phase(const TinySCF_t T, int phase)
+{
+    if (phase == YGPU_FOCK_FP64 && (T->mixed_schedule & 1)) return YGPU_FOCK_MIXED;
+    if ((phase == YGPU_FOCK_FP64 || phase == YGPU_FOCK_MIXED) && (T->mixed_schedule & 2)) return YGPU_FOCK_MIXED6;
+    return YGPU_FOCK_FP64;
+}
+
 void TinySCF_do_SCF(TinySCF_t T)
 {
     if (T->verbose) {
@@ -247,16 +267,29 @@ void TinySCF_do_SCF(TinySCF_t T)
     YS_GPU(ygpu_clear_timings(g));
     T->converged = 0;
     T->diis_error = -1.0;
-    T->mixed_phase = T->mixed;
-    T->mixed_switch_iter = -1;
-    double prev = 0.0, delta = 223.0, best_err = -1.0;
-    int stall = 0;
+    /* Precision phases of a mixed run (mixed_schedule): reduced-precision Fock builds (BF16x3
+     * products, then BF16x6) while the DIIS error is large; then FP64: with bit 2 set, one full
+     * FP64 build becomes the reference of difference-density builds (K = K_ref + K(D - D_ref),
+     * YGPU_FOCK_INCR) until the energy converges, and a last full FP64 build confirms it;
+     * without it, full FP64 builds to convergence.  Energies of reduced-precision builds never
+     * end the SCF. */
+    const int use_incr = T->mixed && (T->mixed_schedule & 4);
+    T->mixed_phase = T->mixed ? next_reduced_phase(T, YGPU_FOCK_FP64) : YGPU_FOCK_FP64;
+    T->mixed_switch_iter = (T->mixed && T->mixed_phase == YGPU_FOCK_FP64) ? 0 : -1;
+    T->mixed6_switch_iter = T->mixed_phase == YGPU_FOCK_MIXED6 ? 0 : -1;
+    double prev = 0.0, delta = 223.0, best_err = -1.0, diis_at_ref = DBL_MAX;
+    int prev_exact = 0, stall = 0;
     while (T->iter < T->niters && delta >= T->ene_tol) {
         double st0 = get_wtime_sec();
         if (T->verbose) printf("--------------- Iteration %d ---------------\n", T->iter);
-        if (T->mixed)
-            YS_GPU(ygpu_set_fock_precision(g, T->mixed_phase ? YGPU_FOCK_MIXED : YGPU_FOCK_FP64));
+        const int phase = T->mixed_phase;
+        if (T->mixed) YS_GPU(ygpu_set_fock_precision(g, phase));
         TinySCF_build_FockMat(T);
+        if (T->iter < YS_MAX_PHASE_LOG) {
+            int used = phase;
+            if (T->mixed) YS_GPU(ygpu_fock_last_info(g, &used, NULL));   /* a difference build may fall back */
+            T->phase_log[T->iter] = used;
+        }
         TinySCF_calc_energy(T);
         delta = fabs(T->HF_energy - prev);
         prev = T->HF_energy;
@@ -271,52 +304,71 @@ void TinySCF_do_SCF(TinySCF_t T)
         T->t_diis = st2 - st1;
         T->t_density = st3 - st2;
         if (T->verbose) {
+            static const char *tag[] = {"", " [BF16x3]", " [BF16x6]", " [FP64 + difference]"};
             printf("* DIIS procedure        : %.3lf (s)\n", st2 - st1);
             printf("* Build density matrix  : %.3lf (s)\n", st3 - st2);
             printf("* Iteration runtime     = %.3lf (s)\n", st3 - st0);
             printf("* Energy = %.10lf (%.10lf)", T->HF_energy + T->nuc_energy, T->HF_energy);
             if (T->iter > 0) printf(", delta = %e", delta);
             if (T->iter >= 2) printf(", DIIS error = %.3e", T->diis_error);
-            printf("%s\n", T->mixed_phase ? " [mixed]" : "");
+            printf("%s\n", tag[phase]);
         }
-        if (T->iter == 0) delta = 223.0;   /* iteration 0 never converges (TinySCF.c:429) */
-        if (T->mixed_phase) {
-            /* a reduced-precision energy never ends the SCF; the phase ends once
-             * the DIIS error is below mixed_tol or the reduced-precision floor */
-            delta = 223.0;
+        /* an energy from full or difference FP64 builds is compared only with another one */
+        const int exact = phase == YGPU_FOCK_FP64 || phase == YGPU_FOCK_INCR;
+        if (T->iter == 0 || !exact || !prev_exact) delta = 223.0;   /* iteration 0 never converges (TinySCF.c:429) */
+        prev_exact = exact;
+        if (phase == YGPU_FOCK_MIXED || phase == YGPU_FOCK_MIXED6) {
+            /* a phase ends once the DIIS error is below mixed_tol or the phase's reduced-precision
+             * floor, or stops falling */
             if (T->iter >= 2 && T->diis_error >= 0.0) {
-                /* or when the error has stopped falling (the reduced-precision noise
-                 * floor lies above mixed_tol): no new best by 30 % in 3 iterations */
+                /* no new best by 30 % in 3 iterations: the floor lies above the estimate */
                 if (best_err < 0.0 || T->diis_error < 0.7 * best_err) {
                     best_err = T->diis_error;
                     stall = 0;
                 } else {
                     stall++;
                 }
-                /* the reduced-precision floor: BF16-split products carry a relative error of
-                 * about 2^-16, so |dK_ij| <= 2^-16 sum_k |X_ik||X_jk| <= 2^-16 max_i K_ii
-                 * (K_ii = sum_k X_ik^2); the DIIS error cannot fall much below 1.5 times that
-                 * (measured floors 0.4-1.8x on C10H22, C54H18, C96H24) */
+                /* the reduced-precision floor: products with a relative error e give
+                 * |dK_ij| <= e sum_k |X_ik||X_jk| <= e max_i K_ii (K_ii = sum_k X_ik^2), and the
+                 * DIIS error cannot fall much below that.  BF16x3: e ~ 2^-16 (representation; measured
+                 * floors 0.4-1.8 x 1.5 2^-16 max K_ii on C10H22, C54H18, C96H24).  BF16x6: the
+                 * products are FP32-accurate and the FP32 accumulation sets e */
                 double kdiag = 0.0;
                 for (int i = 0; i < T->nbasfuncs; i++)
                     kdiag = fmax(kdiag, fabs(T->K_mat[(size_t) i * T->nbasfuncs + i]));
-                const double floor_est = 1.5 * ldexp(1.0, -16) * kdiag;
+                const double floor_est = (phase == YGPU_FOCK_MIXED ? 1.5 * ldexp(1.0, -16) : T->mixed6_floor) * kdiag;
                 if (T->diis_error < fmax(T->mixed_tol, floor_est) || stall >= 3) {
-                    T->mixed_phase = 0;
-                    T->mixed_switch_iter = T->iter + 1;
-                    /* the stored Fock matrices carry the reduced-precision error: the FP64
-                     * phase restarts the DIIS history from its own Fock matrices */
-                    T->DIIS_len = 0;
-                    T->DIIS_bmax_id = 0;
-                    T->DIIS_bmax = -DBL_MAX;
-                    /* B as initialised (TinySCF_init_free.c:178-182): -1 off the diagonal
-                     * (the constraint row and column), 0 on it */
-                    for (int i = 0; i < (MAX_DIIS + 1) * (MAX_DIIS + 1); i++) T->B_mat[i] = -1.0;
-                    for (int i = 0; i < MAX_DIIS + 1; i++) T->B_mat[i * (MAX_DIIS + 1) + i] = 0.0;
+                    const int next = next_reduced_phase(T, phase);
+                    const int keep = (next == YGPU_FOCK_FP64) ? (T->mixed_keep_diis & 2) : (T->mixed_keep_diis & 1);
+                    /* the stored Fock matrices carry the reduced-precision error: unless kept, the
+                     * next phase restarts the DIIS history from its own Fock matrices */
+                    if (!keep) diis_restart(T);
+                    if (next == YGPU_FOCK_FP64) T->mixed_switch_iter = T->iter + 1;
+                    else T->mixed6_switch_iter = T->iter + 1;
+                    T->mixed_phase = next;
+                    best_err = -1.0;
+                    stall = 0;
                 }
             }
-        } else if (T->iter == T->mixed_switch_iter) {
-            delta = 223.0;   /* the first FP64 energy is compared with a mixed one */
+        } else if (use_incr) {
+            /* FP64 and difference builds: a full FP64 build is the reference of the difference
+             * builds that follow.  The rounding error of a difference build scales with the
+             * distance of its density from the reference's, and with it the noise of its energy:
+             * once the DIIS error has fallen below mixed_reref times its value at the reference,
+             * the next build is a full FP64 one, the new reference.  Convergence among
+             * difference builds is confirmed by a full FP64 build, which ends the SCF if its
+             * energy agrees within ene_tol */
+            if (phase == YGPU_FOCK_FP64) {
+                /* a reference build, or a confirming one whose energy disagreed: difference
+                 * builds follow (an agreeing confirmation ends the loop) */
+                diis_at_ref = T->diis_error >= 0.0 ? T->diis_error : DBL_MAX;
+                if (delta >= T->ene_tol) T->mixed_phase = YGPU_FOCK_INCR;
+            } else if (delta < T->ene_tol) {
+                T->mixed_phase = YGPU_FOCK_FP64;
+                delta = 223.0;
+            } else if (T->diis_error >= 0.0 && T->diis_error < T->mixed_reref * diis_at_ref) {
+                T->mixed_phase = YGPU_FOCK_FP64;
+            }
         }
         T->iter++;
     }

@@ -110,9 +110,25 @@ struct TinySCF_struct
                                over the stored residuals each iteration */
     int    host_df_copy;    /* keep host pqA / df_tensor copies, default 0 */
     int    mixed;           /* 1: reduced-precision Fock builds until the DIIS error < mixed_tol */
+    int    mixed_schedule;  /* phases of a mixed run: bit 0 BF16x3 products, bit 1 BF16x6
+                               products (FP32-accurate), in that order, then FP64; bit 2: the
+                               FP64 phase runs difference-density builds against a full FP64
+                               reference build and confirms convergence with a full FP64
+                               build; default 5 */
+    int    mixed_keep_diis; /* keep the DIIS history across a phase change (bit 0: BF16x3 ->
+                               BF16x6, bit 1: -> FP64) instead of restarting it; default 0 */
     double mixed_tol;       /* DIIS error (max |R|) that ends the mixed phase, default 1e-5 */
-    int    mixed_phase;     /* 1 while the Fock builds run in reduced precision */
+    double mixed_reref;     /* difference-density phase: a new FP64 reference build once the
+                               DIIS error is below this fraction of its value at the last
+                               reference, default 0.02 */
+    double mixed6_floor;    /* reduced-precision floor of the BF16x6 phase in units of
+                               max_i K_ii (see TinySCF_do_SCF), default 1.5 * 2^-21 */
+    int    mixed_phase;     /* Fock arithmetic of the current iteration: YGPU_FOCK_FP64,
+                               YGPU_FOCK_MIXED or YGPU_FOCK_MIXED6 */
     int    mixed_switch_iter; /* first FP64 iteration of a mixed run (-1: never switched) */
+    int    mixed6_switch_iter; /* first BF16x6 iteration of a mixed run (-1: none) */
+#define YS_MAX_PHASE_LOG 512
+    int    phase_log[YS_MAX_PHASE_LOG]; /* Fock arithmetic (YGPU_FOCK_*) of each SCF iteration */
     double diis_error;      /* max |R| of the newest DIIS residual (iterations >= 2) */
     char   guess_weights[512]; /* learned-guess network file ("" = SAD guess only) */
     int    guess_ml;        /* 1 if the last initial guess used the learned correction */

@@ -62,8 +62,21 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
         const char *e = getenv("YATSCF_MIXED_TOL");
         if (e && *e && atof(e) > 0.0) T->mixed_tol = atof(e);
     }
+    T->mixed_schedule = env_int("YATSCF_MIXED_SCHEDULE", 5);
+    T->mixed_keep_diis = env_int("YATSCF_MIXED_KEEP_DIIS", 0);
+    T->mixed6_floor = 1.5 * 4.76837158203125e-07;     /* 1.5 * 2^-21 */
+    T->mixed_reref = 0.02;
+    {
+        const char *e = getenv("YATSCF_MIXED_REREF");
+        if (e && *e && atof(e) > 0.0) T->mixed_reref = atof(e);
+    }
+    {
+        const char *e = getenv("YATSCF_MIXED6_FLOOR");
+        if (e && *e && atof(e) > 0.0) T->mixed6_floor = atof(e);
+    }
     T->mixed_phase = 0;
     T->mixed_switch_iter = -1;
+    T->mixed6_switch_iter = -1;
     T->diis_error = -1.0;
     T->guess_ml = 0;
     T->guess_weights[0] = 0;

@@ -95,6 +95,7 @@ int *tinyscf_iptr(TinySCF_t T, const char *n, size_t *count)
 {
 #define RET(name, ptr, cnt) if (!strcmp(n, name)) { *count = (ptr) ? (cnt) : 0; return (ptr); }
     RET("bf_pair_mask", T->bf_pair_mask, (size_t) T->mat_size)
+    RET("phase_log", T->phase_log, (size_t) (T->iter < YS_MAX_PHASE_LOG ? T->iter : YS_MAX_PHASE_LOG))
     RET("bf_pair_j", T->bf_pair_j, (size_t) T->num_bf_pair_scr)
     RET("bf_pair_diag", T->bf_pair_diag, (size_t) T->nbasfuncs)
     RET("bf_mask_displs", T->bf_mask_displs, (size_t) T->nbasfuncs + 1)
@@ -127,6 +128,9 @@ int tinyscf_geti(TinySCF_t T, const char *n)
     if (!strcmp(n, "quirk_a5")) return T->quirk_a5;
     if (!strcmp(n, "mixed")) return T->mixed;
     if (!strcmp(n, "mixed_switch_iter")) return T->mixed_switch_iter;
+    if (!strcmp(n, "mixed6_switch_iter")) return T->mixed6_switch_iter;
+    if (!strcmp(n, "mixed_schedule")) return T->mixed_schedule;
+    if (!strcmp(n, "mixed_keep_diis")) return T->mixed_keep_diis;
     if (!strcmp(n, "guess_ml")) return T->guess_ml;
     if (!strcmp(n, "electron")) return T->electron;
     return -1;
@@ -141,6 +145,8 @@ void tinyscf_seti(TinySCF_t T, const char *n, int v)
     else if (!strcmp(n, "quirk_a2")) T->quirk_a2 = v;
     else if (!strcmp(n, "quirk_a5")) T->quirk_a5 = v;
     else if (!strcmp(n, "mixed")) T->mixed = v;
+    else if (!strcmp(n, "mixed_schedule")) T->mixed_schedule = v;
+    else if (!strcmp(n, "mixed_keep_diis")) T->mixed_keep_diis = v;
 }
 
 double tinyscf_getd(TinySCF_t T, const char *n)
@@ -162,6 +168,8 @@ void tinyscf_setd(TinySCF_t T, const char *n, double v)
 {
     if (!strcmp(n, "ene_tol")) T->ene_tol = v;
     else if (!strcmp(n, "mixed_tol")) T->mixed_tol = v;
+    else if (!strcmp(n, "mixed6_floor")) T->mixed6_floor = v;
+    else if (!strcmp(n, "mixed_reref")) T->mixed_reref = v;
 }
 
 void *tinyscf_gpu(TinySCF_t T) { return T->gpu; }

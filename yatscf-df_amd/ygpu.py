@@ -71,6 +71,7 @@ SIGNATURES = {
     "ygpu_diis_error": [vp, ctypes.c_int, dp],
     "ygpu_set_fock_precision": [vp, ctypes.c_int],
     "ygpu_set_metric_factor": [vp, ctypes.c_int],
+    "ygpu_fock_last_info": [vp, ip, dp],
     "ygpu_mlp_forward": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "ygpu_mlp_trainer_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -82,7 +83,7 @@ SIGNATURES = {
     "ygpu_mlp_trainer_destroy": [vp],
 }
 
-FOCK_FP64, FOCK_MIXED = 0, 1
+FOCK_FP64, FOCK_MIXED, FOCK_MIXED6, FOCK_INCR = 0, 1, 2, 3
 METRIC_CHOLESKY, METRIC_INVSQRT = 0, 1
 
 fp = ctypes.POINTER(ctypes.c_float)
@@ -272,6 +273,12 @@ class Engine:
         t = DfTiming()
         check(self.L.ygpu_build_df_tensor_ex(self.h, _d(pqA), _d(Jpq), _d(B), ctypes.byref(t)))
         return dict(pqA=pqA, Jpq=Jpq, B=B, timing=t)
+
+    def fock_last_info(self):
+        """(arithmetic the last Fock build ran, max |Q^T Q - I| of a difference build's alignment)"""
+        m, e = ctypes.c_int(0), ctypes.c_double(0.0)
+        check(self.L.ygpu_fock_last_info(self.h, ctypes.byref(m), ctypes.byref(e)))
+        return m.value, e.value
 
     def set_metric_factor(self, mode):
         """METRIC_CHOLESKY (W = L^-1, the default) or METRIC_INVSQRT (W = J^-1/2, the reference's)."""
@@ -467,7 +474,7 @@ class TinySCF:
 
 
 def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, fock_check=False,
-          final_fock=False):
+          final_fock=False, schedule=None, keep_diis=None, reref=None, verbose=False):
     """One complete SCF of ``xyz`` through the TinySCF engine, timed from after input parsing
     (basis shells and nuclei uploaded) to the passing convergence test: S/H/X, Schwarz
     screening, pair layout, initial guess (SAD, plus the learned pair correction when
@@ -478,12 +485,18 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
     same density, the reduced-precision arithmetic's own error in F and in the orbital energies
     of X^T F X.  Returns a dict; the engine is closed."""
     import time
-    T = TinySCF(xyz, niters=niters)
+    T = TinySCF(xyz, niters=niters, verbose=verbose)
     try:
         T.L.tinyscf_set_guess_weights(T.T, (weights or "").encode())
         T.setd("ene_tol", ene_tol)
         T.seti("quirk_a2", int(quirk_a2))
         T.seti("mixed", int(mixed))
+        if schedule is not None:
+            T.seti("mixed_schedule", int(schedule))
+        if keep_diis is not None:
+            T.seti("mixed_keep_diis", int(keep_diis))
+        if reref is not None:
+            T.setd("mixed_reref", float(reref))
         t0 = time.perf_counter()
         T.setup()
         t1 = time.perf_counter()
@@ -496,7 +509,9 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
                    energy=T.getd("total_energy"), converged=bool(T.geti("converged")),
                    hist=T.d("t_hist").reshape(-1, 6).copy(), t_df=T.d("t_df").copy(), nbf=nbf,
                    naux=T.geti("df_nbf"), n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"),
-                   switch=T.geti("mixed_switch_iter"), guess_ml=T.geti("guess_ml"),
+                   switch=T.geti("mixed_switch_iter"), switch6=T.geti("mixed6_switch_iter"),
+                   phases=T.i("phase_log").copy(),
+                   guess_ml=T.geti("guess_ml"),
                    D=T.mat("D_mat").copy(), S=T.mat("S_mat").copy(),
                    X=T.mat("X_mat").copy(), H=T.mat("Hcore_mat").copy(), eigval=T.d("eigval").copy())
         mask = T.i("bf_pair_mask").reshape(nbf, nbf)
