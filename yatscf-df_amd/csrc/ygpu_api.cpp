@@ -556,6 +556,18 @@ try {
         c->max_row_len = std::max(c->max_row_len, (int) xup.size() - xptr[i]);
     }
     xptr[nbf] = (int) xup.size();
+    std::vector<int> kofs(nbf + 1);
+    long ks = 0;
+    for (int i = 0; i < nbf; ++i) {
+        kofs[i] = (int) ks;
+        ks += (xptr[i + 1] - xptr[i] + 7) / 8 * 8;
+    }
+    kofs[nbf] = (int) ks;
+    c->ksum = ks;
+    std::vector<int> xorder(nbf);
+    for (int i = 0; i < nbf; ++i) xorder[i] = i;
+    std::stable_sort(xorder.begin(), xorder.end(),
+                     [&](int a, int b) { return xptr[a + 1] - xptr[a] > xptr[b + 1] - xptr[b]; });
     c->num_uniq_sp = num_uniq_sp;
     c->h_uniq_lid.assign(lid, lid + num_uniq_sp);
     c->h_uniq_rid.assign(rid, rid + num_uniq_sp);
@@ -568,6 +580,8 @@ try {
     YG_TRY(upload(c->d_xptr, xptr.data(), xptr.size() * sizeof(int), c->stream));
     YG_TRY(upload(c->d_xup, xup.data(), std::max<size_t>(1, xup.size()) * sizeof(int), c->stream));
     YG_TRY(upload(c->d_xpartner, xpartner.data(), std::max<size_t>(1, xpartner.size()) * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_kofs, kofs.data(), kofs.size() * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_xorder, xorder.data(), xorder.size() * sizeof(int), c->stream));
     YG_TRY(upload(c->d_sp_scr, c->h_sp_scrval.data(), c->h_sp_scrval.size() * sizeof(double), c->stream));
     YG_TRY(upload(c->d_df_scr, c->h_df_sp_scrval.data(), c->h_df_sp_scrval.size() * sizeof(double), c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
@@ -617,6 +631,23 @@ static int join_side(ygpu_ctx *c, hipStream_t s)
     YG_HIP(hipEventRecord(c->join, s));
     YG_HIP(hipStreamWaitEvent(c->stream, c->join, 0));
     return YGPU_OK;
+}
+
+// X of the reduced-precision builds into c->d_X, nrow columns of C32 (nbf x nrow, FP32): two
+// parts from the k-contiguous blocks (C_occ's laid out here, B's in d_Bt), three parts by the
+// gathering X build
+static int xbuild_mixed(ygpu_ctx *c, int nparts, int nrow, const float *C32, size_t btpart)
+{
+    const int nbf = c->nbf;
+    if (nparts != 2)
+        return yg_xbuild_bf(c->stream, nparts, nbf, nrow, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
+                            c->d_xpartner.as<int>(), C32, c->d_B32.as<float>(), c->d_X.p);
+    const size_t cpart = (size_t) c->ksum * nrow;
+    YG_TRY(c->d_Cp.ensure(2 * cpart * sizeof(__bf16)));
+    YG_TRY(yg_gather_split_t(c->stream, nbf, c->max_row_len, C32, nrow, nrow, c->d_xpartner.as<int>(), c->d_xptr.as<int>(),
+                             c->d_kofs.as<int>(), c->d_Cp.p, cpart));
+    return yg_xbuild_ps(c->stream, nbf, nrow, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_kofs.as<int>(),
+                        c->d_xorder.as<int>(), c->d_Cp.p, cpart, c->d_Bt.p, btpart, c->d_X.p);
 }
 
 int ygpu_build_df_tensor_ex(ygpu_ctx_t *c, double *pqA_out, double *Jpq_out, double *df_out, ygpu_df_timing_t *t)
@@ -696,6 +727,7 @@ try {
     c->d_pqA.release();   // (mn|P) is not needed once B is formed
     c->have_df = 1;
     c->have_B32 = 0;
+    c->have_Bt = 0;
     c->kref_valid = 0;
     c->jinc_valid = 0;
     return YGPU_OK;
@@ -733,6 +765,7 @@ try {
     YG_HIP(hipStreamSynchronize(c->stream));
     c->have_df = 1;
     c->have_B32 = 0;
+    c->have_Bt = 0;
     c->kref_valid = 0;
     c->jinc_valid = 0;
     return YGPU_OK;
@@ -830,6 +863,15 @@ try {
             YG_TRY(yg_to_f32(c->stream, c->d_B.as<double>(), c->d_B32.as<float>(), c->n_up * c->naux_pad));
             c->have_B32 = 1;
         }
+        // two-part products: B's rows of every basis function transposed into k-contiguous
+        // blocks (both triangles, two BF16 parts) once per DF tensor
+        const size_t btpart = (size_t) c->ksum * c->naux_pad;
+        if (nparts == 2 && !c->have_Bt) {
+            YG_TRY(c->d_Bt.ensure(2 * btpart * sizeof(__bf16)));
+            YG_TRY(yg_gather_split_t(c->stream, nbf, c->max_row_len, c->d_B32.as<float>(), c->naux_pad, (int) c->naux_pad,
+                                     c->d_xup.as<int>(), c->d_xptr.as<int>(), c->d_kofs.as<int>(), c->d_Bt.p, btpart));
+            c->have_Bt = 1;
+        }
         YG_TRY(c->d_Dprev.ensure(ms));
         YG_TRY(c->d_Jprev.ensure(ms));
         YG_TRY(c->d_dD.ensure(ms));
@@ -860,18 +902,14 @@ try {
             // K = K_ref (FP64) + that difference
             YG_TRY(c->d_PM32.ensure((size_t) nbf * 2 * n_occ * sizeof(float)));
             YG_TRY(yg_pm32(c->stream, nbf, n_occ, c->d_C1.as<double>(), c->d_Cref.as<double>(), c->d_PM32.as<float>()));
-            YG_TRY(yg_xbuild_bf(c->stream, nparts, nbf, 2 * n_occ, (int) c->naux_pad, c->d_xptr.as<int>(),
-                                c->d_xup.as<int>(), c->d_xpartner.as<int>(), c->d_PM32.as<float>(),
-                                c->d_B32.as<float>(), c->d_X.as<float>()));
+            YG_TRY(xbuild_mixed(c, nparts, 2 * n_occ, c->d_PM32.as<float>(), btpart));
             YG_HIP(hipEventRecord(c->ev[4], c->stream));
             YG_TRY(yg_incr_k_bf(c->stream, nparts, nbf, n_occ, (int) c->naux_pad, c->d_X.as<float>(),
                                 c->d_Kref.as<double>(), c->d_K.as<double>(), c->d_Kpart32));
         } else {
             // K from BF16-split products; X stored pre-split in the X buffer
             YG_TRY(yg_to_f32(c->stream, c->d_Cocc.as<double>(), c->d_Cocc32.as<float>(), (long) nbf * n_occ));
-            YG_TRY(yg_xbuild_bf(c->stream, nparts, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(),
-                                c->d_xup.as<int>(), c->d_xpartner.as<int>(), c->d_Cocc32.as<float>(),
-                                c->d_B32.as<float>(), c->d_X.as<float>()));
+            YG_TRY(xbuild_mixed(c, nparts, n_occ, c->d_Cocc32.as<float>(), btpart));
             YG_HIP(hipEventRecord(c->ev[4], c->stream));
             YG_TRY(yg_syrk_bf(c->stream, nparts, nbf, (long) n_occ * c->naux_pad, c->d_X.as<float>(),
                               c->d_K.as<double>(), c->d_Kpart32));

@@ -86,6 +86,12 @@ struct ygpu_ctx {
     DBuf d_upmask, d_up_i, d_up_j;
     DBuf d_xptr, d_xup, d_xpartner;  // per bf i: up rows and partners (full CSR order)
     int max_row_len = 0;
+    // k-contiguous operand blocks of the two-part BF16 X build: block i holds L_i = (row length
+    // of i rounded up to 8) values per row and starts kofs[i] * (rows) elements into its part
+    DBuf d_kofs;
+    DBuf d_xorder;                // basis functions by decreasing row length (X build order)
+    long ksum = 0;                // sum of L_i
+
     // screened 3c work list: per (la,lb) class, list of (lid, rid) shell pairs
     std::vector<int> h_sp_lid, h_sp_rid;
     DBuf d_sp_scr, d_df_scr;
@@ -113,6 +119,8 @@ struct ygpu_ctx {
     // mixed-precision Fock builds (ygpu_set_fock_precision)
     int fock_mode = YGPU_FOCK_FP64;
     int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
+    int have_Bt = 0;              // d_Bt holds the transposed two-part BF16 blocks of d_B32
+    DBuf d_Bt, d_Cp;              // B and C_occ operand blocks of the two-part X build
     int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
     // difference-density exchange (YGPU_FOCK_INCR): the C_occ and FP64 K of the last FP64 build,
@@ -219,6 +227,14 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
 // X (nbf x n_occ naux_pad) is stored pre-split into nparts (2 or 3) BF16 arrays, one after the
 // other (part q of element e at q * nbf * n_occ * naux_pad + e); the X build writes it, the
 // SYRK reads it
+// dst[q][kofs_i R + r L_i + kk] = BF16 part q (of 2) of src[idx[xptr_i + kk]][r], r < R, kk < L_i
+// (zero for kk >= row length): the gathered rows of every basis function i transposed
+int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
+                      const int *xptr, const int *kofs, void *dst, size_t dpart);
+// X (two BF16 parts, nrow rows per basis function) from the k-contiguous blocks Cp (nrow rows)
+// and Bt (naux_pad rows) made by yg_gather_split_t; order: the basis functions, longest row first
+int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs, const int *order,
+                 const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X);
 int yg_xbuild_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
                   const int *xpartner, const float *Cocc32, const float *B32, void *X);
 int yg_syrk_bf(hipStream_t s, int nparts, int nbf, long kdim, const void *X, double *K, DBuf &part);

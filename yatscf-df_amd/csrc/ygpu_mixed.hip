@@ -310,7 +310,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_bf(const int *__restric
 
 // ---- SYRK: A(m,k) = X[m][k], B(k,n) = X[n][k], both k-contiguous, X pre-split into NP BF16
 // arrays by the X build: a stage moves 16-byte groups straight into LDS.  Rows are ld apart
-// (ld = kdim for the SYRK's X)
+// (ld = kdim for the SYRK's X; L_i for the blocks of the pre-split X build)
 template <int NP, int BK>
 struct SLoad {
     static constexpr bool kPresplit = true;
@@ -333,6 +333,88 @@ struct SLoad {
         }
     }
 };
+
+// ---- two-part X build from k-contiguous blocks.  For basis function i with klen_i pair rows,
+// L_i = klen_i rounded up to 8 and kofs_i = sum_{j<i} L_j.  k_gather_split_t writes, for each
+// row r < R of the source and kk < L_i, the two BF16 parts of src[idx[xptr_i + kk]][r] at
+// dst[q * dpart + kofs_i * R + r * L_i + kk] (zero for kk >= klen_i): the gathered rows
+// transposed so that k runs contiguously, the order the matrix-core operands are read in.  B
+// (idx = the pair rows) is laid out once per DF tensor, C_occ (idx = the partners) once per
+// build; the X build then reads 16-byte groups of both operands with no gather and no split.
+__global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict__ src, long ld, int R,
+                                                        const int *__restrict__ idx, const int *__restrict__ xptr,
+                                                        const int *__restrict__ kofs, __bf16 *__restrict__ dst,
+                                                        size_t dpart)
+{
+    // workgroup (row tile of 64, k tile of 64, function i); k tiles beyond L_i have nothing to do
+    const int i = blockIdx.z, r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int kb = xptr[i], klen = xptr[i + 1] - kb, L = (klen + 7) & ~7;
+    if (k0 >= L) return;
+    __bf16 *d0 = dst + (size_t) kofs[i] * R, *d1 = d0 + dpart;
+    __shared__ float tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int kk = ty; kk < 64; kk += 4) {
+        const int k = k0 + kk;
+        tile[kk][tx] = (k < klen && r0 + tx < R) ? src[(size_t) idx[kb + k] * ld + r0 + tx] : 0.f;
+    }
+    __syncthreads();
+    const int rl = threadIdx.x >> 2, ks = (threadIdx.x & 3) * 16, r = r0 + rl;
+    if (r >= R) return;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const int k = k0 + ks + 8 * g;
+        if (k >= L) break;
+        v8bf h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = tile[ks + 8 * g + e][rl];
+            h[e] = (__bf16) x;
+            l[e] = (__bf16) (x - (float) h[e]);
+        }
+        *(v8bf *) (d0 + (size_t) r * L + k) = h;
+        *(v8bf *) (d1 + (size_t) r * L + k) = l;
+    }
+}
+
+// X_i (nrow x naux_pad, two parts) = Cp_i Bt_i^T over k < L_i; workgroup (64 WM) x (64 WN)
+template <int WM, int WN, int BK, int NBUF>
+__global__ __launch_bounds__(64 * WM * WN) void k_xbuild_ps(const int *__restrict__ xptr,
+                                                            const int *__restrict__ kofs,
+                                                            const int *__restrict__ order,
+                                                            const __bf16 *__restrict__ Cp, size_t cpart,
+                                                            const __bf16 *__restrict__ Bt, size_t btpart,
+                                                            __bf16 *__restrict__ Xp, size_t xpart, int nrow,
+                                                            int naux_pad, int ncolt, int nrowt, int nbf)
+{
+    // basis functions in order of decreasing row length (order[]), dealt round-robin over the 8
+    // XCDs (the hardware deals workgroup b to XCD b % 8), every tile of the o-th function on
+    // XCD o % 8: the row lengths, which vary 4-5x between functions and run in long stretches
+    // along the basis, even out over the XCDs, the longest run first, and the column tiles of a
+    // function and the row tiles of a column tile share its blocks in one L2
+    const long q = blockIdx.x / 8;
+    const int tiles = ncolt * nrowt;
+    const int o = (int) (q / tiles) * 8 + (int) (blockIdx.x % 8);
+    if (o >= nbf) return;
+    const int i = order[o];
+    const int rem = (int) (q % tiles), ct = rem / nrowt, rt = rem % nrowt;
+    const int L = (xptr[i + 1] - xptr[i] + 7) & ~7;
+    const size_t ko = (size_t) kofs[i];
+    const int m0 = rt * 64 * WM, n0 = ct * 64 * WN;
+    SLoad<2, BK> la{Cp + ko * nrow, cpart, L, nrow, m0};
+    SLoad<2, BK> lb{Bt + ko * naux_pad, btpart, L, naux_pad, n0};
+    v16f acc[2][2];
+    bfs_tile<2, WM, WN, BK, NBUF, true>(la, lb, 0, L, acc, nrow - m0, naux_pad - n0);
+    const size_t xo = (size_t) i * nrow * naux_pad;
+    bfs_epilogue<WN>(acc, [&](int r, int c, float v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < nrow && n < naux_pad) {
+            const __bf16 h = (__bf16) v;
+            Xp[xo + (size_t) m * naux_pad + n] = h;
+            Xp[xpart + xo + (size_t) m * naux_pad + n] = (__bf16) (v - (float) h);
+        }
+    });
+}
 
 constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
 
@@ -478,6 +560,54 @@ int yg_xbuild_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, co
     if (nparts == 3) return xbuild_np<3>(s, nbf, n_occ, naux_pad, xptr, xup, xpartner, Cocc32, B32, X);
     ygpu_set_error(YGPU_ERR_INVALID, "X build: %d BF16 parts (2 or 3)", nparts);
     return YGPU_ERR_INVALID;
+}
+
+int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
+                      const int *xptr, const int *kofs, void *dst, size_t dpart)
+{
+    if (nbf <= 0 || R <= 0 || max_row_len <= 0) return YGPU_OK;
+    if (nbf > 65535) {
+        ygpu_set_error(YGPU_ERR_INVALID, "operand blocks: %d basis functions exceed the grid limit", nbf);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_gather_split_t, dim3((R + 63) / 64, (max_row_len + 63) / 64, nbf), dim3(256), 0, s, src, ld,
+                       R, idx, xptr, kofs, (__bf16 *) dst, dpart);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+template <int WM, int WN, int BK, int NBUF>
+static int launch_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs,
+                            const int *order, const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
+{
+    const int ncolt = (naux_pad + 64 * WN - 1) / (64 * WN), nrowt = (nrow + 64 * WM - 1) / (64 * WM);
+    const size_t xpart = (size_t) nbf * nrow * naux_pad;
+    // 8 XCD lanes of ceil(nbf / 8) functions each (functions past nbf: empty workgroups)
+    const long total = 8L * ((nbf + 7) / 8) * ncolt * nrowt;
+    if (total > 0x7fffffffL) {
+        ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
+        return YGPU_ERR_INVALID;
+    }
+    hipLaunchKernelGGL((k_xbuild_ps<WM, WN, BK, NBUF>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, kofs,
+                       order, (const __bf16 *) Cp, cpart, (const __bf16 *) Bt, btpart, (__bf16 *) X, xpart, nrow, naux_pad,
+                       ncolt, nrowt, nbf);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs, const int *order,
+                 const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
+{
+    if (nbf <= 0 || nrow <= 0) return YGPU_OK;
+#define YG_XPS(WM, WN) \
+    launch_xbuild_ps<WM, WN, 32, 1>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
+    // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD)
+    if (nrow <= 64) return YG_XPS(1, 4);
+    if (nrow <= 128) return YG_XPS(2, 4);
+    if (nrow <= 192) return YG_XPS(3, 4);
+    if (nrow <= 256) return YG_XPS(4, 4);
+    return YG_XPS(5, 3);
+#undef YG_XPS
 }
 
 template <int NP>
