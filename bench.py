@@ -46,11 +46,11 @@ for p in (PKG, ORACLE):
         sys.path.insert(0, p)
 
 WORKLOADS = {
-    "c96h24": "C96H24 graphene flake, seeded 0.01 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 1560, naux 6696)",
-    "c10h22": "C10H22 linear alkane, seeded 0.02 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 260, naux 990)",
-    "water": "H2O, O-H 0.9572 A, 104.52 deg, cc-pVDZ/cc-pVDZ-RI (cart. nbf 25, naux 96)",
-    "c54h18": "C54H18 circumcoronene, seeded 0.01 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 900, naux 3834)",
-    "c30h62": "C30H62 alkane, torsions +/-30 deg, 0.05 A jitter, cc-pVDZ/cc-pVDZ-RI (cart. nbf 760, naux 2910)",
+    "c96h24": "C96H24 graphene flake, seeded 0.01 A jitter, cc-pVDZ/cc-pVDZ-RI",
+    "c10h22": "C10H22 linear alkane, seeded 0.02 A jitter, cc-pVDZ/cc-pVDZ-RI",
+    "water": "H2O, O-H 0.9572 A, 104.52 deg, cc-pVDZ/cc-pVDZ-RI",
+    "c54h18": "C54H18 circumcoronene, seeded 0.01 A jitter, cc-pVDZ/cc-pVDZ-RI",
+    "c30h62": "C30H62 alkane, torsions +/-30 deg, 0.05 A jitter, cc-pVDZ/cc-pVDZ-RI",
 }
 # convergence tolerance of each configuration; the small cases keep the reference's DIIS rule
 # and |dE| < 1e-11, the large ones replace the DIIS slot of the largest stored residual
@@ -226,6 +226,9 @@ def main():
     ap.add_argument("--workload", default="c96h24", choices=sorted(WORKLOADS))
     ap.add_argument("--niters", type=int, default=80)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--basis-type", default="cartesian", choices=["cartesian", "spherical"],
+                    help="Cartesian components (the reference's) or the spherical-harmonic basis (the sizes "
+                         "quoted for cc-pVDZ / cc-pVDZ-RI: C96H24 nbf 1464, naux 5712) for every solve")
     ap.add_argument("--guess", default="learned", choices=["sad", "learned"],
                     help="initial guess of the timed solves: SAD, or SAD plus the learned pair correction "
                          "(network trained on the workload's family, yatscf-df_amd/data/guess_mlp_<family>.bin)")
@@ -270,7 +273,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     xyz = os.path.join(XYZ, args.workload + ".xyz")
-    kw = dict(niters=args.niters, ene_tol=settings["ene_tol"], quirk_a2=settings["quirk_a2"])
+    kw = dict(niters=args.niters, ene_tol=settings["ene_tol"], quirk_a2=settings["quirk_a2"],
+              basis_type=args.basis_type)
     timed_weights = weights if args.guess == "learned" else None
 
     def solve(mx, w, **extra):
@@ -331,7 +335,8 @@ def main():
                    "ene_tol": settings["ene_tol"],
                    "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
                                    "largest stored residual",
-                   "basis": "cc-pVDZ", "aux_basis": "cc-pVDZ-RI", "nbf": nbf, "naux": naux, "n_occ": n_occ,
+                   "basis": "cc-pVDZ", "aux_basis": "cc-pVDZ-RI", "basis_type": args.basis_type,
+                   "nbf": last["nbf_sph"], "nbf_cartesian_components": nbf, "naux": naux, "n_occ": n_occ,
                    "pairs_both_triangles": nnz, "pairs_packed": n_up, "schwarz_tol": 1e-11},
         "iterations": last["iters"],
         "iterations_by_arithmetic": {KIND_NAMES[k]: int(np.sum(kinds(last) == k)) for k in KIND_NAMES

@@ -84,6 +84,37 @@ int ygpu_host_free(void *ptr);
 int ygpu_set_basis(ygpu_ctx_t *ctx, int which, int nshells, const int *am, const int *nprim,
                    const int *prim_off, const double *xyz, const double *alpha, const double *coef);
 
+/* Spherical-harmonic basis option.  The reference reads a "spherical" flag
+ * from the first line of a .gbs file (CMS_BasisSet.c:464-483) and counts 2l+1
+ * functions per shell (:277-281), with no integral code for such shells.
+ * Here a spherical basis is the span of the real solid harmonics inside the
+ * Cartesian one; integrals stay Cartesian.
+ *
+ * ygpu_sph_coefficients: T of degree l (0..4), (l+1)(l+2)/2 rows (Cartesian
+ * components x^a y^b z^c, a = l..0, b = l-a..0, normalised like x^l) by 2l+1
+ * columns (real solid harmonics, m = -l..l), row-major, each column of unit
+ * norm in the metric of those components; the identity for l <= 1.  Needs no
+ * context and no device.
+ *
+ * ygpu_set_aux_spherical: the density-fitting basis is spherical (on != 0):
+ * the columns of (mn|P) and both sides of J_PQ are transformed by T before
+ * the metric factor, and every tensor and matrix that counts auxiliary
+ * functions (B, Jpq, the packed layout) counts the spherical ones.  Call
+ * after ygpu_set_basis(which = 1) and before ygpu_set_sparsity (the layout is
+ * cleared).
+ *
+ * ygpu_set_orth_dim: the orthogonaliser X passed to the density build and the
+ * DIIS calls uses only its first n_orth columns (the rest zero), as X =
+ * T (T^T S T)^{-1/2} of a spherical orbital basis does; F' = X^T F X then has
+ * zero rows and columns beyond n_orth, and the density build adds
+ * YGPU_ORTH_SHIFT (Eh) to their diagonal so that those directions, which hold
+ * no orbital, are never occupied (their eigenvalues read YGPU_ORTH_SHIFT).
+ * n_orth = 0 or nbf: every column (the default). */
+#define YGPU_ORTH_SHIFT 1.0e4
+int ygpu_sph_coefficients(int l, double *T);
+int ygpu_set_aux_spherical(ygpu_ctx_t *ctx, int on);
+int ygpu_set_orth_dim(ygpu_ctx_t *ctx, int n_orth);
+
 /* Nuclear charges and positions (Bohr) for the core Hamiltonian. */
 int ygpu_set_atoms(ygpu_ctx_t *ctx, int natoms, const double *charge, const double *xyz);
 

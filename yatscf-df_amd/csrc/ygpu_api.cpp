@@ -408,9 +408,59 @@ try {
     YG_TRY(upload(b.d_alpha, b.alpha.data(), b.alpha.size() * sizeof(double), c->stream));
     YG_TRY(upload(b.d_coef, b.coef.data(), b.coef.size() * sizeof(double), c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
-    if (which == 0) c->nbf = nbf;
-    else c->naux = nbf;
+    if (which == 0) {
+        c->nbf = nbf;
+        c->n_orth = 0;
+    } else {
+        c->naux = c->naux_cart = nbf;
+        c->aux_sph = 0;
+    }
     invalidate_handoffs(c);
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_sph_coefficients(int l, double *T)
+{
+    if (yg_sph_coefficients(l, T) != YGPU_OK) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_sph_coefficients: l = %d (0..4) / null T", l);
+        return YGPU_ERR_INVALID;
+    }
+    return YGPU_OK;
+}
+
+int ygpu_set_aux_spherical(ygpu_ctx_t *c, int on)
+try {
+    YG_TRY(check_ctx(c));
+    if (c->aux.nshells <= 0) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_aux_spherical: set the auxiliary basis first");
+        return YGPU_ERR_INVALID;
+    }
+    if (on && c->aux.max_am > 4) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_aux_spherical: angular momentum %d beyond g", c->aux.max_am);
+        return YGPU_ERR_INVALID;
+    }
+    if (on) YG_TRY(yg_aux_sph_setup(c));
+    else c->naux = c->naux_cart;
+    c->aux_sph = on ? 1 : 0;
+    // the tensor layout follows from naux: set the sparsity (and build B) again
+    c->have_sparsity = 0;
+    c->have_df = 0;
+    invalidate_handoffs(c);
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_orth_dim(ygpu_ctx_t *c, int n_orth)
+try {
+    YG_TRY(check_ctx(c));
+    if (n_orth < 0 || n_orth > c->nbf) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_orth_dim: %d columns for %d basis functions", n_orth, c->nbf);
+        return YGPU_ERR_INVALID;
+    }
+    c->n_orth = n_orth == c->nbf ? 0 : n_orth;
     return YGPU_OK;
 } catch (...) {
     return exception_status();
@@ -516,6 +566,7 @@ try {
         return YGPU_ERR_INVALID;
     }
     c->naux_pad = (c->naux + ROUND_AUX - 1) / ROUND_AUX * ROUND_AUX;
+    c->naux_cart_pad = (c->naux_cart + ROUND_AUX - 1) / ROUND_AUX * ROUND_AUX;
     c->h_mask.assign(mask, mask + (size_t) nbf * nbf);
     c->h_upmask.assign((size_t) nbf * nbf, -1);
     std::vector<int> up_i, up_j;
@@ -658,20 +709,35 @@ try {
         return YGPU_ERR_INVALID;
     }
     const long ld = c->naux_pad;
-    const size_t tsz = (size_t) c->n_up * ld * sizeof(double);
-    YG_TRY(c->d_pqA.ensure(tsz));
-    YG_TRY(c->d_B.ensure(tsz));
+    // the integrals are made over the Cartesian auxiliary functions (leading dimension ldc); a
+    // spherical auxiliary basis transforms them to its own columns before the metric factor
+    const long ldc = c->aux_sph ? c->naux_cart_pad : ld;
+    const size_t tszc = (size_t) c->n_up * ldc * sizeof(double);
+    YG_TRY(c->d_pqA.ensure(tszc));
+    YG_TRY(c->d_B.ensure(tszc));
     YG_TRY(c->d_Jpq.ensure((size_t) ld * ld * sizeof(double)));
-    YG_TRY(c->d_mat0.ensure((size_t) ld * ld * sizeof(double)));
-    YG_TRY(c->d_mat1.ensure((size_t) ld * ld * sizeof(double)));
+    YG_TRY(c->d_mat0.ensure((size_t) ldc * ldc * sizeof(double)));
+    YG_TRY(c->d_mat1.ensure((size_t) ldc * ldc * sizeof(double)));
     // in sequence: beside the 3-centre integrals (which fill every CU) the latency-bound inverse
     // square root of the metric measured longer by their whole duration
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
-    YG_HIP(hipMemsetAsync(c->d_pqA.p, 0, tsz, c->stream));
+    YG_HIP(hipMemsetAsync(c->d_pqA.p, 0, tszc, c->stream));
     YG_TRY(yg_launch_eri3(c, c->d_pqA.as<double>(), c->stream));
+    if (c->aux_sph) {
+        // (mn|P) T into the other tensor buffer, which then holds the integrals
+        YG_TRY(yg_aux_sph_cols(c, c->stream, c->n_up, c->d_pqA.as<double>(), ldc, c->d_B.as<double>(), ld));
+        c->d_pqA.swap(c->d_B);
+    }
     YG_HIP(hipEventRecord(c->ev[1], c->stream));
-    YG_HIP(hipMemsetAsync(c->d_mat0.p, 0, (size_t) ld * ld * sizeof(double), c->stream));
-    YG_TRY(yg_launch_eri2(c, c->d_mat0.as<double>(), (int) ld));
+    YG_HIP(hipMemsetAsync(c->d_mat0.p, 0, (size_t) ldc * ldc * sizeof(double), c->stream));
+    YG_TRY(yg_launch_eri2(c, c->d_mat0.as<double>(), (int) ldc));
+    if (c->aux_sph) {
+        // T^T J T: the columns of J, then (J symmetric) the columns of (J T)^T = T^T J
+        YG_TRY(c->d_mat2.ensure((size_t) ldc * ldc * sizeof(double)));
+        YG_TRY(yg_aux_sph_cols(c, c->stream, ldc, c->d_mat0.as<double>(), ldc, c->d_mat1.as<double>(), ld));
+        YG_TRY(yg_transpose(c->stream, (int) ldc, (int) ld, c->d_mat1.as<double>(), c->d_mat2.as<double>()));
+        YG_TRY(yg_aux_sph_cols(c, c->stream, ld, c->d_mat2.as<double>(), ldc, c->d_mat0.as<double>(), ld));
+    }
     YG_HIP(hipEventRecord(c->ev[2], c->stream));
     const bool chol = c->metric_mode == YGPU_METRIC_CHOLESKY;
     if (chol) {
@@ -1020,6 +1086,9 @@ try {
     // Fo NULL: the last DIIS transform or extrapolation, already on the device
     if (!fo_dev) YG_HIP(hipMemcpyAsync(c->d_mat0.p, Fo, ms, hipMemcpyHostToDevice, c->stream));
     else YG_HIP(hipMemcpyAsync(c->d_mat0.p, c->d_Fcur.p, ms, hipMemcpyDeviceToDevice, c->stream));
+    // directions outside the orthogonaliser's columns (zero rows and columns of F') moved far
+    // above the spectrum, so that they are never occupied
+    YG_TRY(yg_shift_tail_diag(c->stream, nbf, c->n_orth, c->d_mat0.as<double>(), YGPU_ORTH_SHIFT));
     c->fock_valid = 0;                          // d_D and d_Cocc are overwritten below
     c->density_valid = 0;
     const double *dX = c->d_Xo.as<double>();

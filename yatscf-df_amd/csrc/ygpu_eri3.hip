@@ -239,7 +239,7 @@ int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s)
     g.sp_scr = c->d_sp_scr.as<double>(); g.df_scr = c->d_df_scr.as<double>();
     g.scrtol2 = c->scrtol2;
     g.tab = c->boys.as<double>();
-    g.out = dpqA; g.ld = c->naux_pad;
+    g.out = dpqA; g.ld = c->naux_cart_pad;   // the Cartesian auxiliary functions
     for (int la = 0; la < 3; ++la)
         for (int lb = 0; lb <= la; ++lb)
             for (int lc = 0; lc < 4; ++lc) {

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/yatscf_gpu.h"
@@ -52,6 +53,11 @@ struct DBuf {
     int ensure(size_t n);
     void release();
     template <class T> T *as() const { return (T *) p; }
+    void swap(DBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 struct BasisDev {
@@ -80,6 +86,11 @@ struct ygpu_ctx {
     // sparsity (set by ygpu_set_sparsity)
     int have_sparsity = 0;
     int nbf = 0, naux = 0, naux_pad = 0;
+    // spherical auxiliary basis (ygpu_set_aux_spherical): naux counts spherical functions; the
+    // integrals are made over the naux_cart Cartesian ones and transformed column by column
+    int aux_sph = 0, naux_cart = 0, naux_cart_pad = 0, sph_cmax = 1;
+    DBuf d_sph_cs, d_sph_nc, d_sph_coef;
+    int n_orth = 0;               // orthogonaliser columns in use (0: all); ygpu_set_orth_dim
     long n_up = 0, nnz = 0;
     std::vector<int> h_upmask;    // nbf*nbf -> up row or -1
     std::vector<int> h_mask;      // reference CSR row of (i,j) or -1
@@ -229,6 +240,12 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
 // SYRK reads it
 // dst[q][kofs_i R + r L_i + kk] = BF16 part q (of 2) of src[idx[xptr_i + kk]][r], r < R, kk < L_i
 // (zero for kk >= row length): the gathered rows of every basis function i transposed
+// spherical option: T of degree l (ncart(l) x (2l+1), row-major); the aux column tables and
+// naux (spherical) from c->aux; out = in T over the aux columns; A[k][k] += shift, k >= n_orth
+int yg_sph_coefficients(int l, double *T);
+int yg_aux_sph_setup(ygpu_ctx *c);
+int yg_aux_sph_cols(ygpu_ctx *c, hipStream_t s, long rows, const double *in, long ldin, double *out, long ldout);
+int yg_shift_tail_diag(hipStream_t s, int n, int n_orth, double *A, double shift);
 int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
                       const int *xptr, const int *kofs, void *dst, size_t dpart);
 // X (two BF16 parts, nrow rows per basis function) from the k-contiguous blocks Cp (nrow rows)

@@ -43,6 +43,9 @@ int    CMS_getAtomEndInd(BasisSet_t basis, int atomid);
 int    CMS_getTotalCharge(BasisSet_t basis);
 int    CMS_getNneutral(BasisSet_t basis);
 int    CMS_getMaxMomentum(BasisSet_t basis);
+/* 1 if the .gbs file's first token is "spherical" (CMS_BasisSet.c:464-483); the
+ * shells are stored with their Cartesian components in both cases */
+int    CMS_isSpherical(BasisSet_t basis);
 double CMS_getNucEnergy(BasisSet_t basis);
 void   CMS_getInitialGuess(BasisSet_t basis, int atomid, double **guess, int *spos, int *epos);
 void   CMS_getShellxyz(BasisSet_t basis, int shellid, double *x, double *y, double *z);

@@ -18,13 +18,121 @@
 #include "utils.h"
 #include "yatscf_gpu.h"
 
+/* ---- spherical orbital basis: the real solid harmonics among the Cartesian components
+ * (ygpu_sph_coefficients gives T_l per degree l) */
+static double ys_dfact(int n)
+{
+    double r = 1.0;
+    for (int i = n; i > 1; i -= 2) r *= i;
+    return r;
+}
+
+/* P_l = T_l T_l^T M_l (ncart x ncart, row-major), M_l the overlap of the x^l-normalised
+ * components of one shell (angular factor (a+a'-1)!!(b+b'-1)!!(c+c'-1)!!/(2l-1)!!): the
+ * projector, in that metric, of a shell's Cartesian coefficients onto its solid harmonics */
+static void sph_projector(int l, double *P)
+{
+    const int nc = (l + 1) * (l + 2) / 2, ns = 2 * l + 1;
+    double Tl[15 * 9], M[15 * 15], TT[15 * 15];
+    int pa[15], pb[15], pc[15], i = 0;
+    ygpu_sph_coefficients(l, Tl);
+    for (int a = l; a >= 0; a--)
+        for (int b = l - a; b >= 0; b--, i++) {
+            pa[i] = a;
+            pb[i] = b;
+            pc[i] = l - a - b;
+        }
+    for (i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            const int x = pa[i] + pa[j], y = pb[i] + pb[j], z = pc[i] + pc[j];
+            M[i * nc + j] = ((x | y | z) & 1) ? 0.0 : ys_dfact(x - 1) * ys_dfact(y - 1) * ys_dfact(z - 1) / ys_dfact(2 * l - 1);
+            double t = 0.0;
+            for (int q = 0; q < ns; q++) t += Tl[i * ns + q] * Tl[j * ns + q];
+            TT[i * nc + j] = t;
+        }
+    for (i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            double t = 0.0;
+            for (int k = 0; k < nc; k++) t += TT[i * nc + k] * M[k * nc + j];
+            P[i * nc + j] = t;
+        }
+}
+
+/* A <- P A P^T with P block-diagonal over the shells (P_l of each shell of degree >= 2) */
+static void sph_project_sym(TinySCF_t T, double *A)
+{
+    const int n = T->nbasfuncs;
+    const int *am, *np, *po;
+    const double *xyz, *al, *co;
+    double P[5][15 * 15], v[15];
+    for (int l = 2; l <= 4; l++) sph_projector(l, P[l]);
+    CMS_getShellArrays(T->basis, &am, &np, &po, &xyz, &al, &co);
+    for (int s = 0; s < T->nshells; s++) {
+        const int l = am[s], nc = (l + 1) * (l + 2) / 2, f0 = T->shell_bf_sind[s];
+        if (l < 2) continue;
+        for (int j = 0; j < n; j++) {          /* rows of the shell: P A */
+            for (int c = 0; c < nc; c++) v[c] = A[(size_t) (f0 + c) * n + j];
+            for (int c = 0; c < nc; c++) {
+                double t = 0.0;
+                for (int k = 0; k < nc; k++) t += P[l][c * nc + k] * v[k];
+                A[(size_t) (f0 + c) * n + j] = t;
+            }
+        }
+        for (int i = 0; i < n; i++) {          /* columns of the shell: A P^T */
+            double *row = A + (size_t) i * n + f0;
+            for (int c = 0; c < nc; c++) v[c] = row[c];
+            for (int c = 0; c < nc; c++) {
+                double t = 0.0;
+                for (int k = 0; k < nc; k++) t += P[l][c * nc + k] * v[k];
+                row[c] = t;
+            }
+        }
+    }
+}
+
+/* X = T (T^T S T)^{-1/2} (nbf x nbf_sph) in the first nbf_sph columns of X_mat, zero beyond */
+static void sph_orthogonaliser(TinySCF_t T)
+{
+    ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
+    const int n = T->nbasfuncs, m = T->nbf_sph;
+    const int *am, *np, *po;
+    const double *xyz, *al, *co;
+    double Tl[15 * 9];
+    double *Ts = (double *) calloc((size_t) n * m, DBL_SIZE), *W = (double *) malloc(DBL_SIZE * (size_t) n * m),
+           *Ss = (double *) malloc(DBL_SIZE * (size_t) m * m), *Xs = (double *) malloc(DBL_SIZE * (size_t) m * m);
+    if (!Ts || !W || !Ss || !Xs) {
+        free(Ts); free(W); free(Ss); free(Xs);
+        ys_check_gpu(YGPU_ERR_NOMEM, "spherical orthogonaliser (host allocation)");
+    }
+    CMS_getShellArrays(T->basis, &am, &np, &po, &xyz, &al, &co);
+    for (int s = 0, q0 = 0; s < T->nshells; s++) {
+        const int l = am[s], nc = (l + 1) * (l + 2) / 2, ns = 2 * l + 1, f0 = T->shell_bf_sind[s];
+        ygpu_sph_coefficients(l, Tl);
+        for (int c = 0; c < nc; c++)
+            for (int q = 0; q < ns; q++) Ts[(size_t) (f0 + c) * m + q0 + q] = Tl[c * ns + q];
+        q0 += ns;
+    }
+    YS_GPU(ygpu_dgemm(g, 0, 0, n, m, n, 1.0, T->S_mat, n, Ts, m, 0.0, W, m));      /* S T */
+    YS_GPU(ygpu_dgemm(g, 1, 0, m, m, n, 1.0, Ts, m, W, m, 0.0, Ss, m));            /* T^T S T */
+    YS_GPU(ygpu_sym_inv_sqrt(g, m, Ss, Xs, NULL));
+    YS_GPU(ygpu_dgemm(g, 0, 0, n, m, m, 1.0, Ts, m, Xs, m, 0.0, W, m));            /* T (T^T S T)^{-1/2} */
+    for (int i = 0; i < n; i++) {
+        memcpy(T->X_mat + (size_t) i * n, W + (size_t) i * m, DBL_SIZE * m);
+        memset(T->X_mat + (size_t) i * n + m, 0, DBL_SIZE * (n - m));
+    }
+    YS_GPU(ygpu_set_orth_dim(g, m));
+    free(Ts); free(W); free(Ss); free(Xs);
+}
+
 void TinySCF_compute_Hcore_Ovlp_mat(TinySCF_t T)
 {
     double st = get_wtime_sec();
     ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
     YS_GPU(ygpu_ovlp_hcore(g, T->S_mat, T->Hcore_mat));
-    /* X = S^{-1/2} = U s^{-1/2} U^T (TinySCF.c:57-76) */
-    YS_GPU(ygpu_sym_inv_sqrt(g, T->nbasfuncs, T->S_mat, T->X_mat, NULL));
+    /* X = S^{-1/2} = U s^{-1/2} U^T (TinySCF.c:57-76); a spherical basis confines X's columns
+     * to the solid harmonics */
+    if (T->sph_orb) sph_orthogonaliser(T);
+    else YS_GPU(ygpu_sym_inv_sqrt(g, T->nbasfuncs, T->S_mat, T->X_mat, NULL));
     YS_GPU(ygpu_set_overlap(g, T->S_mat, T->X_mat));
     T->S_Hcore_time = get_wtime_sec() - st;
     if (T->verbose)
@@ -212,6 +320,8 @@ void TinySCF_get_initial_guess(TinySCF_t T)
     if (T->charge != 0 && T->electron != 0) R = (double) (T->electron - T->charge) / (double) T->electron;
     R *= 0.5;
     for (int i = 0; i < T->mat_size; i++) T->D_mat[i] *= R;
+    /* a spherical basis keeps the solid-harmonic part of each shell's block */
+    if (T->sph_orb) sph_project_sym(T, T->D_mat);
     T->nuc_energy = CMS_getNucEnergy(T->basis);
     double t1 = get_wtime_sec();
     TinySCF_D2Cocc(T);

@@ -132,6 +132,12 @@ struct TinySCF_struct
     double diis_error;      /* max |R| of the newest DIIS residual (iterations >= 2) */
     char   guess_weights[512]; /* learned-guess network file ("" = SAD guess only) */
     int    guess_ml;        /* 1 if the last initial guess used the learned correction */
+    int    sph_orb, sph_aux; /* spherical orbital / auxiliary basis (the .gbs flag, or
+                               YATSCF_BASIS_TYPE=spherical|cartesian for both): the orbitals
+                               span the real solid harmonics among the Cartesian components
+                               (X = T (T^T S T)^{-1/2}, zero columns beyond nbf_sph) and the
+                               auxiliary functions are the solid harmonics (df_nbf counts them) */
+    int    nbf_sph;         /* orbital basis functions of the calculation (nbasfuncs if Cartesian) */
     int    verbose;         /* reference-style printout, default 1 */
     int    converged;
     double energy_delta;

@@ -105,6 +105,28 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->electron = CMS_getNneutral(T->basis);
     T->df_nbf = CMS_getNumFuncs(T->df_basis);
     T->df_nshells = CMS_getNumShells(T->df_basis);
+    {
+        /* basis type: the first token of each .gbs file (CMS_BasisSet.c:464-483), or both set by
+         * YATSCF_BASIS_TYPE */
+        const char *bt = getenv("YATSCF_BASIS_TYPE");
+        T->sph_orb = CMS_isSpherical(T->basis);
+        T->sph_aux = CMS_isSpherical(T->df_basis);
+        if (bt && !strcmp(bt, "spherical")) T->sph_orb = T->sph_aux = 1;
+        else if (bt && !strcmp(bt, "cartesian")) T->sph_orb = T->sph_aux = 0;
+        const int *am, *np, *po;
+        const double *xyz, *al, *co;
+        T->nbf_sph = T->nbasfuncs;
+        if (T->sph_orb) {
+            CMS_getShellArrays(T->basis, &am, &np, &po, &xyz, &al, &co);
+            T->nbf_sph = 0;
+            for (int s = 0; s < T->nshells; s++) T->nbf_sph += 2 * am[s] + 1;
+        }
+        if (T->sph_aux) {
+            CMS_getShellArrays(T->df_basis, &am, &np, &po, &xyz, &al, &co);
+            T->df_nbf = 0;
+            for (int s = 0; s < T->df_nshells; s++) T->df_nbf += 2 * am[s] + 1;
+        }
+    }
     if (T->n_occ <= 0 || T->n_occ > T->nbasfuncs) {
         fprintf(stderr, "TinySCF: %d occupied orbitals for %d basis functions\n", T->n_occ, T->nbasfuncs);
         return -3;
@@ -116,9 +138,9 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
         printf("    molecule             = %s\n", basename(xyz_fname));
         printf("    # atoms              = %d\n", T->natoms);
         printf("    # shells             = %d\n", T->nshells);
-        printf("    # basis functions    = %d\n", T->nbasfuncs);
+        printf("    # basis functions    = %d%s\n", T->nbf_sph, T->sph_orb ? " (spherical)" : "");
         printf("    # DF shells          = %d\n", T->df_nshells);
-        printf("    # DF basis functions = %d\n", T->df_nbf);
+        printf("    # DF basis functions = %d%s\n", T->df_nbf, T->sph_aux ? " (spherical)" : "");
         printf("    # occupied orbits    = %d\n", T->n_occ);
         printf("    # charge             = %d\n", T->charge);
         printf("    # electrons          = %d\n", T->electron);
@@ -162,9 +184,13 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->shell_bf_sind[T->nshells] = T->nbasfuncs;
     T->df_shell_bf_sind = (int *) xalloc(INT_SIZE * (T->df_nshells + 1), mem);
     T->df_shell_bf_num = (int *) xalloc(INT_SIZE * T->df_nshells, mem);
-    for (int i = 0; i < T->df_nshells; i++) {
-        T->df_shell_bf_sind[i] = CMS_getFuncStartInd(T->df_basis, i);
-        T->df_shell_bf_num[i] = CMS_getShellDim(T->df_basis, i);
+    for (int i = 0, f = 0; i < T->df_nshells; i++) {
+        /* spherical auxiliary basis: 2l+1 functions per shell */
+        const int dim = CMS_getShellDim(T->df_basis, i);
+        const int l = dim == 1 ? 0 : dim == 3 ? 1 : dim == 6 ? 2 : dim == 10 ? 3 : 4;
+        T->df_shell_bf_sind[i] = T->sph_aux ? f : CMS_getFuncStartInd(T->df_basis, i);
+        T->df_shell_bf_num[i] = T->sph_aux ? 2 * l + 1 : dim;
+        f += T->df_shell_bf_num[i];
     }
     T->df_shell_bf_sind[T->df_nshells] = T->df_nbf;
 
@@ -217,6 +243,7 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     if (ygpu_set_basis(g, 0, T->nshells, am, np, po, xyz, al, co) != YGPU_OK) goto gpu_fail;
     CMS_getShellArrays(T->df_basis, &am, &np, &po, &xyz, &al, &co);
     if (ygpu_set_basis(g, 1, T->df_nshells, am, np, po, xyz, al, co) != YGPU_OK) goto gpu_fail;
+    if (T->sph_aux && ygpu_set_aux_spherical(g, 1) != YGPU_OK) goto gpu_fail;
     CMS_getAtoms(T->basis, &chg, &axyz);
     if (ygpu_set_atoms(g, T->natoms, chg, axyz) != YGPU_OK) goto gpu_fail;
     /* factor of the Coulomb metric: the reference's J^{-1/2} (YATSCF_METRIC=invsqrt) or the

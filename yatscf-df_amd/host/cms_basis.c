@@ -9,7 +9,9 @@
  *        re-reads the previous symbol and fails, :365-381); nuclear repulsion
  *        by the closed form (:409-421).
  *   gbs: first token "spherical" selects spherical, anything else Cartesian
- *        (:468-483); element blocks end at "****"; "<letters> <nexp> <scale>"
+ *        (:468-483); the shells are stored with their Cartesian components
+ *        either way (a spherical basis is the span of the solid harmonics
+ *        among them, see CMS_isSpherical); element blocks end at "****"; "<letters> <nexp> <scale>"
  *        shell lines, one shell per letter sharing the exponents (SP), scale
  *        ignored; primitive lines read with one persistent coefficient array
  *        (:556-616).
@@ -166,7 +168,6 @@ static CMSStatus_t read_gbs(BasisSet_t b, const char *path)
     tok[0] = 0;
     sscanf(line, "%1023s", tok);
     b->cartesian = strcmp(tok, "spherical") != 0;
-    if (!b->cartesian) { fclose(fp); return fail(CMS_STATUS_INVALID_VALUE, "%s: spherical basis sets are not supported", path); }
     double cc[3] = {0.0, 0.0, 0.0};
     while (fgets(line, sizeof(line), fp)) {
         if (!isalpha((unsigned char) line[0])) continue;
@@ -332,6 +333,7 @@ int CMS_getAtomEndInd(BasisSet_t b, int a) { return b->s_start[a + 1] - 1; }
 int CMS_getTotalCharge(BasisSet_t b) { return b->Q; }
 int CMS_getNneutral(BasisSet_t b) { return b->nelectrons; }
 int CMS_getMaxMomentum(BasisSet_t b) { return b->max_momentum; }
+int CMS_isSpherical(BasisSet_t b) { return !b->cartesian; }
 double CMS_getNucEnergy(BasisSet_t b) { return b->ene_nuc; }
 
 void CMS_getInitialGuess(BasisSet_t b, int atomid, double **guess, int *spos, int *epos)

@@ -111,6 +111,9 @@ int *tinyscf_iptr(TinySCF_t T, const char *n, size_t *count)
 int tinyscf_geti(TinySCF_t T, const char *n)
 {
     if (!strcmp(n, "nbasfuncs")) return T->nbasfuncs;
+    if (!strcmp(n, "nbf_sph")) return T->nbf_sph;
+    if (!strcmp(n, "sph_orb")) return T->sph_orb;
+    if (!strcmp(n, "sph_aux")) return T->sph_aux;
     if (!strcmp(n, "df_nbf")) return T->df_nbf;
     if (!strcmp(n, "nshells")) return T->nshells;
     if (!strcmp(n, "df_nshells")) return T->df_nshells;

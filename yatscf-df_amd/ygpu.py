@@ -71,6 +71,9 @@ SIGNATURES = {
     "ygpu_diis_error": [vp, ctypes.c_int, dp],
     "ygpu_set_fock_precision": [vp, ctypes.c_int],
     "ygpu_set_metric_factor": [vp, ctypes.c_int],
+    "ygpu_sph_coefficients": [ctypes.c_int, dp],
+    "ygpu_set_aux_spherical": [vp, ctypes.c_int],
+    "ygpu_set_orth_dim": [vp, ctypes.c_int],
     "ygpu_fock_last_info": [vp, ip, dp],
     "ygpu_mlp_forward": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
@@ -392,11 +395,26 @@ class Engine:
 class TinySCF:
     """The TinySCF host engine (libtinyscf.so) with the reference's call sequence."""
 
-    def __init__(self, xyz, niters=50, basis=BASIS, auxbasis=AUXBASIS, verbose=False):
+    def __init__(self, xyz, niters=50, basis=BASIS, auxbasis=AUXBASIS, verbose=False, basis_type=None):
+        """basis_type: None (the .gbs files' own flags), "cartesian" or "spherical" (both basis
+        sets)."""
         self.L = host_lib()
         self.T = self.L.tinyscf_alloc()
         os.environ.setdefault("YATSCF_VERBOSE", "1" if verbose else "0")
-        st = self.L.TinySCF_try_init(self.T, basis.encode(), auxbasis.encode(), xyz.encode(), niters)
+        if basis_type not in (None, "cartesian", "spherical"):
+            raise ValueError("basis_type: None, 'cartesian' or 'spherical'")
+        old = os.environ.get("YATSCF_BASIS_TYPE")
+        if basis_type:
+            os.environ["YATSCF_BASIS_TYPE"] = basis_type
+        else:
+            os.environ.pop("YATSCF_BASIS_TYPE", None)
+        try:
+            st = self.L.TinySCF_try_init(self.T, basis.encode(), auxbasis.encode(), xyz.encode(), niters)
+        finally:
+            if old is None:
+                os.environ.pop("YATSCF_BASIS_TYPE", None)
+            else:
+                os.environ["YATSCF_BASIS_TYPE"] = old
         if st != 0:
             raise RuntimeError("TinySCF init failed (%d): %s" % (st, last_error()))
         self.L.tinyscf_seti(self.T, b"verbose", int(verbose))
@@ -474,7 +492,8 @@ class TinySCF:
 
 
 def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, fock_check=False,
-          final_fock=False, schedule=None, keep_diis=None, reref=None, verbose=False):
+          final_fock=False, schedule=None, keep_diis=None, reref=None, verbose=False, basis_type=None,
+          basis=BASIS, auxbasis=AUXBASIS):
     """One complete SCF of ``xyz`` through the TinySCF engine, timed from after input parsing
     (basis shells and nuclei uploaded) to the passing convergence test: S/H/X, Schwarz
     screening, pair layout, initial guess (SAD, plus the learned pair correction when
@@ -483,9 +502,11 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
     FP64 Fock matrix of the converged density (AO basis) as "F" (the engine's F_mat holds the
     DIIS output in the orthogonal basis); with ``fock_check`` also one mixed Fock build on the
     same density, the reduced-precision arithmetic's own error in F and in the orbital energies
-    of X^T F X.  Returns a dict; the engine is closed."""
+    of X^T F X.  ``basis_type``: see TinySCF.  Returns a dict ("nbf" the Cartesian components
+    the matrices are written in, "nbf_sph" the orbital basis functions of the calculation); the
+    engine is closed."""
     import time
-    T = TinySCF(xyz, niters=niters, verbose=verbose)
+    T = TinySCF(xyz, niters=niters, verbose=verbose, basis_type=basis_type, basis=basis, auxbasis=auxbasis)
     try:
         T.L.tinyscf_set_guess_weights(T.T, (weights or "").encode())
         T.setd("ene_tol", ene_tol)
@@ -511,7 +532,8 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
                    naux=T.geti("df_nbf"), n_occ=T.geti("n_occ"), nnz=T.geti("num_bf_pair_scr"),
                    switch=T.geti("mixed_switch_iter"), switch6=T.geti("mixed6_switch_iter"),
                    phases=T.i("phase_log").copy(),
-                   guess_ml=T.geti("guess_ml"),
+                   guess_ml=T.geti("guess_ml"), nbf_sph=T.geti("nbf_sph"), sph_orb=T.geti("sph_orb"),
+                   sph_aux=T.geti("sph_aux"),
                    D=T.mat("D_mat").copy(), S=T.mat("S_mat").copy(),
                    X=T.mat("X_mat").copy(), H=T.mat("Hcore_mat").copy(), eigval=T.d("eigval").copy())
         mask = T.i("bf_pair_mask").reshape(nbf, nbf)
@@ -521,8 +543,9 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
         if fock_check:
             X, F64 = res["X"], res["F"]
             Fmx = T.build_fock(FOCK_MIXED)
-            e64 = np.linalg.eigvalsh(X.T @ F64 @ X)
-            emx = np.linalg.eigvalsh(X.T @ Fmx @ X)
+            m = res["nbf_sph"]                  # X's columns beyond the orbital basis are zero
+            e64 = np.linalg.eigvalsh((X.T @ F64 @ X)[:m, :m])
+            emx = np.linalg.eigvalsh((X.T @ Fmx @ X)[:m, :m])
             no = res["n_occ"]
             res["fock_check"] = {"max_abs_dF": float(np.max(np.abs(Fmx - F64))),
                                  "max_abs_d_orbital_energy": float(np.max(np.abs(emx - e64))),
