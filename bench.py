@@ -161,14 +161,16 @@ def kernel_table(runs, mixed):
             np_ = 3 if kind == FOCK_MIXED else 6
             peak = BF16_PEAK_TFLOPS / np_
             tag = "BF16x%d split" % np_
-            add("k_xbuild_bf (X = C_occ^T B, %s)" % tag, "mfma", xb, n, xb_flops, peak, "TFLOP/s", tag + ", f32 acc")
+            # two parts: from the k-contiguous operand blocks; three: the gathering X build
+            xname = "k_xbuild_ps" if kind == FOCK_MIXED else "k_xbuild_bf"
+            add("%s (X = C_occ^T B, %s)" % (xname, tag), "mfma", xb, n, xb_flops, peak, "TFLOP/s", tag + ", f32 acc")
             add("k_syrk_bf (K = X X^T, %s)" % tag, "mfma", sy, n, syrk_flops, peak, "TFLOP/s", tag + ", f32 acc")
             add("J build (%s iterations, FP32 tensor, two passes)" % tag, "hbm", jj, n, j32, HBM_PEAK_GBS, "GB/s",
                 "f32 tensor, f64 sums")
         elif kind == FOCK_INCR:
             peak = BF16_PEAK_TFLOPS / 3.0
             # X^P and X^M together: the X build over 2 n_occ columns; G = X^M (X^P)^T: every tile
-            add("k_xbuild_bf (X^[P|M], 2 n_occ columns, BF16x3 split)", "mfma", xb, n, 2.0 * xb_flops, peak, "TFLOP/s",
+            add("k_xbuild_ps (X^[P|M], 2 n_occ columns, BF16x3 split)", "mfma", xb, n, 2.0 * xb_flops, peak, "TFLOP/s",
                 "BF16x3 split, f32 acc")
             add("k_gemm_bf_nt (G = X^M X^P^T, difference K, BF16x3 split)", "mfma", sy, n,
                 2.0 * nbf * nbf * n_occ * naux, peak, "TFLOP/s", "BF16x3 split, f32 acc")
@@ -330,8 +332,9 @@ def main():
                    "precision": ("BF16x3-split K (FP32 accumulation) and FP32-tensor incremental J until the DIIS "
                                  "error reaches the split's floor; then a full FP64 reference build and "
                                  "difference-density builds K = K_ref + K(D - D_ref) (BF16x3 products of "
-                                 "aligned orbitals) with FP64 re-references; convergence confirmed by a full "
-                                 "FP64 build" if mixed else "FP64 (plain path)"),
+                                 "aligned orbitals) with FP64 re-references; converged when two energies "
+                                 "agree and the later is a full FP64 build or the difference build right "
+                                 "after one" if mixed else "FP64 (plain path)"),
                    "ene_tol": settings["ene_tol"],
                    "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
                                    "largest stored residual",

@@ -686,17 +686,21 @@ static int join_side(ygpu_ctx *c, hipStream_t s)
 
 // X of the reduced-precision builds into c->d_X, nrow columns of C32 (nbf x nrow, FP32): two
 // parts from the k-contiguous blocks (C_occ's laid out here, B's in d_Bt), three parts by the
-// gathering X build
+// gathering X build.  Event ev[8] is recorded right before the X-build kernel, which the "aux K"
+// timer then brackets alone.
 static int xbuild_mixed(ygpu_ctx *c, int nparts, int nrow, const float *C32, size_t btpart)
 {
     const int nbf = c->nbf;
-    if (nparts != 2)
+    if (nparts != 2) {
+        YG_HIP(hipEventRecord(c->ev[8], c->stream));
         return yg_xbuild_bf(c->stream, nparts, nbf, nrow, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
                             c->d_xpartner.as<int>(), C32, c->d_B32.as<float>(), c->d_X.p);
+    }
     const size_t cpart = (size_t) c->ksum * nrow;
     YG_TRY(c->d_Cp.ensure(2 * cpart * sizeof(__bf16)));
     YG_TRY(yg_gather_split_t(c->stream, nbf, c->max_row_len, C32, nrow, nrow, c->d_xpartner.as<int>(), c->d_xptr.as<int>(),
                              c->d_kofs.as<int>(), c->d_Cp.p, cpart));
+    YG_HIP(hipEventRecord(c->ev[8], c->stream));
     return yg_xbuild_ps(c->stream, nbf, nrow, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_kofs.as<int>(),
                         c->d_xorder.as<int>(), c->d_Cp.p, cpart, c->d_Bt.p, btpart, c->d_X.p);
 }
@@ -961,7 +965,6 @@ try {
             YG_TRY(yg_accum(c->side, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
         }
         YG_HIP(hipEventRecord(c->ev[3], c->side));
-        YG_HIP(hipEventRecord(c->ev[8], c->stream));
         if (mode == YGPU_FOCK_INCR) {
             // D - D_ref = (P M^T + M P^T) / 2 with P = C1 + C_ref, M = C1 - C_ref: K(D - D_ref) =
             // (X^M (X^P)^T + X^P (X^M)^T) / 2, BF16-split products whose error scales with |M|;

@@ -7,8 +7,10 @@
 //     reduces its four wavefronts in LDS and leaves one partial per row group,
 //     summed in a fixed order (deterministic; replaces the per-thread buffers
 //     and tree reduction of :15-33, :61-98);
-//   pass 2 "J matrix": J_ij = B[r] . temp_J with temp_J staged in LDS, one
-//     wavefront per row, cross-lane reduction with shuffles.
+//   pass 2 "J matrix": J_ij = B[r] . temp_J with temp_J staged in LDS, four
+//     rows in flight per wavefront (each temp_J value read once for the four),
+//     cross-lane reductions with shuffles.
+// Every lane moves 16 bytes of a row per load in both passes.
 // Screened pairs keep J = 0 (quirk A7 of the reference made explicit).
 // Both passes read the tensor as FP64 or, in the mixed-precision phase, as its
 // FP32 copy (half the bytes); the arithmetic is FP64 either way.
@@ -20,24 +22,29 @@ namespace {
 
 constexpr int JWAVES = 4;      // wavefronts per pass-1 workgroup, each on its own rows
 
-// columns per lane: every lane moves 8 bytes per row (one FP64 or two FP32 values)
-template <class TB> constexpr int cpl() { return (int) (8 / sizeof(TB)); }
+// columns per lane: every lane moves 16 bytes per row (two FP64 or four FP32 values); rows are
+// ld values apart with ld a multiple of 64, so these loads are aligned
+template <class TB> constexpr int cpl() { return (int) (16 / sizeof(TB)); }
 
 template <class TB>
 __device__ __forceinline__ void load_cols(const TB *p, double (&v)[cpl<TB>()])
 {
-    if constexpr (cpl<TB>() == 1) {
-        v[0] = (double) p[0];
-    } else {
-        const float2 f = *(const float2 *) p;
+    if constexpr (sizeof(TB) == 8) {
+        const double2 f = *(const double2 *) p;
         v[0] = f.x;
         v[1] = f.y;
+    } else {
+        const float4 f = *(const float4 *) p;
+        v[0] = f.x;
+        v[1] = f.y;
+        v[2] = f.z;
+        v[3] = f.w;
     }
 }
 
 // pass 1: partial[g][c] = sum over the rows of row group g of w_r B[r][c]; grid
 // (column tiles of 64 lanes x cpl columns, row groups); rows are interleaved over
-// (group, wavefront) so every wavefront streams whole 512-byte row segments, and
+// (group, wavefront) so every wavefront streams whole 1 KB row segments, and
 // the row index is wave-uniform (scalar loads of the pair indices and of D).
 // ld is a multiple of 64, so a lane's cpl columns are all inside or all outside.
 template <class TB>
@@ -116,32 +123,47 @@ __global__ void k_tempj_reduce(const double *__restrict__ part, int ngroup, long
     tempJ[col] = s0 + s1;
 }
 
+constexpr int JM_ROWS = 4;      // pass-2 rows in flight per wavefront
+constexpr int JM_THREADS = 1024;
+
+// pass 2: wavefront g of the grid-stride sequence takes rows JM_ROWS g .. JM_ROWS g + 3 (the
+// last group's missing rows read row n_up - 1 again and are not stored)
 template <class TB>
-__global__ __launch_bounds__(256) void k_jmat(const TB *__restrict__ B, long ld, long n_up,
-                                              const int *__restrict__ up_i, const int *__restrict__ up_j,
-                                              const double *__restrict__ tempJ, int nbf, int rows_per_wave,
-                                              double *__restrict__ J)
+__global__ __launch_bounds__(JM_THREADS) void k_jmat(const TB *__restrict__ B, long ld, long n_up,
+                                                     const int *__restrict__ up_i, const int *__restrict__ up_j,
+                                                     const double *__restrict__ tempJ, int nbf,
+                                                     double *__restrict__ J)
 {
     extern __shared__ double tj[];
     for (long c = threadIdx.x; c < ld; c += blockDim.x) tj[c] = tempJ[c];
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const long rbase = ((long) blockIdx.x * 4 + w) * rows_per_wave;
-    for (int q = 0; q < rows_per_wave; ++q) {
-        const long r = rbase + q;
-        if (r >= n_up) break;
-        constexpr int C = cpl<TB>();
-        const TB *row = B + (size_t) r * ld;
-        double s = 0.0;
-        for (long c = lane * C; c < ld; c += 64 * C) {
-            double v[C];
-            load_cols<TB>(row + c, v);
+    constexpr int C = cpl<TB>();
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (long g = (long) blockIdx.x * nw + (threadIdx.x >> 6); g * JM_ROWS < n_up; g += (long) gridDim.x * nw) {
+        const TB *row[JM_ROWS];
 #pragma unroll
-            for (int e = 0; e < C; ++e) s += v[e] * tj[c + e];
+        for (int q = 0; q < JM_ROWS; ++q) row[q] = B + (size_t) std::min<long>(g * JM_ROWS + q, n_up - 1) * ld;
+        double s[JM_ROWS];
+#pragma unroll
+        for (int q = 0; q < JM_ROWS; ++q) s[q] = 0.0;
+        for (long c = (long) lane * C; c < ld; c += 64 * C) {
+            double v[JM_ROWS][C], t[C];
+#pragma unroll
+            for (int q = 0; q < JM_ROWS; ++q) load_cols<TB>(row[q] + c, v[q]);
+#pragma unroll
+            for (int e = 0; e < C; ++e) t[e] = tj[c + e];
+#pragma unroll
+            for (int q = 0; q < JM_ROWS; ++q)
+#pragma unroll
+                for (int e = 0; e < C; ++e) s[q] += v[q][e] * t[e];
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) J[(size_t) up_i[r] * nbf + up_j[r]] = s;
+        for (int q = 0; q < JM_ROWS; ++q) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off);
+            const long r = g * JM_ROWS + q;
+            if (lane == 0 && r < n_up) J[(size_t) up_i[r] * nbf + up_j[r]] = s[q];
+        }
     }
 }
 
@@ -433,14 +455,15 @@ int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const
     YG_HIP(hipEventRecord(c->ev[1], s));
     // J entries of screened-out pairs stay zero (a fill kernel, not a memset node)
     YG_TRY(yg_fill(s, (long) nbf * nbf, dJ, 0.0));
-    const int rpw = 4;
-    const long nblk = (n_up + 4L * rpw - 1) / (4L * rpw);
+    // two workgroups of 16 wavefronts per CU (temp_J in LDS: ld doubles each), grid-stride rows
+    const long ngroups = (n_up + JM_ROWS - 1) / JM_ROWS, wpb = JM_THREADS / 64;
+    const long nblk = std::max<long>(1, std::min<long>(512, (ngroups + wpb - 1) / wpb));
     if (B32)
-        hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(256), ld * sizeof(double), s, B32, ld, n_up,
-                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, rpw, dJ);
+        hipLaunchKernelGGL(k_jmat<float>, dim3(nblk), dim3(JM_THREADS), ld * sizeof(double), s, B32, ld, n_up,
+                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, dJ);
     else
-        hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(256), ld * sizeof(double), s, c->d_B.as<double>(),
-                           ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, rpw, dJ);
+        hipLaunchKernelGGL(k_jmat<double>, dim3(nblk), dim3(JM_THREADS), ld * sizeof(double), s,
+                           c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), tempJ, nbf, dJ);
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[2], s));
     return YGPU_OK;

@@ -388,7 +388,7 @@ void TinySCF_do_SCF(TinySCF_t T)
     T->mixed_switch_iter = (T->mixed && T->mixed_phase == YGPU_FOCK_FP64) ? 0 : -1;
     T->mixed6_switch_iter = T->mixed_phase == YGPU_FOCK_MIXED6 ? 0 : -1;
     double prev = 0.0, delta = 223.0, best_err = -1.0, diis_at_ref = DBL_MAX;
-    int prev_exact = 0, stall = 0;
+    int prev_exact = 0, stall = 0, since_ref = 0;
     while (T->iter < T->niters && delta >= T->ene_tol) {
         double st0 = get_wtime_sec();
         if (T->verbose) printf("--------------- Iteration %d ---------------\n", T->iter);
@@ -465,19 +465,25 @@ void TinySCF_do_SCF(TinySCF_t T)
              * builds that follow.  The rounding error of a difference build scales with the
              * distance of its density from the reference's, and with it the noise of its energy:
              * once the DIIS error has fallen below mixed_reref times its value at the reference,
-             * the next build is a full FP64 one, the new reference.  Convergence among
-             * difference builds is confirmed by a full FP64 build, which ends the SCF if its
-             * energy agrees within ene_tol */
+             * the next build is a full FP64 one, the new reference.  Energies agreeing within
+             * ene_tol end the SCF when the later one is a full FP64 build, or the difference
+             * build right after one (its difference density is that one step's, its energy the
+             * reference arithmetic's to rounding); agreement among later difference builds is
+             * confirmed by a full FP64 build */
             if (phase == YGPU_FOCK_FP64) {
                 /* a reference build, or a confirming one whose energy disagreed: difference
                  * builds follow (an agreeing confirmation ends the loop) */
                 diis_at_ref = T->diis_error >= 0.0 ? T->diis_error : DBL_MAX;
+                since_ref = 0;
                 if (delta >= T->ene_tol) T->mixed_phase = YGPU_FOCK_INCR;
             } else if (delta < T->ene_tol) {
-                T->mixed_phase = YGPU_FOCK_FP64;
-                delta = 223.0;
-            } else if (T->diis_error >= 0.0 && T->diis_error < T->mixed_reref * diis_at_ref) {
-                T->mixed_phase = YGPU_FOCK_FP64;
+                if (++since_ref > 1) {
+                    T->mixed_phase = YGPU_FOCK_FP64;
+                    delta = 223.0;
+                }
+            } else {
+                since_ref++;
+                if (T->diis_error >= 0.0 && T->diis_error < T->mixed_reref * diis_at_ref) T->mixed_phase = YGPU_FOCK_FP64;
             }
         }
         T->iter++;

@@ -113,10 +113,13 @@ struct TinySCF_struct
     int    mixed_schedule;  /* phases of a mixed run: bit 0 BF16x3 products, bit 1 BF16x6
                                products (FP32-accurate), in that order, then FP64; bit 2: the
                                FP64 phase runs difference-density builds against a full FP64
-                               reference build and confirms convergence with a full FP64
-                               build; default 5 */
+                               reference build, converged when two energies agree and the later
+                               is a full FP64 build or the difference build right after one;
+                               default 5 */
     int    mixed_keep_diis; /* keep the DIIS history across a phase change (bit 0: BF16x3 ->
-                               BF16x6, bit 1: -> FP64) instead of restarting it; default 0 */
+                               BF16x6, bit 1: -> FP64) instead of restarting it; default 2 (the
+                               reduced-precision Fock matrices stay in the history of the FP64 and
+                               difference builds, which replace them as it fills) */
     double mixed_tol;       /* DIIS error (max |R|) that ends the mixed phase, default 1e-5 */
     double mixed_reref;     /* difference-density phase: a new FP64 reference build once the
                                DIIS error is below this fraction of its value at the last

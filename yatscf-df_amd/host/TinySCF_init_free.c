@@ -63,7 +63,7 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
         if (e && *e && atof(e) > 0.0) T->mixed_tol = atof(e);
     }
     T->mixed_schedule = env_int("YATSCF_MIXED_SCHEDULE", 5);
-    T->mixed_keep_diis = env_int("YATSCF_MIXED_KEEP_DIIS", 0);
+    T->mixed_keep_diis = env_int("YATSCF_MIXED_KEEP_DIIS", 2);
     T->mixed6_floor = 1.5 * 4.76837158203125e-07;     /* 1.5 * 2^-21 */
     T->mixed_reref = 0.02;
     {
