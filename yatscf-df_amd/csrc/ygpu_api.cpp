@@ -611,7 +611,7 @@ try {
     long ks = 0;
     for (int i = 0; i < nbf; ++i) {
         kofs[i] = (int) ks;
-        ks += (xptr[i + 1] - xptr[i] + 7) / 8 * 8;
+        ks += (xptr[i + 1] - xptr[i] + 31) / 32 * 32;
     }
     kofs[nbf] = (int) ks;
     c->ksum = ks;

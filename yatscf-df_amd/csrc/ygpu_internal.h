@@ -97,8 +97,9 @@ struct ygpu_ctx {
     DBuf d_upmask, d_up_i, d_up_j;
     DBuf d_xptr, d_xup, d_xpartner;  // per bf i: up rows and partners (full CSR order)
     int max_row_len = 0;
-    // k-contiguous operand blocks of the two-part BF16 X build: block i holds L_i = (row length
-    // of i rounded up to 8) values per row and starts kofs[i] * (rows) elements into its part
+    // panel-major operand blocks of the two-part BF16 X build: block i holds L_i = (row length
+    // of i rounded up to 32) values per row, in 32-deep k panels, and starts kofs[i] * (rows)
+    // elements into its part
     DBuf d_kofs;
     DBuf d_xorder;                // basis functions by decreasing row length (X build order)
     long ksum = 0;                // sum of L_i
@@ -238,8 +239,9 @@ int yg_to_f32(hipStream_t s, const double *in, float *out, long n);
 // X (nbf x n_occ naux_pad) is stored pre-split into nparts (2 or 3) BF16 arrays, one after the
 // other (part q of element e at q * nbf * n_occ * naux_pad + e); the X build writes it, the
 // SYRK reads it
-// dst[q][kofs_i R + r L_i + kk] = BF16 part q (of 2) of src[idx[xptr_i + kk]][r], r < R, kk < L_i
-// (zero for kk >= row length): the gathered rows of every basis function i transposed
+// dst[q][kofs_i R + (kk / 32) 32 R + 32 r + kk % 32] = BF16 part q (of 2) of src[idx[xptr_i + kk]][r],
+// r < R, kk < L_i (zero for kk >= row length): the gathered rows of every basis function i
+// transposed into 32-deep k panels
 // spherical option: T of degree l (ncart(l) x (2l+1), row-major); the aux column tables and
 // naux (spherical) from c->aux; out = in T over the aux columns; A[k][k] += shift, k >= n_orth
 int yg_sph_coefficients(int l, double *T);
@@ -248,7 +250,7 @@ int yg_aux_sph_cols(ygpu_ctx *c, hipStream_t s, long rows, const double *in, lon
 int yg_shift_tail_diag(hipStream_t s, int n, int n_orth, double *A, double shift);
 int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
                       const int *xptr, const int *kofs, void *dst, size_t dpart);
-// X (two BF16 parts, nrow rows per basis function) from the k-contiguous blocks Cp (nrow rows)
+// X (two BF16 parts, nrow rows per basis function) from the panel-major blocks Cp (nrow rows)
 // and Bt (naux_pad rows) made by yg_gather_split_t; order: the basis functions, longest row first
 int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs, const int *order,
                  const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X);

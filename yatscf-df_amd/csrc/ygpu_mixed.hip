@@ -334,13 +334,15 @@ struct SLoad {
     }
 };
 
-// ---- two-part X build from k-contiguous blocks.  For basis function i with klen_i pair rows,
-// L_i = klen_i rounded up to 8 and kofs_i = sum_{j<i} L_j.  k_gather_split_t writes, for each
-// row r < R of the source and kk < L_i, the two BF16 parts of src[idx[xptr_i + kk]][r] at
-// dst[q * dpart + kofs_i * R + r * L_i + kk] (zero for kk >= klen_i): the gathered rows
-// transposed so that k runs contiguously, the order the matrix-core operands are read in.  B
-// (idx = the pair rows) is laid out once per DF tensor, C_occ (idx = the partners) once per
-// build; the X build then reads 16-byte groups of both operands with no gather and no split.
+// ---- two-part X build from panel-major operand blocks.  For basis function i with klen_i pair
+// rows, L_i = klen_i rounded up to 32 and kofs_i = sum_{j<i} L_j.  Block i of a source with R
+// rows (rows r of the operand, k = the pair rows of i) holds the two BF16 parts of
+// src[idx[xptr_i + kk]][r] at dst[q * dpart + kofs_i * R + (kk / 32) * 32 R + 32 r + kk % 32]
+// (zero for kk >= klen_i): 32-deep k panels, each panel the R rows one after the other.  B (idx =
+// the pair rows) is laid out once per DF tensor, C_occ (idx = the partners) once per build.  A
+// 32-deep stage of any row range is then one contiguous stretch of memory, which the X build
+// reads with whole cache lines (k-contiguous rows read 32 or 64 bytes per row and kept the
+// texture path, not the matrix cores, busy).
 __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict__ src, long ld, int R,
                                                         const int *__restrict__ idx, const int *__restrict__ xptr,
                                                         const int *__restrict__ kofs, __bf16 *__restrict__ dst,
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict_
 {
     // workgroup (row tile of 64, k tile of 64, function i); k tiles beyond L_i have nothing to do
     const int i = blockIdx.z, r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
-    const int kb = xptr[i], klen = xptr[i + 1] - kb, L = (klen + 7) & ~7;
+    const int kb = xptr[i], klen = xptr[i + 1] - kb, L = (klen + 31) & ~31;
     if (k0 >= L) return;
     __bf16 *d0 = dst + (size_t) kofs[i] * R, *d1 = d0 + dpart;
     __shared__ float tile[64][65];
@@ -372,21 +374,35 @@ __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict_
             h[e] = (__bf16) x;
             l[e] = (__bf16) (x - (float) h[e]);
         }
-        *(v8bf *) (d0 + (size_t) r * L + k) = h;
-        *(v8bf *) (d1 + (size_t) r * L + k) = l;
+        const size_t off = (size_t) (k >> 5) * R * 32 + (size_t) r * 32 + (k & 31);
+        *(v8bf *) (d0 + off) = h;
+        *(v8bf *) (d1 + off) = l;
     }
 }
 
-// X_i (nrow x naux_pad, two parts) = Cp_i Bt_i^T over k < L_i; workgroup (64 WM) x (64 WN)
-template <int WM, int WN, int BK, int NBUF>
-__global__ __launch_bounds__(64 * WM * WN) void k_xbuild_ps(const int *__restrict__ xptr,
-                                                            const int *__restrict__ kofs,
-                                                            const int *__restrict__ order,
-                                                            const __bf16 *__restrict__ Cp, size_t cpart,
-                                                            const __bf16 *__restrict__ Bt, size_t btpart,
-                                                            __bf16 *__restrict__ Xp, size_t xpart, int nrow,
-                                                            int naux_pad, int ncolt, int nrowt, int nbf)
+#define YG_LDSP(p) ((__attribute__((address_space(3))) void *) (p))
+
+// X_i (nrow x naux_pad, two parts) = Cp_i Bt_i^T over k < L_i.  Four wavefronts (2 x 2), each
+// with a (32 BI) x (32 BJ) register tile of 32x32 MFMA blocks (BI = 5, BJ = 3: 240 accumulators,
+// one wavefront per SIMD), so every 16-byte LDS operand read feeds BJ (or BI) block products.
+// A stage is one 32-deep k panel of the (64 BI)-row A range and the (64 BJ)-row B range, copied
+// global -> LDS by the texture path's direct loads (global_load_lds_dwordx4, no registers), each
+// wavefront instruction one contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1
+// issued while stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR
+// (r >> 1) & 3), which spreads the rows of one operand read over all banks.  Rows beyond the
+// operand are read from its last row (their results are not stored); k beyond klen_i is zero.
+template <int BI, int BJ, int NS>
+__global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xptr, const int *__restrict__ kofs,
+                                                      const int *__restrict__ order, const __bf16 *__restrict__ Cp,
+                                                      size_t cpart, const __bf16 *__restrict__ Bt, size_t btpart,
+                                                      __bf16 *__restrict__ Xp, size_t xpart, int nrow, int naux_pad,
+                                                      int ncolt, int nrowt, int nbf)
 {
+    constexpr int BM = 64 * BI, BN = 64 * BJ;
+    constexpr int SA = 2 * BM * 4, SB = 2 * BN * 4, SLOTS = SA + SB, PW = SLOTS / 256;   // 16-byte slots
+    static_assert(SLOTS % 256 == 0, "a stage is whole wavefront instructions of every wavefront");
+    static_assert((NS - 2) * PW <= 63, "vmcnt range");
+    __shared__ __attribute__((aligned(16))) v8bf ring[NS][SLOTS];
     // basis functions in order of decreasing row length (order[]), dealt round-robin over the 8
     // XCDs (the hardware deals workgroup b to XCD b % 8), every tile of the o-th function on
     // XCD o % 8: the row lengths, which vary 4-5x between functions and run in long stretches
@@ -398,22 +414,98 @@ __global__ __launch_bounds__(64 * WM * WN) void k_xbuild_ps(const int *__restric
     if (o >= nbf) return;
     const int i = order[o];
     const int rem = (int) (q % tiles), ct = rem / nrowt, rt = rem % nrowt;
-    const int L = (xptr[i + 1] - xptr[i] + 7) & ~7;
+    const int L = (xptr[i + 1] - xptr[i] + 31) & ~31;
     const size_t ko = (size_t) kofs[i];
-    const int m0 = rt * 64 * WM, n0 = ct * 64 * WN;
-    SLoad<2, BK> la{Cp + ko * nrow, cpart, L, nrow, m0};
-    SLoad<2, BK> lb{Bt + ko * naux_pad, btpart, L, naux_pad, n0};
-    v16f acc[2][2];
-    bfs_tile<2, WM, WN, BK, NBUF, true>(la, lb, 0, L, acc, nrow - m0, naux_pad - n0);
-    const size_t xo = (size_t) i * nrow * naux_pad;
-    bfs_epilogue<WN>(acc, [&](int r, int c, float v) {
-        const int m = m0 + r, n = n0 + c;
-        if (m < nrow && n < naux_pad) {
-            const __bf16 h = (__bf16) v;
-            Xp[xo + (size_t) m * naux_pad + n] = h;
-            Xp[xpart + xo + (size_t) m * naux_pad + n] = (__bf16) (v - (float) h);
+    const int m0 = rt * BM, n0 = ct * BN;
+    const __bf16 *Ab = Cp + ko * nrow, *Bb = Bt + ko * naux_pad;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 32 * BI, wn = (w & 1) * 32 * BJ;
+    // slot t of an operand part: row t / 4, swizzled group (t % 4); source group g = that XOR
+    auto issue = [&](int slot, int p) {
+        v8bf *dst = ring[slot];
+#pragma unroll
+        for (int jj = 0; jj < PW; ++jj) {
+            const int j = 4 * jj + w, sl = 64 * j + lane;
+            const __bf16 *src;
+            if (sl < SA) {
+                const int part = sl / (BM * 4), t = sl % (BM * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
+                src = Ab + part * cpart + (size_t) p * nrow * 32 + (size_t) min(m0 + r, nrow - 1) * 32 + 8 * g;
+            } else {
+                const int s2 = sl - SA;
+                const int part = s2 / (BN * 4), t = s2 % (BN * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
+                src = Bb + part * btpart + (size_t) p * naux_pad * 32 + (size_t) min(n0 + r, naux_pad - 1) * 32 + 8 * g;
+            }
+            __builtin_amdgcn_global_load_lds((const void *) src, YG_LDSP(dst + 64 * j), 16, 0, 0);
         }
-    });
+    };
+    v16f acc[BI][BJ];
+#pragma unroll
+    for (int a = 0; a < BI; ++a)
+#pragma unroll
+        for (int b = 0; b < BJ; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    const int rl = lane & 31, hk = lane >> 5;
+    const int nst = L / 32;
+    for (int p = 0; p < NS - 1 && p < nst; ++p) issue(p, p);
+    for (int st = 0; st < nst; ++st) {
+        // stage st landed (the later stages issued so far may still be in flight), every LDS
+        // read of this wavefront done, then the workgroup: the ring slot written next was read
+        // by all wavefronts in the previous stage
+        const int later = min(NS - 2, nst - 1 - st);
+        asm volatile("" ::: "memory");
+        switch (later) {
+#define YG_WAIT(n) \
+    case n: __builtin_amdgcn_s_waitcnt((((n) * PW) & 15) | ((((n) * PW) >> 4) << 14) | (7 << 4)); break;
+            YG_WAIT(0) YG_WAIT(1) YG_WAIT(2) YG_WAIT(3)
+#undef YG_WAIT
+        }
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (st + NS - 1 < nst) issue((st + NS - 1) % NS, st + NS - 1);
+        const v8bf *sb = ring[st % NS];
+        const v8bf *sa0 = sb, *sa1 = sb + BM * 4, *sb0 = sb + SA, *sb1 = sb0 + BN * 4;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v8bf a[BI][2], b[BJ][2];
+#pragma unroll
+            for (int x = 0; x < BI; ++x) {
+                const int r = wm + 32 * x + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
+                a[x][0] = sa0[off];
+                a[x][1] = sa1[off];
+            }
+#pragma unroll
+            for (int y = 0; y < BJ; ++y) {
+                const int r = wn + 32 * y + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
+                b[y][0] = sb0[off];
+                b[y][1] = sb1[off];
+            }
+            // lo*hi, hi*lo, hi*hi: the order of the other two-part products
+#pragma unroll
+            for (int x = 0; x < BI; ++x)
+#pragma unroll
+                for (int y = 0; y < BJ; ++y) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][1], b[y][0], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][1], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][0], acc[x][y], 0, 0, 0);
+                }
+        }
+    }
+    const size_t xo = (size_t) i * nrow * naux_pad;
+#pragma unroll
+    for (int x = 0; x < BI; ++x)
+#pragma unroll
+        for (int y = 0; y < BJ; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int m = m0 + wm + 32 * x + 8 * (v >> 2) + 4 * hk + (v & 3), n = n0 + wn + 32 * y + rl;
+                if (m < nrow && n < naux_pad) {
+                    const float val = acc[x][y][v];
+                    const __bf16 h = (__bf16) val;
+                    Xp[xo + (size_t) m * naux_pad + n] = h;
+                    Xp[xpart + xo + (size_t) m * naux_pad + n] = (__bf16) (val - (float) h);
+                }
+            }
 }
 
 constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
@@ -576,11 +668,11 @@ int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src,
     return YGPU_OK;
 }
 
-template <int WM, int WN, int BK, int NBUF>
+template <int BI, int BJ, int NS>
 static int launch_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs,
                             const int *order, const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
 {
-    const int ncolt = (naux_pad + 64 * WN - 1) / (64 * WN), nrowt = (nrow + 64 * WM - 1) / (64 * WM);
+    const int ncolt = (naux_pad + 64 * BJ - 1) / (64 * BJ), nrowt = (nrow + 64 * BI - 1) / (64 * BI);
     const size_t xpart = (size_t) nbf * nrow * naux_pad;
     // 8 XCD lanes of ceil(nbf / 8) functions each (functions past nbf: empty workgroups)
     const long total = 8L * ((nbf + 7) / 8) * ncolt * nrowt;
@@ -588,8 +680,8 @@ static int launch_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, cons
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL((k_xbuild_ps<WM, WN, BK, NBUF>), dim3((unsigned) total), dim3(64 * WM * WN), 0, s, xptr, kofs,
-                       order, (const __bf16 *) Cp, cpart, (const __bf16 *) Bt, btpart, (__bf16 *) X, xpart, nrow, naux_pad,
+    hipLaunchKernelGGL((k_xbuild_ps<BI, BJ, NS>), dim3((unsigned) total), dim3(256), 0, s, xptr, kofs, order,
+                       (const __bf16 *) Cp, cpart, (const __bf16 *) Bt, btpart, (__bf16 *) X, xpart, nrow, naux_pad,
                        ncolt, nrowt, nbf);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
@@ -599,14 +691,14 @@ int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr
                  const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
 {
     if (nbf <= 0 || nrow <= 0) return YGPU_OK;
-#define YG_XPS(WM, WN) \
-    launch_xbuild_ps<WM, WN, 32, 1>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
-    // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD)
-    if (nrow <= 64) return YG_XPS(1, 4);
-    if (nrow <= 128) return YG_XPS(2, 4);
-    if (nrow <= 192) return YG_XPS(3, 4);
-    if (nrow <= 256) return YG_XPS(4, 4);
-    return YG_XPS(5, 3);
+#define YG_XPS(BI) launch_xbuild_ps<BI, 3, 2>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
+    // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD); two ring
+    // stages (a deeper ring measured no faster at 320 x 192)
+    if (nrow <= 64) return YG_XPS(1);
+    if (nrow <= 128) return YG_XPS(2);
+    if (nrow <= 192) return YG_XPS(3);
+    if (nrow <= 256) return YG_XPS(4);
+    return YG_XPS(5);
 #undef YG_XPS
 }
 
