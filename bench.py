@@ -86,15 +86,19 @@ def host_info():
 
 
 def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` from its committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
-    name = "%s_%s_traffic.md" % (kernel, workload)
-    path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summaries
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size): the line
+    "traffic_bytes_per_launch <kernel> <workload> ... = <bytes>" of a profiles/*_traffic.md file."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
         return None, None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split("=")[1]), "profiles/" + name
+    for name in sorted(os.listdir(d)):
+        if not name.endswith("_traffic.md"):
+            continue
+        for line in open(os.path.join(d, name)):
+            t = line.split()
+            if len(t) >= 4 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel and t[2] == workload:
+                return float(line.split("=")[1]), "profiles/" + name
     return None, None
 
 
