@@ -168,7 +168,9 @@ def kernel_table(runs, mixed):
             # two parts: from the k-contiguous operand blocks; three: the gathering X build
             xname = "k_xbuild_ps" if kind == FOCK_MIXED else "k_xbuild_bf"
             add("%s (X = C_occ^T B, %s)" % (xname, tag), "mfma", xb, n, xb_flops, peak, "TFLOP/s", tag + ", f32 acc")
-            add("k_syrk_bf (K = X X^T, %s)" % tag, "mfma", sy, n, syrk_flops, peak, "TFLOP/s", tag + ", f32 acc")
+            # two parts: the panel engine over the panel-major X; three: 128 x 128 tiles
+            sname = "k_pgemm_ps" if kind == FOCK_MIXED else "k_syrk_bf"
+            add("%s (K = X X^T, %s)" % (sname, tag), "mfma", sy, n, syrk_flops, peak, "TFLOP/s", tag + ", f32 acc")
             add("J build (%s iterations, FP32 tensor, two passes)" % tag, "hbm", jj, n, j32, HBM_PEAK_GBS, "GB/s",
                 "f32 tensor, f64 sums")
         elif kind == FOCK_INCR:
@@ -176,7 +178,7 @@ def kernel_table(runs, mixed):
             # X^P and X^M together: the X build over 2 n_occ columns; G = X^M (X^P)^T: every tile
             add("k_xbuild_ps (X^[P|M], 2 n_occ columns, BF16x3 split)", "mfma", xb, n, 2.0 * xb_flops, peak, "TFLOP/s",
                 "BF16x3 split, f32 acc")
-            add("k_gemm_bf_nt (G = X^M X^P^T, difference K, BF16x3 split)", "mfma", sy, n,
+            add("k_pgemm_ps (G = X^M X^P^T, difference K, BF16x3 split)", "mfma", sy, n,
                 2.0 * nbf * nbf * n_occ * naux, peak, "TFLOP/s", "BF16x3 split, f32 acc")
             add("J build (difference iterations, FP32 tensor, two passes)", "hbm", jj, n, j32, HBM_PEAK_GBS, "GB/s",
                 "f32 tensor, f64 sums")

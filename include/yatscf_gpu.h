@@ -66,6 +66,17 @@ int ygpu_device_count(int *count);
 int ygpu_create(int device, ygpu_ctx_t **ctx);
 int ygpu_destroy(ygpu_ctx_t *ctx);
 
+/* No counterpart in the reference (its buffers are host memory).  Device
+ * blocks of 1 MiB and more that a destroyed engine (or a regrown buffer)
+ * gives up stay in a process-wide cache and are handed to later engines on
+ * the same device (best fit, n to 2 n bytes), so repeated solves do not pay
+ * the driver's allocation of tens of GB each time.  The cache holds at most
+ * half the device's memory (the largest blocks leave first), and an
+ * allocation that fails empties it and tries again.  This returns every
+ * cached block to the driver (call it with no engine in use on another
+ * thread). */
+int ygpu_release_cached_memory(void);
+
 /* Page-locked host memory for the matrices the drop-in exchanges with the
  * device on every SCF iteration (F, D, J, K, C_occ, eigenvalues, X, Hcore, S)
  * and for J^{-1/2}: the reference allocates them with ALIGN64B_MALLOC

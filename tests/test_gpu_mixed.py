@@ -229,3 +229,23 @@ def test_difference_density_fock(ygpu):
     assert err < 2.0 ** -12 * dK
     assert err_rot < 1e-12 * np.abs(ref0["K"]).max()
     assert np.allclose(inc["K"], inc["K"].T, rtol=0, atol=0)
+
+
+def test_repeated_solves_reuse_device_blocks(ygpu):
+    """Consecutive mixed-precision solves in one process: the second takes its large device
+    buffers (FP32 tensor, operand blocks, X, k-chunk partials) from the cache of blocks the
+    first engine gave back, the third runs after the cache went back to the driver; all three
+    reach the same energy in the same iterations (the blocks hold the previous solve's data,
+    which no kernel may read)."""
+    name = "c10h22"
+    log("case: repeated mixed solves", name)
+    runs = []
+    for k in range(3):
+        if k == 2:
+            ygpu.release_cached_memory()
+        r = ygpu.solve(xyz_path(name), ene_tol=1e-10, mixed=True)
+        log("  solve %d: %.3f s, %d iterations, E %.12f" % (k, r["seconds"], r["iters"], r["energy"]))
+        runs.append(r)
+    for r in runs[1:]:
+        assert r["converged"] and r["iters"] == runs[0]["iters"]
+        assert abs(r["energy"] - runs[0]["energy"]) < 1e-10

@@ -19,6 +19,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -34,30 +37,133 @@ void ygpu_set_error(int code, const char *fmt, ...)
     g_err = std::string("[ygpu ") + std::to_string(code) + "] " + buf;
 }
 
+namespace {
+
+constexpr size_t CACHE_MIN = 1u << 20;   // blocks below this go straight back to the driver
+
+struct BlockCache {
+    std::mutex mu;
+    std::map<int, std::multimap<size_t, void *>> free;   // device -> (size, block)
+    std::map<int, size_t> held, cap;                      // cached bytes; at most half the device's memory
+};
+
+BlockCache &block_cache()
+{
+    static BlockCache *g = new BlockCache;   // never destroyed: no device call at process exit
+    return *g;
+}
+
+// a cached block of device dev with n <= size <= 2 n (the smallest such), or nullptr
+void *cache_take(int dev, size_t n, size_t *got)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.mu);
+    auto &m = bc.free[dev];
+    auto it = m.lower_bound(n);
+    if (it == m.end() || it->first / 2 > n) return nullptr;
+    void *p = it->second;
+    *got = it->first;
+    bc.held[dev] -= it->first;
+    m.erase(it);
+    return p;
+}
+
+// called with device dev current
+void cache_put(int dev, void *p, size_t n)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.mu);
+    size_t &cap = bc.cap[dev];
+    if (!cap) {
+        size_t fr = 0, tot = 0;
+        cap = hipMemGetInfo(&fr, &tot) == hipSuccess && tot ? tot / 2 : ((size_t) 64 << 30);
+    }
+    auto &m = bc.free[dev];
+    size_t &held = bc.held[dev];
+    // the largest cached blocks go back to the driver first
+    while (!m.empty() && held + n > cap) {
+        auto last = std::prev(m.end());
+        (void) hipFree(last->second);
+        held -= last->first;
+        m.erase(last);
+    }
+    if (held + n > cap) {
+        (void) hipFree(p);
+        return;
+    }
+    m.emplace(n, p);
+    held += n;
+}
+
+// hipFree every cached block of device dev (all devices for dev < 0)
+void cache_drain(int dev)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.mu);
+    int cur = 0;
+    (void) hipGetDevice(&cur);
+    for (auto &kv : bc.free) {
+        if (dev >= 0 && kv.first != dev) continue;
+        (void) hipSetDevice(kv.first);
+        for (auto &b : kv.second) (void) hipFree(b.second);
+        kv.second.clear();
+        bc.held[kv.first] = 0;
+    }
+    (void) hipSetDevice(cur);
+}
+
+}  // namespace
+
 int DBuf::ensure(size_t n)
 {
     if (n <= bytes && p) return YGPU_OK;
-    if (p) {
-        (void) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
+    release();
     if (n == 0) n = 8;
+    int d = 0;
+    YG_HIP(hipGetDevice(&d));
+    if (n >= CACHE_MIN) {
+        size_t got = 0;
+        if (void *q = cache_take(d, n, &got)) {
+            p = q;
+            bytes = got;
+            dev = d;
+            return YGPU_OK;
+        }
+    }
     hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        cache_drain(d);
+        e = hipMalloc(&p, n);
+    }
     if (e != hipSuccess) {
         p = nullptr;
         ygpu_set_error(YGPU_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
         return YGPU_ERR_NOMEM;
     }
     bytes = n;
+    dev = d;
     return YGPU_OK;
 }
 
 void DBuf::release()
 {
-    if (p) (void) hipFree(p);
+    if (p) {
+        int cur = 0;
+        (void) hipGetDevice(&cur);
+        if (cur != dev) (void) hipSetDevice(dev);
+        if (bytes >= CACHE_MIN) cache_put(dev, p, bytes);
+        else (void) hipFree(p);
+        if (cur != dev) (void) hipSetDevice(cur);
+    }
     p = nullptr;
     bytes = 0;
+}
+
+int ygpu_release_cached_memory(void)
+{
+    cache_drain(-1);
+    return YGPU_OK;
 }
 
 namespace {

@@ -43,9 +43,16 @@ void ygpu_set_error(int code, const char *fmt, ...);
 #define YG_SYTRD_BLOCKED_NMAX 2048
 
 // growable device buffer, freed when released or destroyed (not copyable)
+// Device buffer.  Blocks of 1 MiB and more come from and go back to a process-wide cache of
+// freed device blocks (per device, best fit between n and 2 n bytes), so the large buffers of one
+// SCF are reused by the next one on the same device instead of going back to the driver, whose
+// allocations of tens of GB after a free of as much took from 0.03 to 2 s at random.  The cache
+// holds at most half the device's memory; ygpu_release_cached_memory returns it to the driver,
+// and an allocation that fails empties the device's cache and tries once more.
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    int dev = 0;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
@@ -57,6 +64,7 @@ struct DBuf {
     {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(dev, o.dev);
     }
 };
 

@@ -382,63 +382,53 @@ __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict_
 
 #define YG_LDSP(p) ((__attribute__((address_space(3))) void *) (p))
 
-// X_i (nrow x naux_pad, two parts) = Cp_i Bt_i^T over k < L_i.  Four wavefronts (2 x 2), each
-// with a (32 BI) x (32 BJ) register tile of 32x32 MFMA blocks (BI = 5, BJ = 3: 240 accumulators,
-// one wavefront per SIMD), so every 16-byte LDS operand read feeds BJ (or BI) block products.
-// A stage is one 32-deep k panel of the (64 BI)-row A range and the (64 BJ)-row B range, copied
-// global -> LDS by the texture path's direct loads (global_load_lds_dwordx4, no registers), each
-// wavefront instruction one contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1
-// issued while stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR
-// (r >> 1) & 3), which spreads the rows of one operand read over all banks.  Rows beyond the
-// operand are read from its last row (their results are not stored); k beyond klen_i is zero.
-template <int BI, int BJ, int NS>
-__global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xptr, const int *__restrict__ kofs,
-                                                      const int *__restrict__ order, const __bf16 *__restrict__ Cp,
-                                                      size_t cpart, const __bf16 *__restrict__ Bt, size_t btpart,
-                                                      __bf16 *__restrict__ Xp, size_t xpart, int nrow, int naux_pad,
-                                                      int ncolt, int nrowt, int nbf)
-{
-    constexpr int BM = 64 * BI, BN = 64 * BJ;
-    constexpr int SA = 2 * BM * 4, SB = 2 * BN * 4, SLOTS = SA + SB, PW = SLOTS / 256;   // 16-byte slots
+// LDS ring of the panel engine: NS stages of one 32-deep k panel of the (64 BI)-row A range and
+// the (64 BJ)-row B range, two BF16 parts each, in 16-byte slots
+template <int BI, int BJ> struct PsStage {
+    static constexpr int BM = 64 * BI, BN = 64 * BJ;
+    static constexpr int SA = 2 * BM * 4, SB = 2 * BN * 4, SLOTS = SA + SB, PW = SLOTS / 256;
     static_assert(SLOTS % 256 == 0, "a stage is whole wavefront instructions of every wavefront");
+};
+
+// Panel engine (ring: NS stages of PsStage<BI, BJ>::SLOTS slots in LDS): acc = A B^T for one
+// (64 BI) x (64 BJ) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.  A and B are panel-major: panel p of an operand with R rows holds
+// the R rows one after the other, 32 k each, at p * 32 R; its two BF16 parts are apart / bpart
+// elements apart.  Four wavefronts (2 x 2), each with a (32 BI) x (32 BJ) register tile of 32x32
+// MFMA blocks (BI = 5, BJ = 3: 240 accumulators, one wavefront per SIMD), so every 16-byte LDS
+// operand read feeds BJ (or BI) block products.  A stage is copied global -> LDS by the texture
+// path's direct loads (global_load_lds_dwordx4, no registers), each wavefront instruction one
+// contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while stage s is
+// multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
+// rows of one operand read over all banks.  Rows beyond an operand are read from its last row
+// (their results are not stored).
+template <int BI, int BJ, int NS>
+__device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A,
+                                        size_t apart, int RA, int m0, const __bf16 *__restrict__ B, size_t bpart,
+                                        int RB, int n0, int nst, v16f (&acc)[BI][BJ])
+{
+    using P = PsStage<BI, BJ>;
+    constexpr int BM = P::BM, BN = P::BN, SA = P::SA, PW = P::PW;
     static_assert((NS - 2) * PW <= 63, "vmcnt range");
-    __shared__ __attribute__((aligned(16))) v8bf ring[NS][SLOTS];
-    // basis functions in order of decreasing row length (order[]), dealt round-robin over the 8
-    // XCDs (the hardware deals workgroup b to XCD b % 8), every tile of the o-th function on
-    // XCD o % 8: the row lengths, which vary 4-5x between functions and run in long stretches
-    // along the basis, even out over the XCDs, the longest run first, and the column tiles of a
-    // function and the row tiles of a column tile share its blocks in one L2
-    const long q = blockIdx.x / 8;
-    const int tiles = ncolt * nrowt;
-    const int o = (int) (q / tiles) * 8 + (int) (blockIdx.x % 8);
-    if (o >= nbf) return;
-    const int i = order[o];
-    const int rem = (int) (q % tiles), ct = rem / nrowt, rt = rem % nrowt;
-    const int L = (xptr[i + 1] - xptr[i] + 31) & ~31;
-    const size_t ko = (size_t) kofs[i];
-    const int m0 = rt * BM, n0 = ct * BN;
-    const __bf16 *Ab = Cp + ko * nrow, *Bb = Bt + ko * naux_pad;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w >> 1) * 32 * BI, wn = (w & 1) * 32 * BJ;
     // slot t of an operand part: row t / 4, swizzled group (t % 4); source group g = that XOR
     auto issue = [&](int slot, int p) {
-        v8bf *dst = ring[slot];
+        v8bf *dst = ring + slot * P::SLOTS;
 #pragma unroll
         for (int jj = 0; jj < PW; ++jj) {
             const int j = 4 * jj + w, sl = 64 * j + lane;
             const __bf16 *src;
             if (sl < SA) {
                 const int part = sl / (BM * 4), t = sl % (BM * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
-                src = Ab + part * cpart + (size_t) p * nrow * 32 + (size_t) min(m0 + r, nrow - 1) * 32 + 8 * g;
+                src = A + part * apart + (size_t) p * RA * 32 + (size_t) min(m0 + r, RA - 1) * 32 + 8 * g;
             } else {
                 const int s2 = sl - SA;
                 const int part = s2 / (BN * 4), t = s2 % (BN * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
-                src = Bb + part * btpart + (size_t) p * naux_pad * 32 + (size_t) min(n0 + r, naux_pad - 1) * 32 + 8 * g;
+                src = B + part * bpart + (size_t) p * RB * 32 + (size_t) min(n0 + r, RB - 1) * 32 + 8 * g;
             }
             __builtin_amdgcn_global_load_lds((const void *) src, YG_LDSP(dst + 64 * j), 16, 0, 0);
         }
     };
-    v16f acc[BI][BJ];
 #pragma unroll
     for (int a = 0; a < BI; ++a)
 #pragma unroll
@@ -446,7 +436,6 @@ __global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xp
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
     const int rl = lane & 31, hk = lane >> 5;
-    const int nst = L / 32;
     for (int p = 0; p < NS - 1 && p < nst; ++p) issue(p, p);
     for (int st = 0; st < nst; ++st) {
         // stage st landed (the later stages issued so far may still be in flight), every LDS
@@ -463,7 +452,7 @@ __global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xp
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         if (st + NS - 1 < nst) issue((st + NS - 1) % NS, st + NS - 1);
-        const v8bf *sb = ring[st % NS];
+        const v8bf *sb = ring + (st % NS) * P::SLOTS;
         const v8bf *sa0 = sb, *sa1 = sb + BM * 4, *sb0 = sb + SA, *sb1 = sb0 + BN * 4;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
@@ -491,21 +480,108 @@ __global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xp
                 }
         }
     }
-    const size_t xo = (size_t) i * nrow * naux_pad;
+}
+
+// store(m, n, v) for every element of this wavefront's part of a ps_tile result (m, n relative
+// to the workgroup tile)
+template <int BI, int BJ, class F>
+__device__ __forceinline__ void ps_epilogue(const v16f (&acc)[BI][BJ], F &&store)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = (w >> 1) * 32 * BI, wn = (w & 1) * 32 * BJ, rl = lane & 31, hk = lane >> 5;
 #pragma unroll
     for (int x = 0; x < BI; ++x)
 #pragma unroll
         for (int y = 0; y < BJ; ++y)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int m = m0 + wm + 32 * x + 8 * (v >> 2) + 4 * hk + (v & 3), n = n0 + wn + 32 * y + rl;
-                if (m < nrow && n < naux_pad) {
-                    const float val = acc[x][y][v];
-                    const __bf16 h = (__bf16) val;
-                    Xp[xo + (size_t) m * naux_pad + n] = h;
-                    Xp[xpart + xo + (size_t) m * naux_pad + n] = (__bf16) (val - (float) h);
-                }
+            for (int v = 0; v < 16; ++v)
+                store(wm + 32 * x + 8 * (v >> 2) + 4 * hk + (v & 3), wn + 32 * y + rl, acc[x][y][v]);
+}
+
+// X_i (nrow x naux_pad) = Cp_i Bt_i^T over the L_i = round32(klen_i) pair rows of basis function
+// i (k beyond klen_i is zero in both blocks).  X is stored as two BF16 parts (xpart apart),
+// panel-major over the basis functions for the products that follow: element k = m naux_pad + n
+// of function i at (k / 32) * 32 nbf + 32 i + k % 32, so a 32-deep k panel of any range of
+// functions is one contiguous stretch.
+template <int BI, int BJ, int NS>
+__global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xptr, const int *__restrict__ kofs,
+                                                      const int *__restrict__ order, const __bf16 *__restrict__ Cp,
+                                                      size_t cpart, const __bf16 *__restrict__ Bt, size_t btpart,
+                                                      __bf16 *__restrict__ Xp, size_t xpart, int nrow, int naux_pad,
+                                                      int ncolt, int nrowt, int nbf)
+{
+    __shared__ __attribute__((aligned(16))) v8bf ring[NS][PsStage<BI, BJ>::SLOTS];
+    // basis functions in order of decreasing row length (order[]), dealt round-robin over the 8
+    // XCDs (the hardware deals workgroup b to XCD b % 8), every tile of the o-th function on
+    // XCD o % 8: the row lengths, which vary 4-5x between functions and run in long stretches
+    // along the basis, even out over the XCDs, the longest run first, and the column tiles of a
+    // function and the row tiles of a column tile share its blocks in one L2
+    const long q = blockIdx.x / 8;
+    const int tiles = ncolt * nrowt;
+    const int o = (int) (q / tiles) * 8 + (int) (blockIdx.x % 8);
+    if (o >= nbf) return;
+    const int i = order[o];
+    const int rem = (int) (q % tiles), ct = rem / nrowt, rt = rem % nrowt;
+    const int L = (xptr[i + 1] - xptr[i] + 31) & ~31;
+    const size_t ko = (size_t) kofs[i];
+    const int m0 = rt * 64 * BI, n0 = ct * 64 * BJ;
+    v16f acc[BI][BJ];
+    ps_tile<BI, BJ, NS>(&ring[0][0], Cp + ko * nrow, cpart, nrow, m0, Bt + ko * naux_pad, btpart, naux_pad, n0, L / 32, acc);
+    __bf16 *X0 = Xp + 32 * (size_t) i;
+    ps_epilogue<BI, BJ>(acc, [&](int r, int c, float val) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < nrow && n < naux_pad) {
+            const long k = (long) m * naux_pad + n;
+            const size_t off = (size_t) (k >> 5) * 32 * nbf + (k & 31);
+            const __bf16 h = (__bf16) val;
+            X0[off] = h;
+            X0[xpart + off] = (__bf16) (val - (float) h);
+        }
+    });
+}
+
+// FP32 partials of A B^T over k chunks, A and B panel-major with R rows (the products of the
+// two-part X: the SYRK X X^T and the difference product X^M X^P^T).  Tiles of (64 BI) x (64 BJ);
+// with UPPER only the tiles (rt, ct) holding an element m <= n (rt BM < (ct + 1) BN), else all
+// nrowt x ncolt.  Chunk z covers panels [z cp, min(npan, (z + 1) cp)) and writes its tiles'
+// elements (m, n < R) to part[z] (R x R, FP32).  The chunks are dealt to the XCDs (the hardware
+// deals workgroup b to XCD b % 8), every tile of chunk z on XCD z % 8 one after the other, so the
+// tiles of a chunk run side by side and read its panels from the one L2 (nchunk is a multiple of 8).
+template <int BI, int BJ, int NS, bool UPPER>
+__global__ __launch_bounds__(256, 1) void k_pgemm_ps(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
+                                                     size_t xpart, int R, long npan, long cp, int ncolt, int ntiles,
+                                                     float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) v8bf ring[NS][PsStage<BI, BJ>::SLOTS];
+    constexpr int BM = 64 * BI, BN = 64 * BJ;
+    const long s = blockIdx.x / 8;
+    const long z = (s / ntiles) * 8 + blockIdx.x % 8;
+    int tile = (int) (s % ntiles), rt = 0, ct = 0;
+    if (UPPER) {
+        // tile < ntiles = the count over rt < nrowt of the host's same sum: ends at such an rt
+        for (;; ++rt) {
+            const int c0 = rt * BM / BN, cnt = max(0, ncolt - c0);
+            if (tile < cnt) {
+                ct = c0 + tile;
+                break;
             }
+            tile -= cnt;
+        }
+    } else {
+        rt = tile / ncolt;
+        ct = tile % ncolt;
+    }
+    const long p0 = z * cp, p1 = std::min(npan, p0 + cp);
+    const int nst = p1 > p0 ? (int) (p1 - p0) : 0;
+    const size_t po = (size_t) (p0 < npan ? p0 : 0) * 32 * R;
+    const int m0 = rt * BM, n0 = ct * BN;
+    v16f acc[BI][BJ];
+    ps_tile<BI, BJ, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, acc);
+    float *P = part + (size_t) z * R * R;
+    ps_epilogue<BI, BJ>(acc, [&](int r, int c, float v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < R && n < R) P[(size_t) m * R + n] = v;
+    });
 }
 
 constexpr int ST = 128;   // SYRK tile edge (2 x 2 wavefronts)
@@ -581,18 +657,31 @@ __global__ __launch_bounds__(NTHR) void k_gemm_bf_nt(const __bf16 *__restrict__ 
     });
 }
 
-// K = Kref + (G + G^T) / 2 with G = sum_z part[z] (FP64 sums), one thread per upper element
-__global__ void k_incr_reduce(const float *__restrict__ part, int nsplit, int n, const double *__restrict__ Kref,
-                              double *__restrict__ K)
+// G = sum_z part[z] in FP64, one thread per element (every read a whole row of a part)
+__global__ void k_chunk_sum(const float *__restrict__ part, int nsplit, long nn, double *__restrict__ G)
+{
+    const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nn) return;
+    double s0 = 0.0, s1 = 0.0;
+    int z = 0;
+    for (; z + 1 < nsplit; z += 2) {
+        s0 += (double) part[(size_t) z * nn + idx];
+        s1 += (double) part[(size_t) (z + 1) * nn + idx];
+    }
+    if (z < nsplit) s0 += (double) part[(size_t) z * nn + idx];
+    G[idx] = s0 + s1;
+}
+
+// K = Kref + (G + G^T) / 2, one thread per upper element
+__global__ void k_incr_sym(const double *__restrict__ G, int n, const double *__restrict__ Kref,
+                           double *__restrict__ K)
 {
     const long idx = (long) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long) n * n) return;
     const int m = (int) (idx / n), q = (int) (idx % n);
     if (m > q) return;
     const size_t a = (size_t) m * n + q, b = (size_t) q * n + m;
-    double s = 0.0;
-    for (int z = 0; z < nsplit; ++z) s += (double) part[(size_t) z * n * n + a] + (double) part[(size_t) z * n * n + b];
-    const double v = Kref[a] + 0.5 * s;
+    const double v = Kref[a] + 0.5 * (G[a] + G[b]);
     K[a] = v;
     K[b] = v;
 }
@@ -702,6 +791,53 @@ int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr
 #undef YG_XPS
 }
 
+// K = Kref + (G + G^T) / 2 with G = sum_z part[z]; G (FP64) is kept in part after the nsplit
+// chunks, at the next multiple of 256 bytes (the caller sized part for it)
+static int incr_reduce(hipStream_t s, DBuf &part, int nsplit, int n, const double *Kref, double *K)
+{
+    const long nn = (long) n * n;
+    const size_t goff = ((size_t) nsplit * nn * sizeof(float) + 255) & ~(size_t) 255;
+    double *G = (double *) (part.as<char>() + goff);
+    hipLaunchKernelGGL(k_chunk_sum, dim3((nn + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nn, G);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_incr_sym, dim3((nn + 255) / 256), dim3(256), 0, s, G, n, Kref, K);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+// FP32 partials part[z] (z < *nchunk) of A B^T for the panel-major two-part X (R rows, kdim k,
+// parts xpart apart); UPPER: only the tiles holding the upper triangle
+template <bool UPPER>
+static int pgemm_ps(hipStream_t s, int R, long kdim, const __bf16 *A, const __bf16 *B, size_t xpart, DBuf &part,
+                    int *nchunk_out)
+{
+    constexpr int BI = 5, BJ = 3, NS = 2, BM = 64 * BI, BN = 64 * BJ;
+    const int nrowt = (R + BM - 1) / BM, ncolt = (R + BN - 1) / BN;
+    int ntiles = 0;
+    if (UPPER) {
+        for (int rt = 0; rt < nrowt; ++rt) ntiles += std::max(0, ncolt - rt * BM / BN);
+    } else {
+        ntiles = nrowt * ncolt;
+    }
+    const long npan = kdim / 32;
+    // FP32 accumulation over at most 256 panels (8192 k) per chunk, FP64 across chunks: longer
+    // FP32 sums raised the error of K measurably (the DIIS error of the BF16 phase levelled off
+    // ~25 % higher with 16-36 thousand k per chunk); the chunk count a multiple of 8
+    // (fewer, longer chunks where their partials would pass 8 GiB)
+    const long cap = std::max<long>(8, (long) ((8ull << 30) / ((size_t) R * R * sizeof(float))) / 8 * 8);
+    const int nchunk = (int) std::min<long>(cap, std::max<long>(8, ((npan + 255) / 256 + 7) / 8 * 8));
+    const int c = nchunk / 8;
+    const long cp = (npan + nchunk - 1) / nchunk;
+    // (and an R x R FP64 matrix after the partials for the reduction of the difference build)
+    YG_TRY(part.ensure((size_t) nchunk * R * R * sizeof(float) + (size_t) R * R * sizeof(double) + 256));
+    const long total = 8L * c * ntiles;
+    hipLaunchKernelGGL((k_pgemm_ps<BI, BJ, NS, UPPER>), dim3((unsigned) total), dim3(256), 0, s, A, B, xpart, R, npan,
+                       cp, ncolt, ntiles, part.as<float>());
+    YG_HIP(hipGetLastError());
+    *nchunk_out = nchunk;
+    return YGPU_OK;
+}
+
 template <int NP>
 static int syrk_np(hipStream_t s, int nbf, long kdim, const void *X, double *K, DBuf &part)
 {
@@ -733,7 +869,20 @@ int yg_syrk_bf(hipStream_t s, int nparts, int nbf, long kdim, const void *X, dou
         ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 8", kdim);
         return YGPU_ERR_INVALID;
     }
-    if (nparts == 2) return syrk_np<2>(s, nbf, kdim, X, K, part);
+    if (nparts == 2) {
+        // X panel-major (the two-part X build's layout)
+        if (kdim % 32) {
+            ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld is not a multiple of 32", kdim);
+            return YGPU_ERR_INVALID;
+        }
+        const __bf16 *Xp = (const __bf16 *) X;
+        int nsplit = 0;
+        YG_TRY(pgemm_ps<true>(s, nbf, kdim, Xp, Xp, (size_t) nbf * kdim, part, &nsplit));
+        const long tot = (long) nbf * nbf;
+        hipLaunchKernelGGL(k_syrk_bf_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, K);
+        YG_HIP(hipGetLastError());
+        return YGPU_OK;
+    }
     if (nparts == 3) return syrk_np<3>(s, nbf, kdim, X, K, part);
     ygpu_set_error(YGPU_ERR_INVALID, "SYRK: %d BF16 parts (2 or 3)", nparts);
     return YGPU_ERR_INVALID;
@@ -748,23 +897,33 @@ int yg_incr_k_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, co
         ygpu_set_error(YGPU_ERR_INVALID, "difference K: k extent %ld is not a multiple of 8", kdim);
         return YGPU_ERR_INVALID;
     }
+    if (nparts == 2) {
+        // X panel-major over the basis functions (the two-part X build's layout) for the combined
+        // [P | M] block: k = m naux_pad + n with X^P_i in k < kdim, X^M_i in kdim <= k < 2 kdim, so
+        // X^M starts kdim / 32 panels after X^P
+        if (kdim % 32) {
+            ygpu_set_error(YGPU_ERR_INVALID, "difference K: k extent %ld is not a multiple of 32", kdim);
+            return YGPU_ERR_INVALID;
+        }
+        const __bf16 *XP = (const __bf16 *) X, *XM = XP + (size_t) kdim * nbf;
+        int nsplit = 0;
+        YG_TRY(pgemm_ps<false>(s, nbf, kdim, XM, XP, (size_t) nbf * 2 * kdim, part, &nsplit));
+        return incr_reduce(s, part, nsplit, nbf, Kref, K);
+    }
     const int ntb = (nbf + ST - 1) / ST;
     const int ntiles = ntb * ntb;
     long kchunk = kdim * ntiles / 1024;
     kchunk = std::max<long>(1024, std::min<long>(8192, kchunk));
-    const int bk = nparts == 2 ? syrk_bk<2>() : syrk_bk<3>();
+    const int bk = syrk_bk<3>();
     kchunk = (kchunk + bk - 1) / bk * bk;
     const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
-    YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float)));
+    YG_TRY(part.ensure((size_t) nsplit * nbf * nbf * sizeof(float) + (size_t) nbf * nbf * sizeof(double) + 256));
     const long total = (long) ntiles * nsplit;
     // X of the combined [P | M] block: rows 0 .. n_occ-1 of basis function i hold X^P_i, rows
     // n_occ .. 2 n_occ-1 X^M_i, so X^M_i starts kdim after X^P_i and rows are 2 kdim apart
     const size_t xpart = (size_t) nbf * ld;
     const __bf16 *XP = (const __bf16 *) X, *XM = XP + kdim;
-    if (nparts == 2)
-        hipLaunchKernelGGL(k_gemm_bf_nt<2>, dim3((unsigned) total), dim3(NTHR), 0, s, XM, XP, xpart, ld, nbf, kdim, ntb,
-                           kchunk, total, part.as<float>());
-    else if (nparts == 3)
+    if (nparts == 3)
         hipLaunchKernelGGL(k_gemm_bf_nt<3>, dim3((unsigned) total), dim3(NTHR), 0, s, XM, XP, xpart, ld, nbf, kdim, ntb,
                            kchunk, total, part.as<float>());
     else {
@@ -772,8 +931,5 @@ int yg_incr_k_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, co
         return YGPU_ERR_INVALID;
     }
     YG_HIP(hipGetLastError());
-    const long tot = (long) nbf * nbf;
-    hipLaunchKernelGGL(k_incr_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part.as<float>(), nsplit, nbf, Kref, K);
-    YG_HIP(hipGetLastError());
-    return YGPU_OK;
+    return incr_reduce(s, part, nsplit, nbf, Kref, K);
 }

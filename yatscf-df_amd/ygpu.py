@@ -49,6 +49,7 @@ SIGNATURES = {
     "ygpu_device_count": [ip],
     "ygpu_create": [ctypes.c_int, ctypes.POINTER(vp)],
     "ygpu_destroy": [vp],
+    "ygpu_release_cached_memory": [],
     "ygpu_set_basis": [vp, ctypes.c_int, ctypes.c_int, ip, ip, ip, dp, dp, dp],
     "ygpu_set_atoms": [vp, ctypes.c_int, dp, dp],
     "ygpu_ovlp_hcore": [vp, dp, dp],
@@ -188,6 +189,11 @@ def _d(a):
 
 def _i(a):
     return None if a is None else a.ctypes.data_as(ip)
+
+
+def release_cached_memory():
+    """Return the library's cache of freed device blocks to the driver."""
+    check(gpu_lib().ygpu_release_cached_memory())
 
 
 def device_count():
