@@ -276,6 +276,25 @@ int ygpu_build_fock_noalloc(ygpu_ctx_t *ctx, int n_occ, const double *D, const d
 int ygpu_build_density(ygpu_ctx_t *ctx, int n_occ, const double *F_orth, const double *X,
                        double *D, double *Cocc, double *eigval, int *ev_idx);
 
+/* How ygpu_build_density finds the occupied space (no counterpart in the
+ * reference, which always diagonalises; build_density.c:54).
+ * YGPU_DENSITY_EIGEN (default): the eigensolver, as above.
+ * YGPU_DENSITY_PURIFY: the projector P onto the n_occ lowest eigenvectors of
+ * F_orth by SP2 purification (FP64 matrix squares), and the occupied vectors
+ * V = P V_prev (V_prev^T P V_prev)^{-1/2} from the previous build's V_prev, so
+ * D = X P X^T as from the eigenvectors (C_occ differs by a rotation within the
+ * occupied space).  eigval is then not written.  The eigensolver runs instead
+ * for the first build, after a change of n_occ, when SP2 needs more than 96
+ * steps (a vanishing gap) or when the two occupied spaces are nearly
+ * orthogonal. */
+#define YGPU_DENSITY_EIGEN 0
+#define YGPU_DENSITY_PURIFY 1
+int ygpu_set_density_method(ygpu_ctx_t *ctx, int method);
+/* What the last ygpu_build_density ran: method_used (nullable) the
+ * YGPU_DENSITY_* method, sp2_steps (nullable) the SP2 steps (0 for the
+ * eigensolver). */
+int ygpu_density_last_info(ygpu_ctx_t *ctx, int *method_used, int *sp2_steps);
+
 /* Cache the overlap S and X = S^{-1/2} (nbf x nbf) on the device for the
  * density build and DIIS.  ygpu_ovlp_hcore also caches Hcore; passing NULL
  * for Hcore (ygpu_build_fock) or X (ygpu_build_density) uses the cached

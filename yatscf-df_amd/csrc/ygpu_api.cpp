@@ -470,6 +470,9 @@ static void invalidate_handoffs(ygpu_ctx *c)
 {
     c->fock_valid = c->density_valid = c->fcur_valid = 0;
     c->kref_valid = 0;
+    c->jinc_valid = 0;
+    c->have_vt = 0;
+    c->sp2_last_steps = 0;
     for (int &ok : c->diis_err_ok) ok = 0;
 }
 
@@ -1093,11 +1096,17 @@ try {
         YG_TRY(join_side(c, c->side));
     } else {
         // J on the side stream, K on the main stream (independent until F is assembled)
+        YG_TRY(c->d_Dprev.ensure(ms));
+        YG_TRY(c->d_Jprev.ensure(ms));
         YG_TRY(fork_side(c, c->side));
         SideDrain drain{c->side};
         YG_HIP(hipEventRecord(c->ev[5], c->side));
         YG_TRY(yg_jbuild(c, c->side, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
         YG_HIP(hipEventRecord(c->ev[3], c->side));
+        // this FP64 J and its D are the base of the increments of later FP32-tensor builds
+        YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->side));
+        YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->side));
+        c->jinc_valid = 1;
         YG_HIP(hipEventRecord(c->ev[8], c->stream));
         YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
                          c->d_xpartner.as<int>(), c->d_Cocc.as<double>(), c->d_B.as<double>(), c->d_X.as<double>(),
@@ -1148,8 +1157,8 @@ try {
         ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_fock_precision: unknown mode %d", mode);
         return YGPU_ERR_INVALID;
     }
-    // the incremental J restarts when the arithmetic of J changes (FP64 <-> an FP32-tensor mode)
-    if ((mode == YGPU_FOCK_FP64) != (c->fock_mode == YGPU_FOCK_FP64)) c->jinc_valid = 0;
+    // the increments of J continue across a change of arithmetic: every FP64 build leaves its
+    // exact J and D as their base
     c->fock_mode = mode;
     return YGPU_OK;
 } catch (...) {
@@ -1167,6 +1176,99 @@ try {
     return YGPU_OK;
 } catch (...) {
     return exception_status();
+}
+
+// The density from the projector P onto the n_occ lowest eigenvectors of F' (in d_mat0), by SP2
+// purification (ygpu_purify.hip), and the occupied vectors from the previous build's V_prev:
+// Y = P V_prev spans the range of P while V_prev^T P V_prev = L L^T is well conditioned (its
+// eigenvalues are the squared cosines of the principal angles between the two occupied spaces),
+// and V = Y L^{-T} is an orthonormal basis of it, V V^T = P.  d_Cocc = X V, d_VT = V^T.  *ok = 0
+// (nothing written to d_Cocc or d_VT) when SP2 did not converge within YG_SP2_KMAX steps or the
+// least squared pivot of L is below YG_SP2_MINPIV: the caller then runs the eigensolver.
+static int density_purify(ygpu_ctx *c, int n_occ, const double *dX, int *ok)
+{
+    *ok = 0;
+    const int n = c->nbf;
+    hipStream_t s = c->stream;
+    const long ldx = (n + 15) / 16 * 16;
+    // k split of the squares: about two resident workgroups per CU over the upper tiles
+    // (128 x 128 from n = 256, 64 x 64 below), chunks of at least 64
+    const int tsz = n >= 256 ? 128 : 64, ntb = (n + tsz - 1) / tsz, ntiles = ntb * (ntb + 1) / 2;
+    const int want = std::max(1, std::min<int>((n >= 256 ? 512 : 1024) / ntiles, (int) (ldx / 64)));
+    long kchunk = (ldx + want - 1) / want;
+    kchunk = (kchunk + 15) / 16 * 16;
+    const int nsplit = (int) ((ldx + kchunk - 1) / kchunk);
+    const int nop = (n_occ + 63) / 64 * 64;
+    YG_TRY(c->d_P.ensure((size_t) n * ldx * sizeof(double)));
+    YG_TRY(c->d_sp2part.ensure((size_t) nsplit * n * n * sizeof(double)));
+    YG_TRY(c->d_sp2sum.ensure(yg_sp2_wsum_doubles(n) * sizeof(double)));
+    YG_TRY(c->d_sp2log.ensure(YG_SP2_KMAX * sizeof(double)));
+    YG_TRY(c->d_sp2slog.ensure(YG_SP2_KMAX * sizeof(int)));
+    YG_TRY(c->d_sp2misc.ensure(4 * sizeof(double)));
+    YG_TRY(c->d_Y.ensure((size_t) n * n_occ * sizeof(double)));
+    YG_TRY(c->d_VT2.ensure((size_t) n_occ * n * sizeof(double)));
+    YG_TRY(c->d_Mo.ensure((size_t) nop * nop * sizeof(double)));
+    YG_TRY(c->d_Mw.ensure((size_t) nop * nop * sizeof(double)));
+    YG_TRY(c->d_Mlinv.ensure((size_t) nop * 64 * sizeof(double)));
+    YG_TRY(c->d_MT.ensure((size_t) nop * 64 * sizeof(double)));
+    double *P = c->d_P.as<double>(), *misc = c->d_sp2misc.as<double>();
+    YG_TRY(yg_sp2_begin(s, n, c->n_orth, c->d_mat0.as<double>(), n, P, ldx, c->d_sp2sum.as<double>(), misc));
+    double elog[YG_SP2_KMAX];
+    int done = 0, first_small = -1;
+    int batch = c->sp2_last_steps > 0 ? c->sp2_last_steps : 24;
+    bool conv = false;
+    while (done < YG_SP2_KMAX) {
+        batch = std::max(1, std::min(batch, YG_SP2_KMAX - done));
+        YG_TRY(yg_sp2_steps(s, n, ldx, P, n_occ, c->d_sp2part.as<double>(), kchunk, c->d_sp2sum.as<double>(), done,
+                            batch, c->d_sp2log.as<double>(), c->d_sp2slog.as<int>()));
+        done += batch;
+        YG_HIP(hipMemcpyAsync(elog, c->d_sp2log.p, done * sizeof(double), hipMemcpyDeviceToHost, s));
+        YG_HIP(hipStreamSynchronize(s));
+        first_small = -1;
+        for (int k = 0; k < done; ++k)
+            if (std::fabs(elog[k]) < YG_SP2_TOL) {
+                first_small = k;
+                break;
+            }
+        // X_k is within about sqrt(|e_k|) of a projector; two more steps square that
+        if (first_small >= 0 && done >= first_small + 2) {
+            conv = true;
+            break;
+        }
+        batch = first_small >= 0 ? first_small + 2 - done : 2;
+    }
+    c->density_last_steps = done;
+    if (!conv) {
+        c->sp2_last_steps = 0;
+        return YGPU_OK;
+    }
+    c->sp2_last_steps = first_small + 2;
+    // Y = P V_prev; M = Y^T Y (zero padding to nop) = L L^T; V^T = L^{-1} Y^T
+    YG_TRY(yg_dgemm(s, 0, 1, n, n_occ, n, 1.0, P, (int) ldx, c->d_VT.as<double>(), n, 0.0, c->d_Y.as<double>(), n_occ,
+                    &c->d_gemm_ws));
+    YG_HIP(hipMemsetAsync(c->d_Mo.p, 0, (size_t) nop * nop * sizeof(double), s));
+    YG_TRY(yg_dgemm(s, 1, 0, n_occ, n_occ, n, 1.0, c->d_Y.as<double>(), n_occ, c->d_Y.as<double>(), n_occ, 0.0,
+                    c->d_Mo.as<double>(), nop, &c->d_gemm_ws));
+    YG_TRY(c->d_info.ensure(sizeof(int)));
+    YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), s));
+    YG_TRY(yg_potrf_lower(s, n_occ, nop, nop, c->d_Mo.as<double>(), c->d_Mlinv.as<double>(), c->d_info.as<int>(),
+                          c->d_MT.as<double>()));
+    YG_TRY(yg_min_diag_sq(s, n_occ, nop, c->d_Mo.as<double>(), misc + 2));
+    YG_TRY(yg_trtri_lower(s, nop, nop, c->d_Mo.as<double>(), c->d_Mlinv.as<double>(), c->d_Mw.as<double>(), nop,
+                          c->d_MT.as<double>()));
+    YG_TRY(yg_dgemm(s, 0, 1, n_occ, n, n_occ, 1.0, c->d_Mw.as<double>(), nop, c->d_Y.as<double>(), n_occ, 0.0,
+                    c->d_VT2.as<double>(), n, &c->d_gemm_ws));
+    int info = 0;
+    double minpiv = 0.0;
+    YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    YG_HIP(hipMemcpyAsync(&minpiv, misc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    YG_HIP(hipStreamSynchronize(s));
+    if (info != 0 || !(minpiv > YG_SP2_MINPIV)) return YGPU_OK;
+    c->d_VT.swap(c->d_VT2);
+    YG_TRY(yg_dgemm(s, 0, 1, n, n_occ, n, 1.0, dX, n, c->d_VT.as<double>(), n, 0.0, c->d_Cocc.as<double>(), n_occ,
+                    &c->d_gemm_ws));
+    *ok = 1;
+    return YGPU_OK;
 }
 
 int ygpu_build_density(ygpu_ctx_t *c, int n_occ, const double *Fo, const double *X, double *D, double *Cocc,
@@ -1205,8 +1307,19 @@ try {
         YG_HIP(hipMemcpyAsync(c->d_mat1.p, X, ms, hipMemcpyHostToDevice, c->stream));
         dX = c->d_mat1.as<double>();
     }
-    YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
-    if (nbf <= YG_EIG_NMAX_SMALL) {
+    int used = YGPU_DENSITY_EIGEN;
+    c->density_last_steps = 0;
+    if (c->density_method == YGPU_DENSITY_PURIFY && c->have_vt && c->vt_nocc == n_occ) {
+        int ok = 0;
+        YG_TRY(density_purify(c, n_occ, dX, &ok));
+        if (ok) used = YGPU_DENSITY_PURIFY;
+    }
+    // the eigensolver leaves the occupied vectors' V^T (n_occ x nbf) at the start of vsrc
+    const double *vsrc = nullptr;
+    if (used == YGPU_DENSITY_EIGEN) YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
+    if (used == YGPU_DENSITY_PURIFY) {
+        // d_Cocc and d_VT written by density_purify
+    } else if (nbf <= YG_EIG_NMAX_SMALL) {
         // F' = Q T Q^T on one workgroup; all eigenvalues of T by multisection and its n_occ
         // lowest eigenvectors by inverse iteration; only those are carried back:
         // V_occ = Q Z_occ in place of Z_occ (columns of d_mat2, column-major)
@@ -1223,6 +1336,7 @@ try {
         // C_occ = X V_occ
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
                         c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
+        vsrc = c->d_mat2.as<double>();
     } else if (nbf <= YG_SYTRD_BLOCKED_NMAX) {
         // blocked reduction, all eigenpairs of T, Q applied to the n_occ lowest only: their
         // columns of d_mat2 (column-major) are the rows of V_occ^T
@@ -1233,6 +1347,7 @@ try {
         YG_TRY(solver_tridiag_to_full(c, nbf, nbf, c->d_mat0.as<double>(), c->d_mat2.as<double>(), n_occ));
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
                         c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
+        vsrc = c->d_mat2.as<double>();
     } else {
         rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf,
                                              c->d_mat0.as<double>(), nbf, c->d_eig.as<double>(),
@@ -1245,12 +1360,24 @@ try {
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, nbf, 1.0, dX, nbf, c->d_mat0.as<double>(), nbf, 0.0,
                         c->d_mat2.as<double>(), nbf, &c->d_gemm_ws));
         YG_TRY(yg_gather_cols(c->stream, nbf, n_occ, c->d_mat2.as<double>(), nullptr, c->d_Cocc.as<double>()));
+        vsrc = c->d_mat0.as<double>();
     }
+    if (vsrc) {
+        // the occupied vectors of this build start the purification of the next
+        YG_TRY(c->d_VT.ensure((size_t) n_occ * nbf * sizeof(double)));
+        YG_HIP(hipMemcpyAsync(c->d_VT.p, vsrc, (size_t) n_occ * nbf * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+        c->have_vt = 1;
+        c->vt_nocc = n_occ;
+    }
+    c->density_last_method = used;
     YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, nbf, n_occ, 1.0, c->d_Cocc.as<double>(), n_occ, c->d_Cocc.as<double>(), n_occ,
                     0.0, c->d_D.as<double>(), nbf, &c->d_gemm_ws));
     int info = 0;
-    YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    YG_HIP(hipMemcpyAsync(eigval, c->d_eig.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (used == YGPU_DENSITY_EIGEN) {
+        YG_HIP(hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        YG_HIP(hipMemcpyAsync(eigval, c->d_eig.p, nbf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
     YG_HIP(hipMemcpyAsync(D, c->d_D.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipMemcpyAsync(Cocc, c->d_Cocc.p, (size_t) nbf * n_occ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
@@ -1453,6 +1580,29 @@ try {
                     c->d_gC.as<double>(), ldc, &c->d_gemm_ws));
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_set_density_method(ygpu_ctx_t *c, int method)
+try {
+    YG_TRY(check_ctx(c));
+    if (method != YGPU_DENSITY_EIGEN && method != YGPU_DENSITY_PURIFY) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_set_density_method: unknown method %d", method);
+        return YGPU_ERR_INVALID;
+    }
+    c->density_method = method;
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_density_last_info(ygpu_ctx_t *c, int *method_used, int *sp2_steps)
+try {
+    YG_TRY(check_ctx(c));
+    if (method_used) *method_used = c->density_last_method;
+    if (sp2_steps) *sp2_steps = c->density_last_steps;
     return YGPU_OK;
 } catch (...) {
     return exception_status();

@@ -26,57 +26,63 @@ constexpr int NB = 64;
 // row-major) receives L_kk^{-1}.  Rows / columns at or beyond n keep an identity block.  A
 // pivot that is not positive sets *info to its global column + 1 (first failure kept) and is
 // replaced by 1 so that the remaining work stays finite.
-__global__ __launch_bounds__(256) void k_potrf_diag(int n, long lda, double *__restrict__ A, int k0,
-                                                    double *__restrict__ Linv, int *__restrict__ info)
+// One wavefront, lane i owning row i in registers (Cholesky-Crout by rows): at column j every
+// lane forms a_ij - sum_{k<j} L_ik L_jk with row j of L read from LDS as a broadcast, lane j
+// takes the square root and the lanes below divide by it.  Then lane c solves L x = e_c by
+// forward substitution for column c of L^{-1}, again against broadcast rows of L.
+__global__ __launch_bounds__(64) void k_potrf_diag(int n, long lda, double *__restrict__ A, int k0,
+                                                   double *__restrict__ Linv, int *__restrict__ info)
 {
-    __shared__ double a[NB][NB + 1];
-    __shared__ double x[NB][NB + 1];
-    const int t = threadIdx.x;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e / NB, j = e % NB;
+    __shared__ double Ls[NB][NB + 1];
+    const int i = threadIdx.x;
+    // the block through LDS (row q read by all lanes, one column each), then row i to registers;
+    // the factorisation overwrites Ls[j][k] (k <= j) with L before reading it
+    for (int q = 0; q < NB; ++q) {
         double v = 0.0;
-        if (k0 + i < n && k0 + j < n) v = (j <= i) ? A[(size_t) (k0 + i) * lda + k0 + j] : 0.0;
-        else if (i == j) v = 1.0;
-        a[i][j] = v;
+        if (k0 + q < n && k0 + i < n) v = (i <= q) ? A[(size_t) (k0 + q) * lda + k0 + i] : 0.0;
+        else if (q == i) v = 1.0;
+        Ls[q][i] = v;
     }
     __syncthreads();
+    double r[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) r[j] = Ls[i][j];
+    __syncthreads();
+#pragma unroll
     for (int j = 0; j < NB; ++j) {
-        if (t == 0) {
-            double p = a[j][j];
-            if (!(p > 0.0)) {
+        double s = r[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= r[k] * Ls[j][k];
+        if (i == j) {
+            if (!(s > 0.0)) {
                 if (k0 + j < n) atomicCAS(info, 0, k0 + j + 1);
-                p = 1.0;
+                s = 1.0;
             }
-            a[j][j] = sqrt(p);
+            s = sqrt(s);
+            r[j] = s;
+            Ls[j][j] = s;
         }
         __syncthreads();
-        const double d = a[j][j];
-        for (int i = j + 1 + t; i < NB; i += 256) a[i][j] /= d;
-        __syncthreads();
-        // trailing update of the lower triangle below and right of (j, j)
-        const int m = NB - j - 1;
-        for (int e = t; e < m * m; e += 256) {
-            const int i = j + 1 + e / m, c = j + 1 + e % m;
-            if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+        if (i > j) {
+            r[j] = s / Ls[j][j];
+            Ls[i][j] = r[j];
         }
         __syncthreads();
     }
-    // X = L^{-1} column by column: thread c solves L x = e_c by forward substitution
-    if (t < NB) {
-        const int c = t;
-        for (int i = 0; i < NB; ++i) {
-            double s = (i == c) ? 1.0 : 0.0;
-            if (i > c)
-                for (int p = c; p < i; ++p) s -= a[i][p] * x[p][c];
-            x[i][c] = (i >= c) ? s / a[i][i] : 0.0;
-        }
+    // L back to A row by row (lane = column); Ls holds L on and below the diagonal
+    for (int q = 0; q < NB; ++q)
+        if (k0 + q < n && k0 + i < n) A[(size_t) (k0 + q) * lda + k0 + i] = (i <= q) ? Ls[q][i] : 0.0;
+    // column c = i of L^{-1}: x_q = (delta_qc - sum_{p<q} L_qp x_p) / L_qq (x_p = 0 for p < c)
+    double x[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        double s = (q == i) ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < q; ++p) s -= Ls[q][p] * x[p];
+        x[q] = s / Ls[q][q];
     }
-    __syncthreads();
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e / NB, j = e % NB;
-        if (k0 + i < n && k0 + j < n) A[(size_t) (k0 + i) * lda + k0 + j] = (j <= i) ? a[i][j] : 0.0;
-        Linv[e] = x[i][j];
-    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) Linv[q * NB + i] = x[q];
 }
 
 // W's diagonal blocks from the per-block inverses; every other element of rows [0, npad) and
@@ -105,7 +111,7 @@ int yg_potrf_lower(hipStream_t s, int n, int npad, long lda, double *A, double *
     const int nblk = npad / NB;
     for (int b = 0; b < nblk; ++b) {
         const int k0 = b * NB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, n, lda, A, k0, Linv + (size_t) b * NB * NB, info);
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, n, lda, A, k0, Linv + (size_t) b * NB * NB, info);
         YG_HIP(hipGetLastError());
         const int rest = npad - k0 - NB;
         if (rest <= 0) break;

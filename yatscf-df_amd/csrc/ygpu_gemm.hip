@@ -555,6 +555,16 @@ int yg_syrk_upper_ws(hipStream_t s, int nbf, long kdim, const double *X, double 
     if (nbf <= 0) return YGPU_OK;
     const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
     const long tot = (long) nbf * nbf;
+    YG_TRY(yg_syrk_upper_parts(s, nbf, kdim, X, part, kchunk));
+    hipLaunchKernelGGL(k_syrk_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part, nsplit, nbf, K);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_syrk_upper_parts(hipStream_t s, int nbf, long kdim, const double *X, double *part, long kchunk)
+{
+    if (nbf <= 0) return YGPU_OK;
+    const int nsplit = (int) ((kdim + kchunk - 1) / kchunk);
     if (nbf >= 256) {
         if (kdim % 8 || kchunk % S_BK) {
             ygpu_set_error(YGPU_ERR_INVALID, "SYRK: k extent %ld / chunk %ld not a multiple of 8 / 16", kdim, kchunk);
@@ -576,8 +586,6 @@ int yg_syrk_upper_ws(hipStream_t s, int nbf, long kdim, const double *X, double 
         hipLaunchKernelGGL(k_syrk_part, dim3(ntiles, 1, nsplit), dim3(NTHR), 0, s, X, nbf, kdim, ntb, kchunk, part);
         YG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_syrk_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, part, nsplit, nbf, K);
-    YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
 

@@ -41,6 +41,11 @@ void ygpu_set_error(int code, const char *fmt, ...);
 // largest n of the blocked two-kernel tridiagonal reduction (two rows per thread of its
 // one-workgroup column kernel); rocSOLVER dsyevd beyond
 #define YG_SYTRD_BLOCKED_NMAX 2048
+// SP2 purification: step limit, idempotency threshold that starts the two closing steps, least
+// squared Cholesky pivot of V_prev^T P V_prev accepted for the occupied vectors
+#define YG_SP2_KMAX 96
+#define YG_SP2_TOL 1e-9
+#define YG_SP2_MINPIV 1e-2
 
 // growable device buffer, freed when released or destroyed (not copyable)
 // Device buffer.  Blocks of 1 MiB and more come from and go back to a process-wide cache of
@@ -141,7 +146,7 @@ struct ygpu_ctx {
     int have_B32 = 0;             // d_B32 holds the FP32 copy of d_B
     int have_Bt = 0;              // d_Bt holds the transposed two-part BF16 blocks of d_B32
     DBuf d_Bt, d_Cp;              // B and C_occ operand blocks of the two-part X build
-    int jinc_valid = 0;           // d_Dprev / d_Jprev hold the previous mixed build
+    int jinc_valid = 0;           // d_Dprev / d_Jprev hold a density and its J (the last FP64 build or FP32-tensor increment)
     DBuf d_B32, d_Dprev, d_Jprev, d_dD, d_Kpart32, d_Cocc32;
     // difference-density exchange (YGPU_FOCK_INCR): the C_occ and FP64 K of the last FP64 build,
     // the aligned C_occ, the polar factor and its Newton-Schulz work, the FP32 [P | M] block
@@ -153,6 +158,13 @@ struct ygpu_ctx {
 
     // own eigensolver paths: reflector scalars and tridiagonal workspace
     DBuf d_tau, d_tri;
+    // density by purification (ygpu_set_density_method): the occupied vectors V^T (n_occ x nbf,
+    // orthogonal basis) of the last density build, the projector, the SP2 partials and logs, and
+    // the work of the orthonormalisation V = P V_prev (V_prev^T P V_prev)^{-1/2}
+    int density_method = YGPU_DENSITY_EIGEN;
+    int have_vt = 0, vt_nocc = 0;
+    int density_last_method = YGPU_DENSITY_EIGEN, density_last_steps = 0, sp2_last_steps = 0;
+    DBuf d_VT, d_VT2, d_P, d_Y, d_Mo, d_Mlinv, d_Mw, d_MT, d_sp2part, d_sp2sum, d_sp2log, d_sp2slog, d_sp2misc;
 
     // device-resident hand-offs between the calls of one SCF iteration (NULL host arguments):
     // d_F, d_D, d_Cocc hold the last Fock build's F and its inputs (fock_valid); after a
@@ -206,6 +218,19 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
 int yg_syrk_upper_split(int nbf, long kdim, long *kchunk_out);
 // yg_syrk_upper with the partials in a caller-provided buffer
 int yg_syrk_upper_ws(hipStream_t s, int nbf, long kdim, const double *X, double *K, double *part, long kchunk);
+// the split-K partials of the upper tiles of X X^T only (nbf x nbf each, no reduction)
+int yg_syrk_upper_parts(hipStream_t s, int nbf, long kdim, const double *X, double *part, long kchunk);
+// SP2 purification (ygpu_purify.hip): X (n x ldx, ldx a multiple of 16) = X_0 from F (rows and
+// columns < n_orth), bounds[0..1] = the Gershgorin interval; wsum holds yg_sp2_wsum_doubles(n)
+int yg_sp2_begin(hipStream_t s, int n, int n_orth, const double *F, long ldf, double *X, long ldx, double *wsum,
+                 double *bounds);
+size_t yg_sp2_wsum_doubles(int n);
+// steps first .. first + count - 1 (each a SYRK of X into part, ldx / kchunk splits, then the
+// choice); elog[k] = tr(X_k) - tr(X_k^2), slog[k] = 1 if step k squared
+int yg_sp2_steps(hipStream_t s, int n, long ldx, double *X, int N, double *part, long kchunk, double *wsum,
+                 int first, int count, double *elog, int *slog);
+// out[0] = min_i L[i][i]^2 over i < n
+int yg_min_diag_sq(hipStream_t s, int n, long ld, const double *L, double *out);
 // J from density dD (upper triangle of dJ; the rest zero) on stream s, timed by ev[0..2];
 // B32 != nullptr reads the FP32 copy
 int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, float *t_tempJ_ms,

@@ -324,18 +324,26 @@ void TinySCF_get_initial_guess(TinySCF_t T)
     if (T->sph_orb) sph_project_sym(T, T->D_mat);
     T->nuc_energy = CMS_getNucEnergy(T->basis);
     double t1 = get_wtime_sec();
-    TinySCF_D2Cocc(T);
-    double t2 = get_wtime_sec();
-    if (T->verbose) printf("TinySCF SAD initial guess: D mat / Cocc mat used %lf, %lf (s)\n", t1 - t0, t2 - t1);
     T->guess_ml = 0;
     if (T->guess_weights[0]) {
-        /* learned pair correction; on any failure the SAD guess above stays */
-        const int st = TinySCF_learned_guess(T, T->guess_weights);
-        T->guess_ml = (st == 0);
-        if (T->verbose)
-            printf("TinySCF learned guess correction: %s (%lf s)\n", st == 0 ? "applied" : "not applied, SAD kept",
-                   get_wtime_sec() - t2);
+        /* learned pair correction, which forms its own C_occ (the occupied eigenvectors of the
+         * corrected density); on any failure the SAD guess stays and D2Cocc below factors it */
+        double *sad = (double *) malloc(DBL_SIZE * T->mat_size);
+        if (sad) {
+            memcpy(sad, T->D_mat, DBL_SIZE * T->mat_size);
+            const int st = TinySCF_learned_guess(T, T->guess_weights);
+            T->guess_ml = (st == 0);
+            if (!T->guess_ml) memcpy(T->D_mat, sad, DBL_SIZE * T->mat_size);
+            free(sad);
+            if (T->verbose)
+                printf("TinySCF learned guess correction: %s (%lf s)\n", st == 0 ? "applied" : "not applied, SAD kept",
+                       get_wtime_sec() - t1);
+        }
     }
+    double t2 = get_wtime_sec();
+    if (!T->guess_ml) TinySCF_D2Cocc(T);
+    if (T->verbose && !T->guess_ml)
+        printf("TinySCF SAD initial guess: D mat / Cocc mat used %lf, %lf (s)\n", t1 - t0, get_wtime_sec() - t2);
 }
 
 static void TinySCF_calc_energy(TinySCF_t T)
@@ -343,6 +351,37 @@ static void TinySCF_calc_energy(TinySCF_t T)
     double e = 0.0;
     for (int i = 0; i < T->mat_size; i++) e += T->D_mat[i] * (T->F_mat[i] + T->Hcore_mat[i]);
     T->HF_energy = e;
+}
+
+/*
+ * Energies of a difference-density phase.  With G = F - H = 2J - K, the energy is
+ * 2 tr(D H) + tr(D G(D)).  A difference build forms G(D) = G_r + G~(D - D_r) from the full FP64
+ * build's G_r = G(D_r) and a reduced-precision G~ whose error e is of first order in the
+ * distance D - D_r, and tr(D G) carries tr(D_r e) + tr((D - D_r) e).  The Coulomb and exchange
+ * forms are symmetric, tr(A G(B)) = tr(B G(A)), so tr(D_r G(D - D_r)) = tr((D - D_r) G_r): the
+ * energy taken as tr(D (H + F)) + tr(D G_r) - tr(D_r G) is exact up to tr((D - D_r) e), which is
+ * of second order.  A full FP64 build of the phase becomes the reference (`used` is the
+ * arithmetic the build ran: a difference build whose alignment fails runs in FP64).
+ */
+static void TinySCF_reference_energy(TinySCF_t T, int used)
+{
+    const size_t ms = (size_t) T->mat_size;
+    if (used == YGPU_FOCK_FP64) {
+        if (!T->Gref_mat) T->Gref_mat = (double *) ALIGN64B_MALLOC(DBL_SIZE * ms);
+        if (!T->Dref_mat) T->Dref_mat = (double *) ALIGN64B_MALLOC(DBL_SIZE * ms);
+        if (!T->Gref_mat || !T->Dref_mat) {
+            fprintf(stderr, "TinySCF: reference matrices (%zu bytes) could not be allocated\n", 2 * DBL_SIZE * ms);
+            abort();
+        }
+        for (size_t i = 0; i < ms; i++) T->Gref_mat[i] = T->F_mat[i] - T->Hcore_mat[i];
+        memcpy(T->Dref_mat, T->D_mat, DBL_SIZE * ms);
+        T->ref_valid = 1;
+    } else if (used == YGPU_FOCK_INCR && T->ref_valid) {
+        double c = 0.0;
+        for (size_t i = 0; i < ms; i++)
+            c += T->D_mat[i] * T->Gref_mat[i] - T->Dref_mat[i] * (T->F_mat[i] - T->Hcore_mat[i]);
+        T->HF_energy += c;
+    }
 }
 
 /* the DIIS history as TinySCF_init_free.c:173-180 leaves it */
@@ -384,6 +423,7 @@ void TinySCF_do_SCF(TinySCF_t T)
      * without it, full FP64 builds to convergence.  Energies of reduced-precision builds never
      * end the SCF. */
     const int use_incr = T->mixed && (T->mixed_schedule & 4);
+    T->ref_valid = 0;
     T->mixed_phase = T->mixed ? next_reduced_phase(T, YGPU_FOCK_FP64) : YGPU_FOCK_FP64;
     T->mixed_switch_iter = (T->mixed && T->mixed_phase == YGPU_FOCK_FP64) ? 0 : -1;
     T->mixed6_switch_iter = T->mixed_phase == YGPU_FOCK_MIXED6 ? 0 : -1;
@@ -395,34 +435,17 @@ void TinySCF_do_SCF(TinySCF_t T)
         const int phase = T->mixed_phase;
         if (T->mixed) YS_GPU(ygpu_set_fock_precision(g, phase));
         TinySCF_build_FockMat(T);
-        if (T->iter < YS_MAX_PHASE_LOG) {
-            int used = phase;
-            if (T->mixed) YS_GPU(ygpu_fock_last_info(g, &used, NULL));   /* a difference build may fall back */
-            T->phase_log[T->iter] = used;
-        }
+        int used = phase;
+        if (T->mixed) YS_GPU(ygpu_fock_last_info(g, &used, NULL));   /* a difference build may fall back */
+        if (T->iter < YS_MAX_PHASE_LOG) T->phase_log[T->iter] = used;
         TinySCF_calc_energy(T);
+        if (use_incr) TinySCF_reference_energy(T, used);
         delta = fabs(T->HF_energy - prev);
         prev = T->HF_energy;
         double st1 = get_wtime_sec();
         TinySCF_DIIS(T);
         double st2 = get_wtime_sec();
-        TinySCF_build_DenMat(T);
-        /* D, C_occ, F now stay on the device between the calls of the loop (the host copies are
-         * still written, as the reference's are) */
-        if (T->iter == 0) YS_GPU(ygpu_set_scf_resident(g, 1));
-        double st3 = get_wtime_sec();
-        T->t_diis = st2 - st1;
-        T->t_density = st3 - st2;
-        if (T->verbose) {
-            static const char *tag[] = {"", " [BF16x3]", " [BF16x6]", " [FP64 + difference]"};
-            printf("* DIIS procedure        : %.3lf (s)\n", st2 - st1);
-            printf("* Build density matrix  : %.3lf (s)\n", st3 - st2);
-            printf("* Iteration runtime     = %.3lf (s)\n", st3 - st0);
-            printf("* Energy = %.10lf (%.10lf)", T->HF_energy + T->nuc_energy, T->HF_energy);
-            if (T->iter > 0) printf(", delta = %e", delta);
-            if (T->iter >= 2) printf(", DIIS error = %.3e", T->diis_error);
-            printf("%s\n", tag[phase]);
-        }
+        const double delta_raw = delta;
         /* an energy from full or difference FP64 builds is compared only with another one */
         const int exact = phase == YGPU_FOCK_FP64 || phase == YGPU_FOCK_INCR;
         if (T->iter == 0 || !exact || !prev_exact) delta = 223.0;   /* iteration 0 never converges (TinySCF.c:429) */
@@ -462,29 +485,53 @@ void TinySCF_do_SCF(TinySCF_t T)
             }
         } else if (use_incr) {
             /* FP64 and difference builds: a full FP64 build is the reference of the difference
-             * builds that follow.  The rounding error of a difference build scales with the
-             * distance of its density from the reference's, and with it the noise of its energy:
-             * once the DIIS error has fallen below mixed_reref times its value at the reference,
-             * the next build is a full FP64 one, the new reference.  Energies agreeing within
-             * ene_tol end the SCF when the later one is a full FP64 build, or the difference
-             * build right after one (its difference density is that one step's, its energy the
-             * reference arithmetic's to rounding); agreement among later difference builds is
-             * confirmed by a full FP64 build */
-            if (phase == YGPU_FOCK_FP64) {
+             * builds that follow, whose energies are exact to second order in the distance from
+             * it (TinySCF_reference_energy), so two agreeing energies end the SCF.  Options:
+             * mixed_reref > 0 takes a new reference once the DIIS error has fallen below that
+             * fraction of its value at the last one; mixed_confirm ends the SCF only on a full
+             * FP64 build or the difference build right after one, and confirms agreement among
+             * later difference builds by a full FP64 build */
+            if (used == YGPU_FOCK_FP64) {
                 /* a reference build, or a confirming one whose energy disagreed: difference
                  * builds follow (an agreeing confirmation ends the loop) */
                 diis_at_ref = T->diis_error >= 0.0 ? T->diis_error : DBL_MAX;
                 since_ref = 0;
                 if (delta >= T->ene_tol) T->mixed_phase = YGPU_FOCK_INCR;
             } else if (delta < T->ene_tol) {
-                if (++since_ref > 1) {
+                if (T->mixed_confirm && ++since_ref > 1) {
                     T->mixed_phase = YGPU_FOCK_FP64;
                     delta = 223.0;
                 }
             } else {
                 since_ref++;
-                if (T->diis_error >= 0.0 && T->diis_error < T->mixed_reref * diis_at_ref) T->mixed_phase = YGPU_FOCK_FP64;
+                if (T->mixed_reref > 0.0 && T->diis_error >= 0.0 && T->diis_error < T->mixed_reref * diis_at_ref)
+                    T->mixed_phase = YGPU_FOCK_FP64;
             }
+        }
+        /* the density of an iteration that continues comes from purification of F' (its occupied
+         * space only); the last one, which leaves the orbitals and their energies, from the
+         * eigensolver, as the reference's */
+        const int last = T->iter + 1 >= T->niters || delta < T->ene_tol;
+        YS_GPU(ygpu_set_density_method(g, (last || !T->purify) ? YGPU_DENSITY_EIGEN : YGPU_DENSITY_PURIFY));
+        TinySCF_build_DenMat(T);
+        /* D, C_occ, F now stay on the device between the calls of the loop (the host copies are
+         * still written, as the reference's are) */
+        if (T->iter == 0) YS_GPU(ygpu_set_scf_resident(g, 1));
+        double st3 = get_wtime_sec();
+        T->t_diis = st2 - st1;
+        T->t_density = st3 - st2;
+        if (T->verbose) {
+            static const char *tag[] = {"", " [BF16x3]", " [BF16x6]", " [FP64 + difference]"};
+            printf("* DIIS procedure        : %.3lf (s)\n", st2 - st1);
+            int dm = YGPU_DENSITY_EIGEN, dsteps = 0;
+            YS_GPU(ygpu_density_last_info(g, &dm, &dsteps));
+            if (dm == YGPU_DENSITY_PURIFY) printf("* Build density matrix  : %.3lf (s) [SP2, %d steps]\n", st3 - st2, dsteps);
+            else printf("* Build density matrix  : %.3lf (s)\n", st3 - st2);
+            printf("* Iteration runtime     = %.3lf (s)\n", st3 - st0);
+            printf("* Energy = %.10lf (%.10lf)", T->HF_energy + T->nuc_energy, T->HF_energy);
+            if (T->iter > 0) printf(", delta = %e", delta_raw);
+            if (T->iter >= 2) printf(", DIIS error = %.3e", T->diis_error);
+            printf("%s\n", tag[phase]);
         }
         T->iter++;
     }

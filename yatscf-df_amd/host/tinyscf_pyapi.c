@@ -134,6 +134,8 @@ int tinyscf_geti(TinySCF_t T, const char *n)
     if (!strcmp(n, "mixed6_switch_iter")) return T->mixed6_switch_iter;
     if (!strcmp(n, "mixed_schedule")) return T->mixed_schedule;
     if (!strcmp(n, "mixed_keep_diis")) return T->mixed_keep_diis;
+    if (!strcmp(n, "mixed_confirm")) return T->mixed_confirm;
+    if (!strcmp(n, "purify")) return T->purify;
     if (!strcmp(n, "guess_ml")) return T->guess_ml;
     if (!strcmp(n, "electron")) return T->electron;
     return -1;
@@ -150,6 +152,8 @@ void tinyscf_seti(TinySCF_t T, const char *n, int v)
     else if (!strcmp(n, "mixed")) T->mixed = v;
     else if (!strcmp(n, "mixed_schedule")) T->mixed_schedule = v;
     else if (!strcmp(n, "mixed_keep_diis")) T->mixed_keep_diis = v;
+    else if (!strcmp(n, "mixed_confirm")) T->mixed_confirm = v;
+    else if (!strcmp(n, "purify")) T->purify = v;
 }
 
 double tinyscf_getd(TinySCF_t T, const char *n)

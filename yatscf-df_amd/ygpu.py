@@ -76,6 +76,8 @@ SIGNATURES = {
     "ygpu_set_aux_spherical": [vp, ctypes.c_int],
     "ygpu_set_orth_dim": [vp, ctypes.c_int],
     "ygpu_fock_last_info": [vp, ip, dp],
+    "ygpu_set_density_method": [vp, ctypes.c_int],
+    "ygpu_density_last_info": [vp, ip, ip],
     "ygpu_mlp_forward": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "ygpu_mlp_trainer_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -88,6 +90,7 @@ SIGNATURES = {
 }
 
 FOCK_FP64, FOCK_MIXED, FOCK_MIXED6, FOCK_INCR = 0, 1, 2, 3
+DENSITY_EIGEN, DENSITY_PURIFY = 0, 1
 METRIC_CHOLESKY, METRIC_INVSQRT = 0, 1
 
 fp = ctypes.POINTER(ctypes.c_float)
@@ -343,6 +346,16 @@ class Engine:
         check(self.L.ygpu_build_density(self.h, n_occ, _d(Fo), _d(X), _d(D), _d(C), _d(ev), _i(idx)))
         return D, C, ev, idx
 
+    def set_density_method(self, method):
+        """DENSITY_EIGEN (default) or DENSITY_PURIFY for the following build_density calls."""
+        check(self.L.ygpu_set_density_method(self.h, int(method)))
+
+    def density_last_info(self):
+        """(method used, SP2 steps) of the last build_density."""
+        m, k = ctypes.c_int(), ctypes.c_int()
+        check(self.L.ygpu_density_last_info(self.h, ctypes.byref(m), ctypes.byref(k)))
+        return m.value, k.value
+
     def dgemm(self, ta, tb, A, B, alpha=1.0, beta=0.0, C=None):
         A = np.ascontiguousarray(A, dtype=np.float64)
         B = np.ascontiguousarray(B, dtype=np.float64)
@@ -498,7 +511,8 @@ class TinySCF:
 
 
 def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, fock_check=False,
-          final_fock=False, schedule=None, keep_diis=None, reref=None, verbose=False, basis_type=None,
+          final_fock=False, schedule=None, keep_diis=None, reref=None, confirm=None, purify=None,
+          verbose=False, basis_type=None,
           basis=BASIS, auxbasis=AUXBASIS):
     """One complete SCF of ``xyz`` through the TinySCF engine, timed from after input parsing
     (basis shells and nuclei uploaded) to the passing convergence test: S/H/X, Schwarz
@@ -524,6 +538,10 @@ def solve(xyz, niters=80, ene_tol=1e-11, quirk_a2=1, mixed=False, weights=None, 
             T.seti("mixed_keep_diis", int(keep_diis))
         if reref is not None:
             T.setd("mixed_reref", float(reref))
+        if confirm is not None:
+            T.seti("mixed_confirm", int(confirm))
+        if purify is not None:
+            T.seti("purify", int(purify))
         t0 = time.perf_counter()
         T.setup()
         t1 = time.perf_counter()
