@@ -246,6 +246,16 @@ int ygpu_set_fock_precision(ygpu_ctx_t *ctx, int mode);
  * YGPU_FOCK_FP64), align_err (nullable) max |Q^T Q - I| of a difference build's
  * alignment (0 otherwise). */
 int ygpu_fock_last_info(ygpu_ctx_t *ctx, int *mode_used, double *align_err);
+/* Energy terms of the last ygpu_build_fock, summed on the device in a fixed
+ * order (the sum TinySCF_calc_energy forms on the host, TinySCF.c:366-372):
+ * e_dhf = sum_ij D_ij (Hcore_ij + F_ij).  e_ref: after a YGPU_FOCK_INCR build
+ * that ran as one, sum_ij D (F_r - Hcore) - D_r (F - Hcore) with F_r, D_r the
+ * Fock matrix and density of the last full FP64 build, each element's terms
+ * combined before the sum; 0 otherwise.  With G = F - Hcore symmetric and
+ * linear in D, tr(D_r G(D)) = tr(D G(D_r)), so e_dhf + e_ref carries the
+ * reduced-precision error of the difference only at second order in D - D_r.
+ * Both pointers nullable; valid until the next density build. */
+int ygpu_fock_energy(ygpu_ctx_t *ctx, double *e_dhf, double *e_ref);
 
 /* ---- Fock build without device allocation.  No counterpart in the reference
  * (whose Fock build, build_Fock.c:349-396, allocates nothing per call because
@@ -294,6 +304,15 @@ int ygpu_set_density_method(ygpu_ctx_t *ctx, int method);
  * YGPU_DENSITY_* method, sp2_steps (nullable) the SP2 steps (0 for the
  * eigensolver). */
 int ygpu_density_last_info(ygpu_ctx_t *ctx, int *method_used, int *sp2_steps);
+
+/* The RHF density nearest a guess density D_guess (nbf x nbf), for an initial
+ * guess that is not idempotent (no counterpart in the reference, whose SAD
+ * guess goes to D2Cocc, TinySCF.c:282-322): the n_occ eigenvectors of largest
+ * occupation of S^{1/2} D_guess S^{1/2} (S^{1/2} = S X with the cached S and
+ * X), by the density build on F_orth = -S^{1/2} D_guess S^{1/2}: D, Cocc and
+ * eigval as ygpu_build_density writes them. */
+int ygpu_project_guess(ygpu_ctx_t *ctx, int n_occ, const double *D_guess, double *D, double *Cocc,
+                       double *eigval);
 
 /* Cache the overlap S and X = S^{-1/2} (nbf x nbf) on the device for the
  * density build and DIIS.  ygpu_ovlp_hcore also caches Hcore; passing NULL

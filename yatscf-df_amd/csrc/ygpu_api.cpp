@@ -724,6 +724,15 @@ try {
     }
     kofs[nbf] = (int) ks;
     c->ksum = ks;
+    std::vector<int> ktiles;
+    for (int i = 0; i < nbf; ++i) {
+        const int L = (xptr[i + 1] - xptr[i] + 31) / 32 * 32;
+        for (int k0 = 0; k0 < L; k0 += 64) {
+            ktiles.push_back(i);
+            ktiles.push_back(k0);
+        }
+    }
+    c->nktiles = (int) (ktiles.size() / 2);
     std::vector<int> xorder(nbf);
     for (int i = 0; i < nbf; ++i) xorder[i] = i;
     std::stable_sort(xorder.begin(), xorder.end(),
@@ -741,6 +750,7 @@ try {
     YG_TRY(upload(c->d_xup, xup.data(), std::max<size_t>(1, xup.size()) * sizeof(int), c->stream));
     YG_TRY(upload(c->d_xpartner, xpartner.data(), std::max<size_t>(1, xpartner.size()) * sizeof(int), c->stream));
     YG_TRY(upload(c->d_kofs, kofs.data(), kofs.size() * sizeof(int), c->stream));
+    YG_TRY(upload(c->d_ktiles, ktiles.data(), std::max<size_t>(2, ktiles.size()) * sizeof(int), c->stream));
     YG_TRY(upload(c->d_xorder, xorder.data(), xorder.size() * sizeof(int), c->stream));
     YG_TRY(upload(c->d_sp_scr, c->h_sp_scrval.data(), c->h_sp_scrval.size() * sizeof(double), c->stream));
     YG_TRY(upload(c->d_df_scr, c->h_df_sp_scrval.data(), c->h_df_sp_scrval.size() * sizeof(double), c->stream));
@@ -807,7 +817,8 @@ static int xbuild_mixed(ygpu_ctx *c, int nparts, int nrow, const float *C32, siz
     }
     const size_t cpart = (size_t) c->ksum * nrow;
     YG_TRY(c->d_Cp.ensure(2 * cpart * sizeof(__bf16)));
-    YG_TRY(yg_gather_split_t(c->stream, nbf, c->max_row_len, C32, nrow, nrow, c->d_xpartner.as<int>(), c->d_xptr.as<int>(),
+    YG_TRY(yg_gather_split_t(c->stream, c->nktiles, c->d_ktiles.as<int2>(), C32, nrow, nrow, c->d_xpartner.as<int>(),
+                             c->d_xptr.as<int>(),
                              c->d_kofs.as<int>(), c->d_Cp.p, cpart));
     YG_HIP(hipEventRecord(c->ev[8], c->stream));
     return yg_xbuild_ps(c->stream, nbf, nrow, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_kofs.as<int>(),
@@ -831,8 +842,10 @@ try {
     YG_TRY(c->d_Jpq.ensure((size_t) ld * ld * sizeof(double)));
     YG_TRY(c->d_mat0.ensure((size_t) ldc * ldc * sizeof(double)));
     YG_TRY(c->d_mat1.ensure((size_t) ldc * ldc * sizeof(double)));
-    // in sequence: beside the 3-centre integrals (which fill every CU) the latency-bound inverse
-    // square root of the metric measured longer by their whole duration
+    const bool chol = c->metric_mode == YGPU_METRIC_CHOLESKY;
+    // in sequence: beside the 3-centre integrals, which fill every CU, the metric's factor (a
+    // chain of small launches, or the eigensolver of J^{-1/2}) measured longer by their whole
+    // duration (J_PQ on a second stream started only once the integrals were done)
     YG_HIP(hipEventRecord(c->ev[0], c->stream));
     YG_HIP(hipMemsetAsync(c->d_pqA.p, 0, tszc, c->stream));
     YG_TRY(yg_launch_eri3(c, c->d_pqA.as<double>(), c->stream));
@@ -852,7 +865,6 @@ try {
         YG_TRY(yg_aux_sph_cols(c, c->stream, ld, c->d_mat2.as<double>(), ldc, c->d_mat0.as<double>(), ld));
     }
     YG_HIP(hipEventRecord(c->ev[2], c->stream));
-    const bool chol = c->metric_mode == YGPU_METRIC_CHOLESKY;
     if (chol) {
         // J = L L^T on the device, W = L^{-1} (lower triangular, identity on the padding), then
         // B = pqA W^T with the zeros of W above its diagonal skipped
@@ -1047,7 +1059,8 @@ try {
         const size_t btpart = (size_t) c->ksum * c->naux_pad;
         if (nparts == 2 && !c->have_Bt) {
             YG_TRY(c->d_Bt.ensure(2 * btpart * sizeof(__bf16)));
-            YG_TRY(yg_gather_split_t(c->stream, nbf, c->max_row_len, c->d_B32.as<float>(), c->naux_pad, (int) c->naux_pad,
+            YG_TRY(yg_gather_split_t(c->stream, c->nktiles, c->d_ktiles.as<int2>(), c->d_B32.as<float>(), c->naux_pad,
+                                     (int) c->naux_pad,
                                      c->d_xup.as<int>(), c->d_xptr.as<int>(), c->d_kofs.as<int>(), c->d_Bt.p, btpart));
             c->have_Bt = 1;
         }
@@ -1128,10 +1141,35 @@ try {
     YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
                             c->d_F.as<double>()));
     YG_HIP(hipEventRecord(c->ev[7], c->stream));
+    // energy terms: sum D (H + F), and after a difference build the reference term against the F
+    // and D of the last full FP64 build, which an FP64 build then records
+    const bool eref = mode == YGPU_FOCK_INCR;
+    const int neb = yg_energy_blocks();
+    YG_TRY(c->d_epart.ensure((size_t) 2 * neb * sizeof(double)));
+    YG_TRY(yg_energy_terms(c->stream, nbf, c->d_D.as<double>(), c->d_H.as<double>(), c->d_F.as<double>(),
+                           eref ? c->d_Dref.as<double>() : nullptr, eref ? c->d_Fref.as<double>() : nullptr,
+                           c->d_epart.as<double>()));
+    if (mode == YGPU_FOCK_FP64) {
+        YG_TRY(c->d_Fref.ensure(ms));
+        YG_TRY(c->d_Dref.ensure(ms));
+        YG_HIP(hipMemcpyAsync(c->d_Fref.p, c->d_F.p, ms, hipMemcpyDeviceToDevice, c->stream));
+        YG_HIP(hipMemcpyAsync(c->d_Dref.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->stream));
+    }
+    std::vector<double> epart((size_t) 2 * neb);
+    YG_HIP(hipMemcpyAsync(epart.data(), c->d_epart.p, epart.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
     YG_HIP(hipMemcpyAsync(F, c->d_F.p, ms, hipMemcpyDeviceToHost, c->stream));
     if (J) YG_HIP(hipMemcpyAsync(J, c->d_J.p, ms, hipMemcpyDeviceToHost, c->stream));
     if (K) YG_HIP(hipMemcpyAsync(K, c->d_K.p, ms, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    // block partials added pairwise in a fixed order
+    for (int w = 1; w < neb; w <<= 1)
+        for (int b = 0; b + w < neb; b += 2 * w) {
+            epart[2 * b] += epart[2 * (b + w)];
+            epart[2 * b + 1] += epart[2 * (b + w) + 1];
+        }
+    c->energy_dhf = epart[0];
+    c->energy_ref = eref ? epart[1] : 0.0;
     c->fock_valid = 1;
     ygpu_fock_timing_t tm;
     tj1 = elapsed(c->ev[0], c->ev[1]);
@@ -1580,6 +1618,52 @@ try {
                     c->d_gC.as<double>(), ldc, &c->d_gemm_ws));
     YG_HIP(hipMemcpyAsync(C, c->d_gC.p, csz, hipMemcpyDeviceToHost, c->stream));
     YG_HIP(hipStreamSynchronize(c->stream));
+    return YGPU_OK;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_project_guess(ygpu_ctx_t *c, int n_occ, const double *Dg, double *D, double *Cocc, double *eigval)
+try {
+    YG_TRY(check_ctx(c));
+    const int n = c->nbf;
+    if (!c->have_S || !Dg || !D || !Cocc || !eigval || n_occ <= 0 || n_occ > n) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_project_guess: bad arguments or no cached S / X");
+        return YGPU_ERR_INVALID;
+    }
+    const size_t ms = (size_t) n * n * sizeof(double);
+    YG_TRY(c->d_gA.ensure(ms));
+    YG_TRY(c->d_gB.ensure(ms));
+    YG_TRY(c->d_gC.ensure(ms));
+    YG_TRY(c->d_Fcur.ensure(ms));
+    YG_HIP(hipMemcpyAsync(c->d_gA.p, Dg, ms, hipMemcpyHostToDevice, c->stream));
+    // S^{1/2} = S X; P = -(S X)^T D (S X): its lowest eigenvalues are the largest occupations
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_S.as<double>(), n, c->d_Xo.as<double>(), n, 0.0,
+                    c->d_gB.as<double>(), n, &c->d_gemm_ws));
+    YG_TRY(yg_dgemm(c->stream, 0, 0, n, n, n, 1.0, c->d_gA.as<double>(), n, c->d_gB.as<double>(), n, 0.0,
+                    c->d_gC.as<double>(), n, &c->d_gemm_ws));
+    YG_TRY(yg_dgemm(c->stream, 1, 0, n, n, n, -1.0, c->d_gB.as<double>(), n, c->d_gC.as<double>(), n, 0.0,
+                    c->d_Fcur.as<double>(), n, &c->d_gemm_ws));
+    const int method = c->density_method;
+    c->density_method = YGPU_DENSITY_EIGEN;
+    c->fcur_valid = 1;
+    const int st = ygpu_build_density(c, n_occ, nullptr, nullptr, D, Cocc, eigval, nullptr);
+    c->fcur_valid = 0;
+    c->density_method = method;
+    return st;
+} catch (...) {
+    return exception_status();
+}
+
+int ygpu_fock_energy(ygpu_ctx_t *c, double *e_dhf, double *e_ref)
+try {
+    YG_TRY(check_ctx(c));
+    if (!c->fock_valid) {
+        ygpu_set_error(YGPU_ERR_INVALID, "ygpu_fock_energy: no Fock build since the last density build");
+        return YGPU_ERR_INVALID;
+    }
+    if (e_dhf) *e_dhf = c->energy_dhf;
+    if (e_ref) *e_ref = c->energy_ref;
     return YGPU_OK;
 } catch (...) {
     return exception_status();

@@ -181,6 +181,39 @@ __global__ void k_fock(int nbf, const double *__restrict__ H, double *__restrict
     if (i > j) J[idx] = jv;
 }
 
+// Energy terms of a Fock build, block partials (fixed grid, fixed order): part[2b] = sum D (H + F),
+// part[2b + 1] = sum D (F_r - H) - D_r (F - H) when a reference (F_r, D_r) is given, each product
+// pair formed per element so that the near-cancelling terms cancel before they are summed
+constexpr int EN_BLOCKS = 512;
+__global__ __launch_bounds__(256) void k_energy_terms(long nn, const double *__restrict__ D,
+                                                      const double *__restrict__ H, const double *__restrict__ F,
+                                                      const double *__restrict__ Dr, const double *__restrict__ Fr,
+                                                      double *__restrict__ part)
+{
+    __shared__ double s1[4], s2[4];
+    double a = 0.0, b = 0.0;
+    for (long e = (long) blockIdx.x * 256 + threadIdx.x; e < nn; e += (long) EN_BLOCKS * 256) {
+        const double d = D[e], h = H[e], f = F[e];
+        a += d * (h + f);
+        if (Dr) b += d * (Fr[e] - h) - Dr[e] * (f - h);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        s1[w] = a;
+        s2[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+        part[2 * blockIdx.x + 1] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+    }
+}
+
 // mode 0: A[r][c] *= w[r]; mode 1: A[r][c] *= 1/sqrt(w[r])
 __global__ void k_scale_rows(int n, int ncol, double *__restrict__ A, int lda, const double *__restrict__ w, int mode)
 {
@@ -498,6 +531,16 @@ int yg_fock_assemble(hipStream_t s, int nbf, const double *H, double *J, const d
 {
     const long tot = (long) nbf * nbf;
     hipLaunchKernelGGL(k_fock, dim3((tot + 255) / 256), dim3(256), 0, s, nbf, H, J, K, F);
+    YG_HIP(hipGetLastError());
+    return YGPU_OK;
+}
+
+int yg_energy_blocks() { return EN_BLOCKS; }
+
+int yg_energy_terms(hipStream_t s, int nbf, const double *D, const double *H, const double *F, const double *Dr,
+                    const double *Fr, double *part)
+{
+    hipLaunchKernelGGL(k_energy_terms, dim3(EN_BLOCKS), dim3(256), 0, s, (long) nbf * nbf, D, H, F, Dr, Fr, part);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -114,6 +114,8 @@ struct ygpu_ctx {
     // of i rounded up to 32) values per row, in 32-deep k panels, and starts kofs[i] * (rows)
     // elements into its part
     DBuf d_kofs;
+    DBuf d_ktiles;                // (i, k0) of the 64-deep k tiles of every basis function's block
+    int nktiles = 0;
     DBuf d_xorder;                // basis functions by decreasing row length (X build order)
     long ksum = 0;                // sum of L_i
 
@@ -153,6 +155,10 @@ struct ygpu_ctx {
     int kref_valid = 0, kref_nocc = 0;
     DBuf d_Cref, d_Kref, d_C1, d_SC, d_Qa, d_Qb, d_Qt, d_PM32, d_nserr;
     double incr_align_err = 0.0;  // max |Q^T Q - I| of the last difference build
+    // F and D of the last full FP64 build (the reference term of a difference build's energy) and
+    // the energy terms of the last build (ygpu_fock_energy)
+    DBuf d_Fref, d_Dref, d_epart;
+    double energy_dhf = 0.0, energy_ref = 0.0;
     int incr_fallback = 0;        // the last YGPU_FOCK_INCR call fell back to a full FP64 build
     int last_mode = YGPU_FOCK_FP64;   // arithmetic the last Fock build ran
 
@@ -218,6 +224,11 @@ int yg_syrk_upper(hipStream_t s, int nbf, long kdim, const double *X, double *K,
 int yg_syrk_upper_split(int nbf, long kdim, long *kchunk_out);
 // yg_syrk_upper with the partials in a caller-provided buffer
 int yg_syrk_upper_ws(hipStream_t s, int nbf, long kdim, const double *X, double *K, double *part, long kchunk);
+// energy terms of a Fock build: yg_energy_blocks() pairs of block partials in part (see ygpu_fock.hip);
+// Dr, Fr NULL: no reference term
+int yg_energy_blocks();
+int yg_energy_terms(hipStream_t s, int nbf, const double *D, const double *H, const double *F, const double *Dr,
+                    const double *Fr, double *part);
 // the split-K partials of the upper tiles of X X^T only (nbf x nbf each, no reduction)
 int yg_syrk_upper_parts(hipStream_t s, int nbf, long kdim, const double *X, double *part, long kchunk);
 // SP2 purification (ygpu_purify.hip): X (n x ldx, ldx a multiple of 16) = X_0 from F (rows and
@@ -281,8 +292,10 @@ int yg_sph_coefficients(int l, double *T);
 int yg_aux_sph_setup(ygpu_ctx *c);
 int yg_aux_sph_cols(ygpu_ctx *c, hipStream_t s, long rows, const double *in, long ldin, double *out, long ldout);
 int yg_shift_tail_diag(hipStream_t s, int n, int n_orth, double *A, double shift);
-int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
-                      const int *xptr, const int *kofs, void *dst, size_t dpart);
+// the panel-major two-part blocks of src's gathered rows; ktiles: the (i, k0) 64-deep k tiles of every
+// basis function (ygpu_ctx::d_ktiles)
+int yg_gather_split_t(hipStream_t s, int nktiles, const int2 *ktiles, const float *src, long ld, int R,
+                      const int *idx, const int *xptr, const int *kofs, void *dst, size_t dpart);
 // X (two BF16 parts, nrow rows per basis function) from the panel-major blocks Cp (nrow rows)
 // and Bt (naux_pad rows) made by yg_gather_split_t; order: the basis functions, longest row first
 int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs, const int *order,

@@ -345,38 +345,48 @@ struct SLoad {
 // texture path, not the matrix cores, busy).
 __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict__ src, long ld, int R,
                                                         const int *__restrict__ idx, const int *__restrict__ xptr,
-                                                        const int *__restrict__ kofs, __bf16 *__restrict__ dst,
-                                                        size_t dpart)
+                                                        const int *__restrict__ kofs, const int2 *__restrict__ ktiles,
+                                                        __bf16 *__restrict__ dst, size_t dpart)
 {
-    // workgroup (row tile of 64, k tile of 64, function i); k tiles beyond L_i have nothing to do
-    const int i = blockIdx.z, r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    // workgroup (64-deep k tile (i, k0) of the list, 256 operand rows in four 64-row passes)
+    const int2 kt = ktiles[blockIdx.x];
+    const int i = kt.x, k0 = kt.y;
     const int kb = xptr[i], klen = xptr[i + 1] - kb, L = (klen + 31) & ~31;
-    if (k0 >= L) return;
     __bf16 *d0 = dst + (size_t) kofs[i] * R, *d1 = d0 + dpart;
     __shared__ float tile[64][65];
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-#pragma unroll 4
-    for (int kk = ty; kk < 64; kk += 4) {
-        const int k = k0 + kk;
-        tile[kk][tx] = (k < klen && r0 + tx < R) ? src[(size_t) idx[kb + k] * ld + r0 + tx] : 0.f;
-    }
+    __shared__ int rows[64];
+    if (threadIdx.x < 64) rows[threadIdx.x] = k0 + (int) threadIdx.x < klen ? idx[kb + k0 + threadIdx.x] : -1;
     __syncthreads();
-    const int rl = threadIdx.x >> 2, ks = (threadIdx.x & 3) * 16, r = r0 + rl;
-    if (r >= R) return;
-#pragma unroll
-    for (int g = 0; g < 2; ++g) {
-        const int k = k0 + ks + 8 * g;
-        if (k >= L) break;
-        v8bf h, l;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float x = tile[ks + 8 * g + e][rl];
-            h[e] = (__bf16) x;
-            l[e] = (__bf16) (x - (float) h[e]);
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int rl = threadIdx.x >> 2, ks = (threadIdx.x & 3) * 16;
+    for (int sub = 0; sub < 4; ++sub) {
+        const int r0 = blockIdx.y * 256 + sub * 64;
+        if (r0 >= R) break;
+#pragma unroll 4
+        for (int kk = ty; kk < 64; kk += 4) {
+            const int row = rows[kk];
+            tile[kk][tx] = (row >= 0 && r0 + tx < R) ? src[(size_t) row * ld + r0 + tx] : 0.f;
         }
-        const size_t off = (size_t) (k >> 5) * R * 32 + (size_t) r * 32 + (k & 31);
-        *(v8bf *) (d0 + off) = h;
-        *(v8bf *) (d1 + off) = l;
+        __syncthreads();
+        const int r = r0 + rl;
+        if (r < R) {
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                const int k = k0 + ks + 8 * g;
+                if (k >= L) break;
+                v8bf h, l;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float x = tile[ks + 8 * g + e][rl];
+                    h[e] = (__bf16) x;
+                    l[e] = (__bf16) (x - (float) h[e]);
+                }
+                const size_t off = (size_t) (k >> 5) * R * 32 + (size_t) r * 32 + (k & 31);
+                *(v8bf *) (d0 + off) = h;
+                *(v8bf *) (d1 + off) = l;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -743,16 +753,16 @@ int yg_xbuild_bf(hipStream_t s, int nparts, int nbf, int n_occ, int naux_pad, co
     return YGPU_ERR_INVALID;
 }
 
-int yg_gather_split_t(hipStream_t s, int nbf, int max_row_len, const float *src, long ld, int R, const int *idx,
-                      const int *xptr, const int *kofs, void *dst, size_t dpart)
+int yg_gather_split_t(hipStream_t s, int nktiles, const int2 *ktiles, const float *src, long ld, int R,
+                      const int *idx, const int *xptr, const int *kofs, void *dst, size_t dpart)
 {
-    if (nbf <= 0 || R <= 0 || max_row_len <= 0) return YGPU_OK;
-    if (nbf > 65535) {
-        ygpu_set_error(YGPU_ERR_INVALID, "operand blocks: %d basis functions exceed the grid limit", nbf);
+    if (nktiles <= 0 || R <= 0) return YGPU_OK;
+    if ((R + 255) / 256 > 65535) {
+        ygpu_set_error(YGPU_ERR_INVALID, "operand blocks: %d rows exceed the grid limit", R);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_gather_split_t, dim3((R + 63) / 64, (max_row_len + 63) / 64, nbf), dim3(256), 0, s, src, ld,
-                       R, idx, xptr, kofs, (__bf16 *) dst, dpart);
+    hipLaunchKernelGGL(k_gather_split_t, dim3(nktiles, (R + 255) / 256), dim3(256), 0, s, src, ld, R, idx, xptr, kofs,
+                       ktiles, (__bf16 *) dst, dpart);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

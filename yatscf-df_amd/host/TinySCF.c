@@ -346,42 +346,19 @@ void TinySCF_get_initial_guess(TinySCF_t T)
         printf("TinySCF SAD initial guess: D mat / Cocc mat used %lf, %lf (s)\n", t1 - t0, get_wtime_sec() - t2);
 }
 
+/* E = sum D (Hcore + F) (TinySCF.c:367-376), summed on the device.  Energies of a difference-density
+ * phase: with G = F - H = 2J - K the energy is 2 tr(D H) + tr(D G(D)).  A difference build forms
+ * G(D) = G_r + G~(D - D_r) from the full FP64 build's G_r = G(D_r) and a reduced-precision G~
+ * whose error e is of first order in the distance D - D_r, and tr(D G) carries
+ * tr(D_r e) + tr((D - D_r) e).  The Coulomb and exchange forms are symmetric, tr(A G(B)) =
+ * tr(B G(A)), so tr(D_r G(D - D_r)) = tr((D - D_r) G_r): the energy taken as
+ * tr(D (H + F)) + tr(D G_r) - tr(D_r G) is exact up to tr((D - D_r) e), which is of second
+ * order (ygpu_fock_energy's e_ref, 0 for other builds). */
 static void TinySCF_calc_energy(TinySCF_t T)
 {
-    double e = 0.0;
-    for (int i = 0; i < T->mat_size; i++) e += T->D_mat[i] * (T->F_mat[i] + T->Hcore_mat[i]);
-    T->HF_energy = e;
-}
-
-/*
- * Energies of a difference-density phase.  With G = F - H = 2J - K, the energy is
- * 2 tr(D H) + tr(D G(D)).  A difference build forms G(D) = G_r + G~(D - D_r) from the full FP64
- * build's G_r = G(D_r) and a reduced-precision G~ whose error e is of first order in the
- * distance D - D_r, and tr(D G) carries tr(D_r e) + tr((D - D_r) e).  The Coulomb and exchange
- * forms are symmetric, tr(A G(B)) = tr(B G(A)), so tr(D_r G(D - D_r)) = tr((D - D_r) G_r): the
- * energy taken as tr(D (H + F)) + tr(D G_r) - tr(D_r G) is exact up to tr((D - D_r) e), which is
- * of second order.  A full FP64 build of the phase becomes the reference (`used` is the
- * arithmetic the build ran: a difference build whose alignment fails runs in FP64).
- */
-static void TinySCF_reference_energy(TinySCF_t T, int used)
-{
-    const size_t ms = (size_t) T->mat_size;
-    if (used == YGPU_FOCK_FP64) {
-        if (!T->Gref_mat) T->Gref_mat = (double *) ALIGN64B_MALLOC(DBL_SIZE * ms);
-        if (!T->Dref_mat) T->Dref_mat = (double *) ALIGN64B_MALLOC(DBL_SIZE * ms);
-        if (!T->Gref_mat || !T->Dref_mat) {
-            fprintf(stderr, "TinySCF: reference matrices (%zu bytes) could not be allocated\n", 2 * DBL_SIZE * ms);
-            abort();
-        }
-        for (size_t i = 0; i < ms; i++) T->Gref_mat[i] = T->F_mat[i] - T->Hcore_mat[i];
-        memcpy(T->Dref_mat, T->D_mat, DBL_SIZE * ms);
-        T->ref_valid = 1;
-    } else if (used == YGPU_FOCK_INCR && T->ref_valid) {
-        double c = 0.0;
-        for (size_t i = 0; i < ms; i++)
-            c += T->D_mat[i] * T->Gref_mat[i] - T->Dref_mat[i] * (T->F_mat[i] - T->Hcore_mat[i]);
-        T->HF_energy += c;
-    }
+    double e = 0.0, eref = 0.0;
+    YS_GPU(ygpu_fock_energy((ygpu_ctx_t *) T->gpu, &e, &eref));
+    T->HF_energy = e + eref;
 }
 
 /* the DIIS history as TinySCF_init_free.c:173-180 leaves it */
@@ -423,7 +400,6 @@ void TinySCF_do_SCF(TinySCF_t T)
      * without it, full FP64 builds to convergence.  Energies of reduced-precision builds never
      * end the SCF. */
     const int use_incr = T->mixed && (T->mixed_schedule & 4);
-    T->ref_valid = 0;
     T->mixed_phase = T->mixed ? next_reduced_phase(T, YGPU_FOCK_FP64) : YGPU_FOCK_FP64;
     T->mixed_switch_iter = (T->mixed && T->mixed_phase == YGPU_FOCK_FP64) ? 0 : -1;
     T->mixed6_switch_iter = T->mixed_phase == YGPU_FOCK_MIXED6 ? 0 : -1;
@@ -439,7 +415,6 @@ void TinySCF_do_SCF(TinySCF_t T)
         if (T->mixed) YS_GPU(ygpu_fock_last_info(g, &used, NULL));   /* a difference build may fall back */
         if (T->iter < YS_MAX_PHASE_LOG) T->phase_log[T->iter] = used;
         TinySCF_calc_energy(T);
-        if (use_incr) TinySCF_reference_energy(T, used);
         delta = fabs(T->HF_energy - prev);
         prev = T->HF_energy;
         double st1 = get_wtime_sec();
@@ -486,7 +461,7 @@ void TinySCF_do_SCF(TinySCF_t T)
         } else if (use_incr) {
             /* FP64 and difference builds: a full FP64 build is the reference of the difference
              * builds that follow, whose energies are exact to second order in the distance from
-             * it (TinySCF_reference_energy), so two agreeing energies end the SCF.  Options:
+             * it (TinySCF_calc_energy), so two agreeing energies end the SCF.  Options:
              * mixed_reref > 0 takes a new reference once the DIIS error has fallen below that
              * fraction of its value at the last one; mixed_confirm ends the SCF only on a full
              * FP64 build or the difference build right after one, and confirms agreement among

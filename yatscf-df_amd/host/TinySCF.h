@@ -128,13 +128,10 @@ struct TinySCF_struct
                                or the difference build right after one (agreement among later
                                difference builds is confirmed by a full FP64 build); 0 (default):
                                any two agreeing FP64 or difference-build energies, whose
-                               reduced-precision error is of second order (see TinySCF_do_SCF) */
+                               reduced-precision error is of second order (see TinySCF_calc_energy) */
     int    purify;          /* 1 (default): density builds of the SCF iterations before the last
                                by SP2 purification (ygpu_set_density_method), the last by the
                                eigensolver; 0: the eigensolver throughout */
-    double *Gref_mat;       /* G = F - H of the last full FP64 build of a difference phase */
-    double *Dref_mat;       /* its density */
-    int    ref_valid;       /* Gref_mat / Dref_mat hold a reference */
     double mixed6_floor;    /* reduced-precision floor of the BF16x6 phase in units of
                                max_i K_ii (see TinySCF_do_SCF), default 1.5 * 2^-21 */
     int    mixed_phase;     /* Fock arithmetic of the current iteration: YGPU_FOCK_FP64,

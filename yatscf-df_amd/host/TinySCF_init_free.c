@@ -214,9 +214,6 @@ int TinySCF_try_init(TinySCF_t T, char *bas_fname, char *df_bas_fname, char *xyz
     T->R_mat = (double *) xalloc(DBL_SIZE * ms * MAX_DIIS, mem);
     T->B_mat = (double *) xalloc(DBL_SIZE * (MAX_DIIS + 1) * (MAX_DIIS + 1), mem);
     T->FDS_mat = (double *) xalloc(DBL_SIZE * ms, mem);
-    T->Gref_mat = NULL;          /* allocated by the first difference-density phase */
-    T->Dref_mat = NULL;
-    T->ref_valid = 0;
     T->DIIS_rhs = (double *) xalloc(DBL_SIZE * (MAX_DIIS + 1), mem);
     T->DIIS_ipiv = (int *) xalloc(INT_SIZE * (MAX_DIIS + 1), mem);
     memset(T->F0_mat, 0, DBL_SIZE * ms * MAX_DIIS);
@@ -282,8 +279,7 @@ void free_TinySCF(TinySCF_t T)
     void *ptrs[] = {T->Accum_Fock_buf, T->sp_scrval, T->bf_pair_scrval, T->df_sp_scrval, T->uniq_sp_lid,
                     T->uniq_sp_rid, T->shell_bf_sind, T->shell_bf_num, T->df_shell_bf_sind, T->df_shell_bf_num,
                     T->bf_pair_mask, T->bf_pair_j, T->bf_pair_diag, T->bf_mask_displs, T->F0_mat, T->R_mat,
-                    T->B_mat, T->FDS_mat, T->DIIS_rhs, T->DIIS_ipiv, T->temp_J, T->temp_K, T->pqA, T->df_tensor, T->Gref_mat,
-                    T->Dref_mat};
+                    T->B_mat, T->FDS_mat, T->DIIS_rhs, T->DIIS_ipiv, T->temp_J, T->temp_K, T->pqA, T->df_tensor};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) ALIGN64B_FREE(ptrs[i]);
     void *pinned[] = {T->Hcore_mat, T->S_mat, T->F_mat, T->D_mat, T->J_mat, T->K_mat, T->X_mat,
                       T->tmp_mat, T->Cocc_mat, T->eigval, T->ev_idx, T->Jpq};

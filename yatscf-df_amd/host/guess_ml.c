@@ -311,19 +311,8 @@ int ysml_assemble(const ysml_mol_t *M, int npairs, const int *pairs, const float
 /* C_occ and D = C_occ C_occ^T from the n_occ leading eigenvectors of S^1/2 D_in S^1/2 */
 int ysml_project_density(TinySCF_t T, const double *Din)
 {
-    ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
-    const int n = T->nbasfuncs;
-    const size_t ms = (size_t) T->mat_size;
-    double *SX = (double *) malloc(DBL_SIZE * ms), *W = (double *) malloc(DBL_SIZE * ms),
-           *P = (double *) malloc(DBL_SIZE * ms);
-    int st = (SX && W && P) ? 0 : 1;
-    /* S^1/2 = S X; P = -(S X)^T D (S X) so the lowest eigenvalues are the largest occupations */
-    if (!st) st = ygpu_dgemm(g, 0, 0, n, n, n, 1.0, T->S_mat, n, T->X_mat, n, 0.0, SX, n);
-    if (!st) st = ygpu_dgemm(g, 0, 0, n, n, n, 1.0, Din, n, SX, n, 0.0, W, n);
-    if (!st) st = ygpu_dgemm(g, 1, 0, n, n, n, -1.0, SX, n, W, n, 0.0, P, n);
-    if (!st) st = ygpu_build_density(g, T->n_occ, P, NULL, T->D_mat, T->Cocc_mat, T->eigval, T->ev_idx);
-    free(SX); free(W); free(P);
-    return st;
+    /* the n_occ leading eigenvectors of S^1/2 D S^1/2, on the device */
+    return ygpu_project_guess((ygpu_ctx_t *) T->gpu, T->n_occ, Din, T->D_mat, T->Cocc_mat, T->eigval);
 }
 
 /* ---- weights */
