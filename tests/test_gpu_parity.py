@@ -298,12 +298,47 @@ def s_shells(n):
                 xyz=np.zeros((n, 3)), alpha=np.ones(n), coef=np.ones(n))
 
 
+def wilkinson_plus(m):
+    """Eigenvalues of Wilkinson's W_{2m+1}^+ (diagonal |m - i|, off-diagonal 1), ascending: its
+    largest eigenvalues come in pairs that agree to many digits."""
+    k = 2 * m + 1
+    W = np.diag(np.abs(m - np.arange(k)).astype(float)) + np.diag(np.ones(k - 1), 1) + np.diag(np.ones(k - 1), -1)
+    return np.linalg.eigvalsh(W)
+
+
+def hard_occupied_levels(n, kind, rng):
+    """Occupied-level patterns that stress inverse iteration (ascending, in [-12, 0)):
+    wilkinson: -W_21^+ repeated (near-degenerate pairs); glued: W_21^+ copies 1e-10 apart
+    (clusters of near-equal levels); graded: levels 10^-k from 1e-9 to 1; tight: a run of 24
+    levels within 1e-13 at the top of the occupied block."""
+    if kind in ("wilkinson", "glued"):
+        w = -wilkinson_plus(10)[::-1]                   # 21 levels, pairs at the bottom
+        w = (w - w.min()) / (w.max() - w.min()) * 10.0 - 11.0
+        reps = (n + len(w) - 1) // len(w)
+        if kind == "wilkinson":
+            lev = np.concatenate([w + 0.045 * r for r in range(reps)])
+        else:
+            lev = np.concatenate([w + 1e-10 * r for r in range(reps)])
+        return np.sort(lev[:n])
+    if kind == "graded":
+        return -np.sort(np.logspace(-9, 0, n))[::-1] * 11.0
+    if kind == "tight":
+        lev = np.sort(rng.uniform(-12.0, -1.0, n))
+        t = min(24, n)
+        lev[n - t:] = -0.5 + 1e-13 * np.arange(t)
+        return lev
+    raise ValueError(kind)
+
+
 def spectrum_case(n, kind, seed):
     """Symmetric F' and n_occ with a gap of at least 0.5 above the occupied levels."""
     rng = np.random.RandomState(seed)
     nocc = max(1, n // 4)
     lam = np.sort(rng.uniform(-12.0, 3.0, n))
     lam[nocc:] += 0.5
+    if kind in ("wilkinson", "glued", "graded", "tight"):
+        lam[:nocc] = hard_occupied_levels(nocc, kind, rng)
+        lam[nocc:] = np.sort(rng.uniform(0.5, 3.5, n - nocc))
     if kind == "cluster" and nocc >= 4:
         lam[1:4] = lam[1]                       # an exactly degenerate occupied triple
     if kind == "diag":
@@ -319,7 +354,9 @@ def spectrum_case(n, kind, seed):
 @pytest.mark.parametrize("n,kind", [(1, "dense"), (2, "dense"), (3, "dense"), (17, "cluster"), (31, "dense"),
                                     (32, "dense"), (33, "diag"), (64, "cluster"), (200, "diag"), (255, "dense"),
                                     (271, "cluster"), (272, "dense"), (273, "dense"), (300, "cluster"),
-                                    (600, "cluster"), (1600, "cluster"), (2048, "dense")])
+                                    (600, "cluster"), (1600, "cluster"), (2048, "dense"),
+                                    (200, "wilkinson"), (272, "glued"), (240, "graded"), (260, "tight"),
+                                    (600, "wilkinson"), (700, "glued"), (900, "tight")])
 def test_density_eigensolver_sizes(ygpu, n, kind):
     log("case: density eigensolver n=%d %s" % (n, kind))
     F, nocc = spectrum_case(n, kind, 1000 + n)
@@ -340,6 +377,27 @@ def test_density_eigensolver_sizes(ygpu, n, kind):
     assert errD < 1e-11
     assert errO < 1e-12
     assert np.max(np.abs(C @ C.T - D)) < 1e-12
+
+
+@pytest.mark.parametrize("n,kind", [(200, "dense"), (260, "tight")])
+def test_density_small_route_residual_fallback(ygpu, monkeypatch, n, kind):
+    """With the residual bound of the one-workgroup route at 0 every inverse-iteration vector
+    fails its check, and the build must come from the general eigensolver (rocSOLVER dsyevd)
+    with the same eigenvalues and density."""
+    log("case: density small route forced to the general eigensolver n=%d %s" % (n, kind))
+    monkeypatch.setenv("YGPU_TRIDIAG_RESID", "0")
+    F, nocc = spectrum_case(n, kind, 1000 + n)
+    X = np.eye(n)
+    with make_engine(ygpu) as e:
+        e.set_basis(0, s_shells(n))
+        D, C, ev, idx = e.build_density(F, X, nocc)
+    w, V = np.linalg.eigh(F)
+    Dref = V[:, :nocc] @ V[:, :nocc].T
+    errE, errD = np.max(np.abs(ev - w)), np.max(np.abs(D - Dref))
+    log("  max|d eps| %.2e  max|dD| %.2e" % (errE, errD))
+    assert errE < 1e-12 * max(1.0, np.abs(w).max()) * max(1.0, n / 16)
+    assert errD < 1e-11
+    assert np.max(np.abs(C.T @ C - np.eye(nocc))) < 1e-12
 
 
 # S^-1/2 and metrics: one-workgroup reduction + dstedc up to 272, the blocked reduction +

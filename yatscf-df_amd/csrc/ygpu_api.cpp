@@ -1355,27 +1355,44 @@ try {
     // the eigensolver leaves the occupied vectors' V^T (n_occ x nbf) at the start of vsrc
     const double *vsrc = nullptr;
     if (used == YGPU_DENSITY_EIGEN) YG_TRY(ensure_syevd_workspace(c, nbf, nbf));
-    if (used == YGPU_DENSITY_PURIFY) {
-        // d_Cocc and d_VT written by density_purify
-    } else if (nbf <= YG_EIG_NMAX_SMALL) {
+    // eigensolver route: one workgroup up to YG_EIG_NMAX_SMALL, the blocked reduction up to
+    // YG_SYTRD_BLOCKED_NMAX, rocSOLVER dsyevd beyond (and when the small route's inverse
+    // iteration leaves a vector above its residual bound)
+    int route = used == YGPU_DENSITY_PURIFY ? -1 : nbf <= YG_EIG_NMAX_SMALL ? 0 : nbf <= YG_SYTRD_BLOCKED_NMAX ? 1 : 2;
+    if (route == 0) {
         // F' = Q T Q^T on one workgroup; all eigenvalues of T by multisection and its n_occ
         // lowest eigenvectors by inverse iteration; only those are carried back:
         // V_occ = Q Z_occ in place of Z_occ (columns of d_mat2, column-major)
         const size_t nvp = (size_t) (n_occ + 63) / 64 * 64;
         YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
         YG_TRY(c->d_tri.ensure((nvp + 2) * nbf * sizeof(double)));
+        YG_TRY(c->d_Fsave.ensure(ms));
+        YG_HIP(hipMemcpyAsync(c->d_Fsave.p, c->d_mat0.p, ms, hipMemcpyDeviceToDevice, c->stream));
         double *tri_ws = c->d_tri.as<double>();
         double *dT = tri_ws + nvp * nbf, *eT = dT + nbf;
-        YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), dT, eT, c->d_tau.as<double>()));
-        YG_TRY(yg_tridiag_eig(c->stream, nbf, n_occ, dT, eT, c->d_eig.as<double>(), c->d_mat2.as<double>(), tri_ws));
         YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
-        YG_TRY(yg_backtransform_cols(c->stream, nbf, n_occ, c->d_mat0.as<double>(), nbf, c->d_tau.as<double>(),
-                                     c->d_mat2.as<double>(), nbf));
-        // C_occ = X V_occ
-        YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
-                        c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
-        vsrc = c->d_mat2.as<double>();
-    } else if (nbf <= YG_SYTRD_BLOCKED_NMAX) {
+        YG_TRY(yg_sytrd_small(c->stream, nbf, c->d_mat0.as<double>(), dT, eT, c->d_tau.as<double>()));
+        YG_TRY(yg_tridiag_eig(c->stream, nbf, n_occ, dT, eT, c->d_eig.as<double>(), c->d_mat2.as<double>(), tri_ws,
+                              c->d_info.as<int>()));
+        int resid = 0;
+        YG_HIP(hipMemcpyAsync(&resid, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        YG_HIP(hipStreamSynchronize(c->stream));
+        if (resid != 0) {
+            // F' again into d_mat0 for the general eigensolver
+            YG_HIP(hipMemcpyAsync(c->d_mat0.p, c->d_Fsave.p, ms, hipMemcpyDeviceToDevice, c->stream));
+            YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
+            c->eig_resid_fallbacks++;
+            route = 2;
+        } else {
+            YG_TRY(yg_backtransform_cols(c->stream, nbf, n_occ, c->d_mat0.as<double>(), nbf, c->d_tau.as<double>(),
+                                         c->d_mat2.as<double>(), nbf));
+            // C_occ = X V_occ
+            YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
+                            c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
+            vsrc = c->d_mat2.as<double>();
+        }
+    }
+    if (route == 1) {
         // blocked reduction, all eigenpairs of T, Q applied to the n_occ lowest only: their
         // columns of d_mat2 (column-major) are the rows of V_occ^T
         YG_TRY(c->d_tau.ensure((size_t) nbf * sizeof(double)));
@@ -1386,7 +1403,8 @@ try {
         YG_TRY(yg_dgemm(c->stream, 0, 1, nbf, n_occ, nbf, 1.0, dX, nbf, c->d_mat2.as<double>(), nbf, 0.0,
                         c->d_Cocc.as<double>(), n_occ, &c->d_gemm_ws));
         vsrc = c->d_mat2.as<double>();
-    } else {
+    }
+    if (route == 2) {
         rocblas_status st = rocsolver_dsyevd(c->rb, rocblas_evect_original, rocblas_fill_upper, nbf,
                                              c->d_mat0.as<double>(), nbf, c->d_eig.as<double>(),
                                              c->d_evE.as<double>(), c->d_info.as<int>());

@@ -32,6 +32,8 @@
 // 81 slots and the phases' shared LDS addresses inside 256 VGPRs without spills.
 #include "ygpu_internal.h"
 
+#include <cstdlib>
+
 namespace {
 
 constexpr int RTHR = 512;        // tridiagonal reduction: one workgroup
@@ -622,12 +624,29 @@ __global__ __launch_bounds__(64) void k_tridiag_invit(int n, int nvec, int nvp, 
 
 constexpr int CLMAX = 64;        // clusters up to this size are orthogonalised in LDS
 
+// max_i |(T z - lambda z)_i| of z (element i at z[i * stride]) over wavefront lanes
+__device__ __forceinline__ double tri_residual(int n, const double *__restrict__ d, const double *__restrict__ e,
+                                               double lam, const double *z, long stride, int lane)
+{
+    double r = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        double tz = (d[i] - lam) * z[(size_t) i * stride];
+        if (i > 0) tz += e[i - 1] * z[(size_t) (i - 1) * stride];
+        if (i < n - 1) tz += e[i] * z[(size_t) (i + 1) * stride];
+        r = fmax(r, fabs(tz));
+    }
+    return wave_max(r);
+}
+
 // Gram-Schmidt (two passes) inside each cluster of eigenvalues closer than 1e-3 ||T||:
 // workgroup j works only if eigenvalue j starts a cluster, on that cluster, and writes its
-// rows Z[j'][i] = Zt[i][j']
+// rows Z[j'][i] = Zt[i][j'].  Every vector of the cluster is then checked, |T z - lambda z| <=
+// rtol ||T|| (inverse iteration has no convergence test of its own): the first failing vector j
+// sets *info = j + 1
 __global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp, const double *__restrict__ d,
                                                         const double *__restrict__ e, const double *__restrict__ w,
-                                                        double *__restrict__ Zt, double *__restrict__ Z)
+                                                        double *__restrict__ Zt, double *__restrict__ Z,
+                                                        int *__restrict__ info, double rtol)
 {
     __shared__ double cs[CLMAX * YG_EIG_NMAX_SMALL];
     __shared__ double sdot[CLMAX];
@@ -697,6 +716,10 @@ __global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp
             }
         }
         __syncthreads();
+        for (int v = wv; v < csz; v += NORTH / 64) {
+            const double r = tri_residual(n, d, e, w[j0 + v], cs + (size_t) v * n, 1, lane);
+            if (lane == 0 && !(r <= rtol * bd.tnorm)) atomicCAS(info, 0, j0 + v + 1);
+        }
         for (long idx = t; idx < (long) csz * n; idx += NORTH) Z[(size_t) j0 * n + idx] = cs[idx];
         return;
     }
@@ -720,6 +743,10 @@ __global__ __launch_bounds__(NORTH) void k_tridiag_orth(int n, int nvec, int nvp
         }
     }
     __syncthreads();
+    for (int v = wv; v < csz; v += NORTH / 64) {
+        const double r = tri_residual(n, d, e, w[j0 + v], Zt + j0 + v, nvp, lane);
+        if (lane == 0 && !(r <= rtol * bd.tnorm)) atomicCAS(info, 0, j0 + v + 1);
+    }
     for (long idx = t; idx < (long) csz * n; idx += NORTH) {
         const int v = (int) (idx / n), i = (int) (idx % n);
         Z[(size_t) (j0 + v) * n + i] = Zt[(size_t) i * nvp + j0 + v];
@@ -764,8 +791,12 @@ int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long 
 }
 
 int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double *e, double *w, double *Z,
-                   double *ws)
+                   double *ws, int *info)
 {
+    // the residual bound; YGPU_TRIDIAG_RESID in the environment replaces it (0 sends every
+    // small-route build to the general eigensolver, as the tests of that route do)
+    double rtol = YG_TRIDIAG_RESID;
+    if (const char *v = getenv("YGPU_TRIDIAG_RESID")) rtol = atof(v);
     if (n < 1 || n > YG_EIG_NMAX_SMALL || nvec < 1 || nvec > n) {
         ygpu_set_error(YGPU_ERR_INVALID, "tridiagonal eigensolver: n = %d, nvec = %d out of range", n, nvec);
         return YGPU_ERR_INVALID;
@@ -776,7 +807,7 @@ int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tridiag_invit, dim3((nvec + IVL - 1) / IVL), dim3(64), 0, s, n, nvec, nvp, d, e, w, Zt);
     YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tridiag_orth, dim3(nvec), dim3(NORTH), 0, s, n, nvec, nvp, d, e, w, Zt, Z);
+    hipLaunchKernelGGL(k_tridiag_orth, dim3(nvec), dim3(NORTH), 0, s, n, nvec, nvp, d, e, w, Zt, Z, info, rtol);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }

@@ -41,6 +41,9 @@ void ygpu_set_error(int code, const char *fmt, ...);
 // largest n of the blocked two-kernel tridiagonal reduction (two rows per thread of its
 // one-workgroup column kernel); rocSOLVER dsyevd beyond
 #define YG_SYTRD_BLOCKED_NMAX 2048
+// largest max |T z - lambda z| / ||T|| accepted from the small path's inverse iteration; a vector
+// beyond it sends the density build to the general eigensolver
+#define YG_TRIDIAG_RESID 1e-10
 // SP2 purification: step limit, idempotency threshold that starts the two closing steps, least
 // squared Cholesky pivot of V_prev^T P V_prev accepted for the occupied vectors
 #define YG_SP2_KMAX 96
@@ -162,8 +165,10 @@ struct ygpu_ctx {
     int incr_fallback = 0;        // the last YGPU_FOCK_INCR call fell back to a full FP64 build
     int last_mode = YGPU_FOCK_FP64;   // arithmetic the last Fock build ran
 
-    // own eigensolver paths: reflector scalars and tridiagonal workspace
-    DBuf d_tau, d_tri;
+    // own eigensolver paths: reflector scalars, tridiagonal workspace, the small route's copy of
+    // F' (for the general eigensolver when an inverse-iteration vector fails its residual bound)
+    DBuf d_tau, d_tri, d_Fsave;
+    int eig_resid_fallbacks = 0;
     // density by purification (ygpu_set_density_method): the occupied vectors V^T (n_occ x nbf,
     // orthogonal basis) of the last density build, the projector, the SP2 partials and logs, and
     // the work of the orthonormalisation V = P V_prev (V_prev^T P V_prev)^{-1/2}
@@ -273,9 +278,10 @@ int yg_backtransform_cols(hipStream_t s, int n, int nvec, const double *A, long 
                           long ldz);
 // all eigenvalues w (ascending) of the symmetric tridiagonal tridiag(e, d, e) and the
 // eigenvectors of the nvec lowest as the rows of Z (nvec x n); ws holds
-// n ((nvec + 63) / 64 * 64) doubles
+// n ((nvec + 63) / 64 * 64) doubles; *info (device, zero on entry) = j + 1 for the first vector j
+// whose residual exceeds YG_TRIDIAG_RESID ||T||
 int yg_tridiag_eig(hipStream_t s, int n, int nvec, const double *d, const double *e, double *w, double *Z,
-                   double *ws);
+                   double *ws, int *info);
 int yg_accum(hipStream_t s, long n, double *J, double *Jprev);                     // J += Jprev; Jprev = J
 int yg_maxabs(hipStream_t s, long n, const double *x, double *out);                // out[0] = max |x|
 // mixed precision (BF16-split MFMA, FP32 accumulation)
