@@ -336,11 +336,12 @@ def main():
         "config": {"workload": WORKLOADS[args.workload],
                    "guess": ("SAD + learned pair correction" if last["guess_ml"] else "SAD"),
                    "precision": ("BF16x3-split K (FP32 accumulation) and FP32-tensor incremental J until the DIIS "
-                                 "error reaches the split's floor; then a full FP64 reference build and "
+                                 "error reaches the split's floor; then one full FP64 reference build and "
                                  "difference-density builds K = K_ref + K(D - D_ref) (BF16x3 products of "
-                                 "aligned orbitals) with FP64 re-references; converged when two energies "
-                                 "agree and the later is a full FP64 build or the difference build right "
-                                 "after one" if mixed else "FP64 (plain path)"),
+                                 "aligned orbitals), whose energies are exact to second order in D - D_ref; "
+                                 "converged when two FP64 or difference-build energies agree; densities by "
+                                 "SP2 purification (FP64), the last by the eigensolver" if mixed else
+                                 "FP64 (plain path)"),
                    "ene_tol": settings["ene_tol"],
                    "diis_replace": "largest residual ever (reference)" if settings["quirk_a2"] else
                                    "largest stored residual",

@@ -869,12 +869,10 @@ try {
         // J = L L^T on the device, W = L^{-1} (lower triangular, identity on the padding), then
         // B = pqA W^T with the zeros of W above its diagonal skipped
         YG_TRY(c->d_info.ensure(sizeof(int)));
-        YG_TRY(c->d_mat2.ensure((size_t) ld * 64 * sizeof(double)));
+        YG_TRY(c->d_mat1.ensure(yg_chol_inv_ws((int) ld) * sizeof(double)));
         YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream));
-        YG_TRY(yg_potrf_lower(c->stream, c->naux, (int) ld, ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
-                              c->d_info.as<int>(), c->d_mat2.as<double>()));
-        YG_TRY(yg_trtri_lower(c->stream, (int) ld, ld, c->d_mat0.as<double>(), c->d_mat1.as<double>(),
-                              c->d_Jpq.as<double>(), ld, c->d_mat2.as<double>()));
+        YG_TRY(yg_chol_inv_lower(c->stream, c->naux, (int) ld, ld, c->d_mat0.as<double>(), c->d_Jpq.as<double>(), ld,
+                                 c->d_info.as<int>(), c->d_mat1.as<double>(), &c->d_gemm_ws));
         YG_HIP(hipEventRecord(c->ev[3], c->stream));
         YG_TRY(yg_dgemm_nt128(c->stream, (int) c->n_up, (int) ld, ld, c->d_pqA.as<double>(), ld,
                               c->d_Jpq.as<double>(), ld, c->d_B.as<double>(), ld, true));
@@ -1247,8 +1245,7 @@ static int density_purify(ygpu_ctx *c, int n_occ, const double *dX, int *ok)
     YG_TRY(c->d_VT2.ensure((size_t) n_occ * n * sizeof(double)));
     YG_TRY(c->d_Mo.ensure((size_t) nop * nop * sizeof(double)));
     YG_TRY(c->d_Mw.ensure((size_t) nop * nop * sizeof(double)));
-    YG_TRY(c->d_Mlinv.ensure((size_t) nop * 64 * sizeof(double)));
-    YG_TRY(c->d_MT.ensure((size_t) nop * 64 * sizeof(double)));
+    YG_TRY(c->d_MT.ensure(yg_chol_inv_ws(nop) * sizeof(double)));
     double *P = c->d_P.as<double>(), *misc = c->d_sp2misc.as<double>();
     YG_TRY(yg_sp2_begin(s, n, c->n_orth, c->d_mat0.as<double>(), n, P, ldx, c->d_sp2sum.as<double>(), misc));
     double elog[YG_SP2_KMAX];
@@ -1289,11 +1286,9 @@ static int density_purify(ygpu_ctx *c, int n_occ, const double *dX, int *ok)
                     c->d_Mo.as<double>(), nop, &c->d_gemm_ws));
     YG_TRY(c->d_info.ensure(sizeof(int)));
     YG_HIP(hipMemsetAsync(c->d_info.p, 0, sizeof(int), s));
-    YG_TRY(yg_potrf_lower(s, n_occ, nop, nop, c->d_Mo.as<double>(), c->d_Mlinv.as<double>(), c->d_info.as<int>(),
-                          c->d_MT.as<double>()));
+    YG_TRY(yg_chol_inv_lower(s, n_occ, nop, nop, c->d_Mo.as<double>(), c->d_Mw.as<double>(), nop,
+                             c->d_info.as<int>(), c->d_MT.as<double>(), &c->d_gemm_ws));
     YG_TRY(yg_min_diag_sq(s, n_occ, nop, c->d_Mo.as<double>(), misc + 2));
-    YG_TRY(yg_trtri_lower(s, nop, nop, c->d_Mo.as<double>(), c->d_Mlinv.as<double>(), c->d_Mw.as<double>(), nop,
-                          c->d_MT.as<double>()));
     YG_TRY(yg_dgemm(s, 0, 1, n_occ, n, n_occ, 1.0, c->d_Mw.as<double>(), nop, c->d_Y.as<double>(), n_occ, 0.0,
                     c->d_VT2.as<double>(), n, &c->d_gemm_ws));
     int info = 0;

@@ -4,15 +4,17 @@
 // and its Simint batch calls (CMS_Simint.c:474-566).  McMurchie-Davidson with
 // the aux shell as a single-centre Hermite expansion (the reference pairs each
 // aux shell with an exponent-0 unit shell, CMS_Simint.c:131-141).  Instantiated
-// per angular-momentum class (la >= lb <= 2, lc <= 3) and per Cartesian
-// component of P so that every Hermite index is a compile-time constant (the R
-// recursion stays in registers); one wavefront handles one screened shell pair
-// against 64 aux shells of one class, with the bra primitive-pair data (P
-// centre, exponent, coefficient product, Hermite coefficients) staged once per
-// workgroup in LDS.  Results go straight into the packed pair rows (i <= j).
+// per angular-momentum class (la >= lb <= 2, lc <= 3) and per group of Cartesian
+// components of P so that every Hermite index is a compile-time constant (the R
+// recursion stays in registers, and is formed once for the group); one
+// wavefront handles one screened shell pair against 64 aux shells of one class,
+// with the bra primitive-pair data (P centre, exponent, coefficient product,
+// Hermite coefficients) staged once per workgroup in LDS.  Results go straight
+// into the packed pair rows (i <= j).
 #include "ygpu_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -33,19 +35,47 @@ struct Eri3Args {
     int npp_max;
 };
 
-template <int LA, int LB, int LC, int KK>
+// Cartesian components of P computed together by one lane: as many as keep the accumulators of
+// the group within 48 doubles (measured: two for the (d p|*) classes, one for (d d|*)), the
+// components dealt evenly over the groups.  The Hermite integrals R of a primitive quadruple are
+// formed once per group instead of once per component.
+__host__ __device__ constexpr int eri3_groups(int la, int lb, int lc)
+{
+    const int nab = ncart(la) * ncart(lb), nc = ncart(lc);
+    const int cap = 48 / nab;
+    const int g = cap < 1 ? 1 : (cap > nc ? nc : cap);
+    return (nc + g - 1) / g;
+}
+__host__ __device__ constexpr int eri3_group_size(int la, int lb, int lc)
+{
+    const int ng = eri3_groups(la, lb, lc), nc = ncart(lc);
+    return (nc + ng - 1) / ng;
+}
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// components K0 .. K0 + G - 1 of P for one lane
+template <int LA, int LB, int LC, int K0, int G>
 __device__ __forceinline__ void eri3_body(const Eri3Args &g, const double *__restrict__ lds, int npp,
                                           const ShellRef &A, const ShellRef &B, int P)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), LAB = LA + LB, L = LAB + LC, D = L + 1;
     constexpr int NE1 = (LA + 1) * (LB + 1) * (LAB + 1);
     constexpr int STRIDE = 5 + 3 * NE1;
-    constexpr int cx = cart_pow(LC, KK, 0), cy = cart_pow(LC, KK, 1), cz = cart_pow(LC, KK, 2);
     constexpr int DW = LAB + 1;
     const ShellRef C = shell_ref(g.aux, P);
-    double acc[NA * NB];
+    double acc[G][NA * NB];
 #pragma unroll
-    for (int q = 0; q < NA * NB; ++q) acc[q] = 0.0;
+    for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int q = 0; q < NA * NB; ++q) acc[k][q] = 0.0;
     for (int kp = 0; kp < C.nprim; ++kp) {
         const double gam = g.aux.alpha[C.off + kp], cc = g.aux.coef[C.off + kp];
         double Ec[(LC + 1) * (LC + 1)];
@@ -57,73 +87,81 @@ __device__ __forceinline__ void eri3_body(const Eri3Args &g, const double *__res
             double R[D * D * D];
             herm_R<L>(alpha, rec[1] - C.x, rec[2] - C.y, rec[3] - C.z, g.tab, R);
             const double pref = rec[4] * cc * YG_TWO_PI_25 / (p * gam * sqrt(p + gam));
-            // W(t,u,v) = sum (-1)^{tau+nu+phi} Ec Ec Ec R[t+tau][u+nu][v+phi]
-            double W[DW * DW * DW];
+            const double *Ex = rec + 5, *Ey = rec + 5 + NE1, *Ez = rec + 5 + 2 * NE1;
+            static_for<0, G>([&](auto gi) {
+                constexpr int KK = K0 + decltype(gi)::value;
+                constexpr int cx = cart_pow(LC, KK, 0), cy = cart_pow(LC, KK, 1), cz = cart_pow(LC, KK, 2);
+                // W(t,u,v) = sum (-1)^{tau+nu+phi} Ec Ec Ec R[t+tau][u+nu][v+phi]
+                double W[DW * DW * DW];
 #pragma unroll
-            for (int t = 0; t <= LAB; ++t)
+                for (int t = 0; t <= LAB; ++t)
 #pragma unroll
-                for (int u = 0; u <= LAB - t; ++u)
+                    for (int u = 0; u <= LAB - t; ++u)
 #pragma unroll
-                    for (int v = 0; v <= LAB - t - u; ++v) {
+                        for (int v = 0; v <= LAB - t - u; ++v) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int tau = 0; tau <= cx; ++tau)
+#pragma unroll
+                                for (int nu = 0; nu <= cy; ++nu)
+#pragma unroll
+                                    for (int phi = 0; phi <= cz; ++phi) {
+                                        const double e = Ec[cx * (LC + 1) + tau] * Ec[cy * (LC + 1) + nu] *
+                                                         Ec[cz * (LC + 1) + phi];
+                                        const double sg = ((tau + nu + phi) & 1) ? -1.0 : 1.0;
+                                        s += sg * e * R[((t + tau) * D + u + nu) * D + v + phi];
+                                    }
+                            W[(t * DW + u) * DW + v] = s * pref;
+                        }
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ib = 0; ib < NB; ++ib) {
+                        const int ax = cart_pow(LA, ia, 0), ay = cart_pow(LA, ia, 1), az = cart_pow(LA, ia, 2);
+                        const int bx = cart_pow(LB, ib, 0), by = cart_pow(LB, ib, 1), bz = cart_pow(LB, ib, 2);
                         double s = 0.0;
 #pragma unroll
-                        for (int tau = 0; tau <= cx; ++tau)
+                        for (int t = 0; t <= ax + bx; ++t)
 #pragma unroll
-                            for (int nu = 0; nu <= cy; ++nu)
+                            for (int u = 0; u <= ay + by; ++u) {
+                                const double exy = Ex[(ax * (LB + 1) + bx) * (LAB + 1) + t] *
+                                                   Ey[(ay * (LB + 1) + by) * (LAB + 1) + u];
 #pragma unroll
-                                for (int phi = 0; phi <= cz; ++phi) {
-                                    constexpr int dummy = 0;
-                                    (void) dummy;
-                                    const double e = Ec[cx * (LC + 1) + tau] * Ec[cy * (LC + 1) + nu] * Ec[cz * (LC + 1) + phi];
-                                    const double sg = ((tau + nu + phi) & 1) ? -1.0 : 1.0;
-                                    s += sg * e * R[((t + tau) * D + u + nu) * D + v + phi];
-                                }
-                        W[(t * DW + u) * DW + v] = s * pref;
+                                for (int v = 0; v <= az + bz; ++v)
+                                    s += exy * Ez[(az * (LB + 1) + bz) * (LAB + 1) + v] * W[(t * DW + u) * DW + v];
+                            }
+                        acc[decltype(gi)::value][ia * NB + ib] += s;
                     }
-            const double *Ex = rec + 5, *Ey = rec + 5 + NE1, *Ez = rec + 5 + 2 * NE1;
-#pragma unroll
-            for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib) {
-                    constexpr int dummy2 = 0;
-                    (void) dummy2;
-                    const int ax = cart_pow(LA, ia, 0), ay = cart_pow(LA, ia, 1), az = cart_pow(LA, ia, 2);
-                    const int bx = cart_pow(LB, ib, 0), by = cart_pow(LB, ib, 1), bz = cart_pow(LB, ib, 2);
-                    double s = 0.0;
-#pragma unroll
-                    for (int t = 0; t <= ax + bx; ++t)
-#pragma unroll
-                        for (int u = 0; u <= ay + by; ++u) {
-                            const double exy = Ex[(ax * (LB + 1) + bx) * (LAB + 1) + t] * Ey[(ay * (LB + 1) + by) * (LAB + 1) + u];
-#pragma unroll
-                            for (int v = 0; v <= az + bz; ++v)
-                                s += exy * Ez[(az * (LB + 1) + bz) * (LAB + 1) + v] * W[(t * DW + u) * DW + v];
-                        }
-                    acc[ia * NB + ib] += s;
-                }
+            });
         }
     }
-    const int col = C.bf + KK;
     const bool same = (A.bf == B.bf);
 #pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
+    for (int k = 0; k < G; ++k) {
+        const int col = C.bf + K0 + k;
 #pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-            const int m = A.bf + ia, n = B.bf + ib;
-            if (same && ia > ib) continue;          // (m,n) and (n,m) share one packed row
-            const int lo = m < n ? m : n, hi = m < n ? n : m;
-            const int r = g.upmask[(size_t) lo * g.nbf + hi];
-            if (r >= 0) g.out[(size_t) r * g.ld + col] = acc[ia * NB + ib];
-        }
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                const int m = A.bf + ia, n = B.bf + ib;
+                if (same && ia > ib) continue;          // (m,n) and (n,m) share one packed row
+                const int lo = m < n ? m : n, hi = m < n ? n : m;
+                const int r = g.upmask[(size_t) lo * g.nbf + hi];
+                if (r >= 0) g.out[(size_t) r * g.ld + col] = acc[k][ia * NB + ib];
+            }
+    }
 }
 
-template <int LA, int LB, int LC, int KK>
+// group z of the components: K0 = z GS, the last group possibly shorter
+template <int LA, int LB, int LC, int Z>
 __device__ __forceinline__ void eri3_dispatch(const Eri3Args &g, const double *lds, int npp, const ShellRef &A,
-                                              const ShellRef &B, int P, int kk)
+                                              const ShellRef &B, int P, int z)
 {
-    if constexpr (KK < ncart(LC)) {
-        if (kk == KK) eri3_body<LA, LB, LC, KK>(g, lds, npp, A, B, P);
-        else eri3_dispatch<LA, LB, LC, KK + 1>(g, lds, npp, A, B, P, kk);
+    constexpr int GS = eri3_group_size(LA, LB, LC), NC = ncart(LC);
+    if constexpr (Z * GS < NC) {
+        constexpr int K0 = Z * GS, G = (NC - K0 < GS) ? NC - K0 : GS;
+        if (z == Z) eri3_body<LA, LB, LC, K0, G>(g, lds, npp, A, B, P);
+        else eri3_dispatch<LA, LB, LC, Z + 1>(g, lds, npp, A, B, P, z);
     }
 }
 
@@ -175,7 +213,7 @@ int launch_eri3_class(const Eri3Args &g0, hipStream_t s)
         ygpu_set_error(YGPU_ERR_INVALID, "3-centre class (%d%d|%d): %d primitive pairs exceed LDS", LA, LB, LC, g0.npp_max);
         return YGPU_ERR_INVALID;
     }
-    dim3 grid(g0.npairs, (g0.nP + 63) / 64, ncart(LC));
+    dim3 grid(g0.npairs, (g0.nP + 63) / 64, eri3_groups(LA, LB, LC));
     hipLaunchKernelGGL((k_eri3<LA, LB, LC>), grid, dim3(64), lds, s, g0);
     YG_HIP(hipGetLastError());
     return YGPU_OK;

@@ -449,16 +449,16 @@ static void launch_gemm_part(hipStream_t s, dim3 grid, const Operand &a, const O
 }
 
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
-             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws)
+             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws, int wg_target)
 {
     if (m <= 0 || n <= 0) return YGPU_OK;
     dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
     Operand a{A, lda, nullptr}, b{B, ldb, nullptr};
-    // fewer than 64 output tiles: split k into chunks of at least two stages, up to about 256
-    // workgroups in all, when a workspace is given
+    // fewer than wg_target / 4 output tiles: split k into chunks of at least two stages, up to
+    // about wg_target workgroups in all, when a workspace is given
     const int tiles = (int) (grid.x * grid.y);
-    if (ws && tiles < 64 && k >= 4 * BK) {
-        int nsplit = std::min(k / (2 * BK), (256 + tiles - 1) / tiles);
+    if (ws && tiles < wg_target / 4 && k >= 4 * BK) {
+        int nsplit = std::min(k / (2 * BK), (wg_target + tiles - 1) / tiles);
         int kchunk = (k + nsplit - 1) / nsplit;
         kchunk = (kchunk + BK - 1) / BK * BK;
         nsplit = (k + kchunk - 1) / kchunk;

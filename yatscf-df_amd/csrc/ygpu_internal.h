@@ -175,7 +175,7 @@ struct ygpu_ctx {
     int density_method = YGPU_DENSITY_EIGEN;
     int have_vt = 0, vt_nocc = 0;
     int density_last_method = YGPU_DENSITY_EIGEN, density_last_steps = 0, sp2_last_steps = 0;
-    DBuf d_VT, d_VT2, d_P, d_Y, d_Mo, d_Mlinv, d_Mw, d_MT, d_sp2part, d_sp2sum, d_sp2log, d_sp2slog, d_sp2misc;
+    DBuf d_VT, d_VT2, d_P, d_Y, d_Mo, d_Mw, d_MT, d_sp2part, d_sp2sum, d_sp2log, d_sp2slog, d_sp2misc;
 
     // device-resident hand-offs between the calls of one SCF iteration (NULL host arguments):
     // d_F, d_D, d_Cocc hold the last Fock build's F and its inputs (fock_valid); after a
@@ -202,23 +202,23 @@ int yg_launch_eri2(ygpu_ctx *c, double *dJpq, int ld);
 int yg_launch_eri3(ygpu_ctx *c, double *dpqA, hipStream_t s);
 
 // C[m][n] (ldc) = alpha * sum_k op(A)(m,k) op(B)(k,n) + beta * C, row-major, FP64 MFMA; with a
-// workspace ws, products of fewer than 64 output tiles split k over workgroups (partials in ws,
-// added in a fixed order; C must not overlap A, B or ws)
+// workspace ws, products of fewer than wg_target / 4 output tiles split k over about wg_target
+// workgroups (partials in ws, added in a fixed order; C must not overlap A, B or ws)
 int yg_dgemm(hipStream_t s, int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
-             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws = nullptr);
+             const double *B, int ldb, double beta, double *C, int ldc, DBuf *ws = nullptr, int wg_target = 256);
 // C [m][n] = A [m][k] B [n][k]^T with k-contiguous rows (k, lda, ldb multiples of 8), 128x128 tiles;
 // lower_b: B lower triangular (its zeros above the diagonal are skipped, half the flops at n = k)
 int yg_dgemm_nt128(hipStream_t s, int m, int n, long k, const double *A, long lda, const double *B, long ldb,
                    double *C, long ldc, bool lower_b = false);
-// blocked Cholesky A = L L^T (lower, row-major, order n padded to npad, a multiple of 64, with
-// the padding rows / columns zero on entry): L in the lower triangle of A, the inverses of the
-// 64 x 64 diagonal blocks in Linv (npad / 64 blocks of 64 x 64); a non-positive pivot sets
-// *info (device) to its column + 1; T holds npad x 64 doubles of workspace
-int yg_potrf_lower(hipStream_t s, int n, int npad, long lda, double *A, double *Linv, int *info, double *T);
-// W = L^{-1} (npad x npad, lower triangular, zeros above the diagonal) from yg_potrf_lower's L and
-// Linv; T holds 64 x npad doubles of workspace
-int yg_trtri_lower(hipStream_t s, int npad, long lda, const double *L, const double *Linv, double *W, long ldw,
-                   double *T);
+// Cholesky A = L L^T (lower, row-major, order n padded to npad, a multiple of 64, with the
+// padding rows / columns zero on entry) and W = L^{-1}: L in the lower triangle of A, W (npad x
+// npad, leading dimension ldw, zero above the diagonal, identity on the padding); a non-positive
+// pivot sets *info (device, zero on entry) to its column + 1; T holds yg_chol_inv_ws(npad)
+// doubles of workspace, ws the split-K partials of the inverse's products.  lda and ldw
+// multiples of 8
+size_t yg_chol_inv_ws(int npad);
+int yg_chol_inv_lower(hipStream_t s, int n, int npad, long lda, double *A, double *W, long ldw, int *info, double *T,
+                      DBuf *ws);
 // X[i] = Cocc(partners of row i)^T * Bup(rows of i) for every basis function i
 int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, const int *xup,
               const int *xpartner, const double *Cocc, const double *B, double *X, int max_row_len);
