@@ -395,50 +395,60 @@ __global__ __launch_bounds__(256) void k_gather_split_t(const float *__restrict_
 #define YG_LDSP(p) ((__attribute__((address_space(3))) void *) (p))
 
 // LDS ring of the panel engine: NS stages of one 32-deep k panel of the (64 BI)-row A range and
-// the (64 BJ)-row B range, two BF16 parts each, in 16-byte slots
-template <int BI, int BJ> struct PsStage {
-    static constexpr int BM = 64 * BI, BN = 64 * BJ;
-    static constexpr int SA = 2 * BM * 4, SB = 2 * BN * 4, SLOTS = SA + SB, PW = SLOTS / 256;
-    static_assert(SLOTS % 256 == 0, "a stage is whole wavefront instructions of every wavefront");
+// the (32 BJ WN)-row B range, two BF16 parts each, in 16-byte slots; 2 WN wavefronts
+template <int BI, int BJ, int WN> struct PsStage {
+    static constexpr int NW = 2 * WN, BM = 64 * BI, BN = 32 * BJ * WN;
+    static constexpr int SA = 2 * BM * 4, SB = 2 * BN * 4, SLOTS = SA + SB, PW = SLOTS / (64 * NW);
+    static_assert(SA % (64 * NW) == 0 && SB % (64 * NW) == 0 && BN % 16 == 0,
+                  "each operand part of a stage is whole wavefront copies of every wavefront");
 };
 
-// Panel engine (ring: NS stages of PsStage<BI, BJ>::SLOTS slots in LDS): acc = A B^T for one
-// (64 BI) x (64 BJ) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.  A and B are panel-major: panel p of an operand with R rows holds
+// Panel engine (ring: NS stages of PsStage<BI, BJ, WN>::SLOTS slots in LDS): acc = A B^T for one
+// (64 BI) x (32 BJ WN) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.  A and B are panel-major: panel p of an operand with R rows holds
 // the R rows one after the other, 32 k each, at p * 32 R; its two BF16 parts are apart / bpart
-// elements apart.  Four wavefronts (2 x 2), each with a (32 BI) x (32 BJ) register tile of 32x32
-// MFMA blocks (BI = 5, BJ = 3: 240 accumulators, one wavefront per SIMD), so every 16-byte LDS
-// operand read feeds BJ (or BI) block products.  A stage is copied global -> LDS by the texture
+// elements apart.  2 x WN wavefronts, each with a (32 BI) x (32 BJ) register tile of 32x32 MFMA
+// blocks (WN = 2, BI = 5, BJ = 3: 240 accumulators, one wavefront per SIMD; WN = 4, BJ = 2: 160,
+// two per SIMD), so every 16-byte LDS operand read feeds BJ (or BI) block products.  A stage is copied global -> LDS by the texture
 // path's direct loads (global_load_lds_dwordx4, no registers), each wavefront instruction one
 // contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while stage s is
 // multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
 // rows of one operand read over all banks.  Rows beyond an operand are read from its last row
 // (their results are not stored).
-template <int BI, int BJ, int NS>
+template <int BI, int BJ, int WN, int NS>
 __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A,
                                         size_t apart, int RA, int m0, const __bf16 *__restrict__ B, size_t bpart,
                                         int RB, int n0, int nst, v16f (&acc)[BI][BJ])
 {
-    using P = PsStage<BI, BJ>;
-    constexpr int BM = P::BM, BN = P::BN, SA = P::SA, PW = P::PW;
+    using P = PsStage<BI, BJ, WN>;
+    constexpr int BM = P::BM, BN = P::BN, SA = P::SA, PW = P::PW, NW = P::NW;
     static_assert((NS - 2) * PW <= 63, "vmcnt range");
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wm = (w >> 1) * 32 * BI, wn = (w & 1) * 32 * BJ;
-    // slot t of an operand part: row t / 4, swizzled group (t % 4); source group g = that XOR
+    // the wavefront index as a scalar: the LDS destination and operand panel of each of its
+    // copies are uniform, only the row and group within the panel vary by lane
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (w / WN) * 32 * BI, wn = (w % WN) * 32 * BJ;
+    // wavefront copy j of a stage fills slots 64 j .. 64 j + 63: the first SA / (64 NW) of each
+    // wavefront's copies are A's, the rest B's.  Slot t = 64 q + lane of an
+    // operand part holds row t / 4 = 16 q + lane / 4 and, at position t % 4 = lane % 4, its
+    // 8-k group (lane % 4) XOR ((row / 2) % 4) = (lane % 4) XOR ((lane / 8) % 4)
+    const int lr = lane >> 2, lg = 8 * ((lane & 3) ^ ((lane >> 3) & 3));
     auto issue = [&](int slot, int p) {
         v8bf *dst = ring + slot * P::SLOTS;
 #pragma unroll
         for (int jj = 0; jj < PW; ++jj) {
-            const int j = 4 * jj + w, sl = 64 * j + lane;
-            const __bf16 *src;
-            if (sl < SA) {
-                const int part = sl / (BM * 4), t = sl % (BM * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
-                src = A + part * apart + (size_t) p * RA * 32 + (size_t) min(m0 + r, RA - 1) * 32 + 8 * g;
+            const int j = NW * jj + w;
+            const __bf16 *base;
+            int row;
+            if (jj < SA / (64 * NW)) {
+                const int part = j / (BM / 16), q = j % (BM / 16);
+                base = A + part * apart + (size_t) p * RA * 32;
+                row = min(m0 + 16 * q + lr, RA - 1);
             } else {
-                const int s2 = sl - SA;
-                const int part = s2 / (BN * 4), t = s2 % (BN * 4), r = t >> 2, g = (t & 3) ^ ((r >> 1) & 3);
-                src = B + part * bpart + (size_t) p * RB * 32 + (size_t) min(n0 + r, RB - 1) * 32 + 8 * g;
+                const int j2 = j - SA / 64, part = j2 / (BN / 16), q = j2 % (BN / 16);
+                base = B + part * bpart + (size_t) p * RB * 32;
+                row = min(n0 + 16 * q + lr, RB - 1);
             }
-            __builtin_amdgcn_global_load_lds((const void *) src, YG_LDSP(dst + 64 * j), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *) (base + (unsigned) (row * 32 + lg)), YG_LDSP(dst + 64 * j),
+                                             16, 0, 0);
         }
     };
 #pragma unroll
@@ -496,11 +506,11 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
 
 // store(m, n, v) for every element of this wavefront's part of a ps_tile result (m, n relative
 // to the workgroup tile)
-template <int BI, int BJ, class F>
+template <int BI, int BJ, int WN, class F>
 __device__ __forceinline__ void ps_epilogue(const v16f (&acc)[BI][BJ], F &&store)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wm = (w >> 1) * 32 * BI, wn = (w & 1) * 32 * BJ, rl = lane & 31, hk = lane >> 5;
+    const int wm = (w / WN) * 32 * BI, wn = (w % WN) * 32 * BJ, rl = lane & 31, hk = lane >> 5;
 #pragma unroll
     for (int x = 0; x < BI; ++x)
 #pragma unroll
@@ -515,14 +525,15 @@ __device__ __forceinline__ void ps_epilogue(const v16f (&acc)[BI][BJ], F &&store
 // panel-major over the basis functions for the products that follow: element k = m naux_pad + n
 // of function i at (k / 32) * 32 nbf + 32 i + k % 32, so a 32-deep k panel of any range of
 // functions is one contiguous stretch.
-template <int BI, int BJ, int NS>
-__global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xptr, const int *__restrict__ kofs,
+template <int BI, int BJ, int WN, int NS>
+__global__ __launch_bounds__(128 * WN, 1) void k_xbuild_ps(const int *__restrict__ xptr, const int *__restrict__ kofs,
                                                       const int *__restrict__ order, const __bf16 *__restrict__ Cp,
                                                       size_t cpart, const __bf16 *__restrict__ Bt, size_t btpart,
                                                       __bf16 *__restrict__ Xp, size_t xpart, int nrow, int naux_pad,
                                                       int ncolt, int nrowt, int nbf)
 {
-    __shared__ __attribute__((aligned(16))) v8bf ring[NS][PsStage<BI, BJ>::SLOTS];
+    using P = PsStage<BI, BJ, WN>;
+    __shared__ __attribute__((aligned(16))) v8bf ring[NS][P::SLOTS];
     // basis functions in order of decreasing row length (order[]), dealt round-robin over the 8
     // XCDs (the hardware deals workgroup b to XCD b % 8), every tile of the o-th function on
     // XCD o % 8: the row lengths, which vary 4-5x between functions and run in long stretches
@@ -536,20 +547,56 @@ __global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xp
     const int rem = (int) (q % tiles), ct = rem / nrowt, rt = rem % nrowt;
     const int L = (xptr[i + 1] - xptr[i] + 31) & ~31;
     const size_t ko = (size_t) kofs[i];
-    const int m0 = rt * 64 * BI, n0 = ct * 64 * BJ;
+    const int m0 = rt * P::BM, n0 = ct * P::BN;
     v16f acc[BI][BJ];
-    ps_tile<BI, BJ, NS>(&ring[0][0], Cp + ko * nrow, cpart, nrow, m0, Bt + ko * naux_pad, btpart, naux_pad, n0, L / 32, acc);
+    ps_tile<BI, BJ, WN, NS>(&ring[0][0], Cp + ko * nrow, cpart, nrow, m0, Bt + ko * naux_pad, btpart, naux_pad, n0,
+                            L / 32, acc);
+    // the tile leaves through the (now free) ring: each wavefront writes one 32-row block of its
+    // accumulators as FP32 rows into its own part of LDS, then reads them back as runs of 8
+    // columns, splits them and stores 16 bytes of each part per lane (four lanes fill one 32-k
+    // group of a row), instead of one 2-byte store per element and part
+    constexpr int CW = 32 * BJ, LS = CW + 8;   // row stride: the two half-wavefronts' rows 4 apart land in different banks
+    static_assert(P::NW * 32 * LS * 4 <= (int) sizeof(ring), "epilogue staging fits the ring");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rl = lane & 31, hk = lane >> 5;
+    const int wm = (w / WN) * 32 * BI, wn = (w % WN) * CW;
+    float *stg = (float *) &ring[0][0] + w * 32 * LS;
     __bf16 *X0 = Xp + 32 * (size_t) i;
-    ps_epilogue<BI, BJ>(acc, [&](int r, int c, float val) {
-        const int m = m0 + r, n = n0 + c;
-        if (m < nrow && n < naux_pad) {
-            const long k = (long) m * naux_pad + n;
-            const size_t off = (size_t) (k >> 5) * 32 * nbf + (k & 31);
-            const __bf16 h = (__bf16) val;
-            X0[off] = h;
-            X0[xpart + off] = (__bf16) (val - (float) h);
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < BI; ++x) {
+#pragma unroll
+        for (int y = 0; y < BJ; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                stg[(8 * (v >> 2) + 4 * hk + (v & 3)) * LS + 32 * y + rl] = acc[x][y][v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        constexpr int NCH = CW / 8;             // runs of 8 columns in a row
+#pragma unroll
+        for (int t0 = 0; t0 < 32 * NCH; t0 += 64) {
+            const int t = t0 + lane, r = t / NCH, c = t % NCH;
+            const int m = m0 + wm + 32 * x + r, n = n0 + wn + 8 * c;
+            if (t < 32 * NCH && m < nrow && n < naux_pad) {
+                const float4 u0 = *(const float4 *) (stg + r * LS + 8 * c);
+                const float4 u1 = *(const float4 *) (stg + r * LS + 8 * c + 4);
+                const float u[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+                v8bf h, l;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    h[e] = (__bf16) u[e];
+                    l[e] = (__bf16) (u[e] - (float) h[e]);
+                }
+                const long k = (long) m * naux_pad + n;
+                const size_t off = (size_t) (k >> 5) * 32 * nbf + (k & 31);
+                *(v8bf *) (X0 + off) = h;
+                *(v8bf *) (X0 + xpart + off) = l;
+            }
         }
-    });
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 // FP32 partials of A B^T over k chunks, A and B panel-major with R rows (the products of the
@@ -559,13 +606,14 @@ __global__ __launch_bounds__(256, 1) void k_xbuild_ps(const int *__restrict__ xp
 // elements (m, n < R) to part[z] (R x R, FP32).  The chunks are dealt to the XCDs (the hardware
 // deals workgroup b to XCD b % 8), every tile of chunk z on XCD z % 8 one after the other, so the
 // tiles of a chunk run side by side and read its panels from the one L2 (nchunk is a multiple of 8).
-template <int BI, int BJ, int NS, bool UPPER>
-__global__ __launch_bounds__(256, 1) void k_pgemm_ps(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
+template <int BI, int BJ, int WN, int NS, bool UPPER>
+__global__ __launch_bounds__(128 * WN, 1) void k_pgemm_ps(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
                                                      size_t xpart, int R, long npan, long cp, int ncolt, int ntiles,
                                                      float *__restrict__ part)
 {
-    __shared__ __attribute__((aligned(16))) v8bf ring[NS][PsStage<BI, BJ>::SLOTS];
-    constexpr int BM = 64 * BI, BN = 64 * BJ;
+    using P = PsStage<BI, BJ, WN>;
+    __shared__ __attribute__((aligned(16))) v8bf ring[NS][P::SLOTS];
+    constexpr int BM = P::BM, BN = P::BN;
     const long s = blockIdx.x / 8;
     const long z = (s / ntiles) * 8 + blockIdx.x % 8;
     int tile = (int) (s % ntiles), rt = 0, ct = 0;
@@ -588,11 +636,11 @@ __global__ __launch_bounds__(256, 1) void k_pgemm_ps(const __bf16 *__restrict__ 
     const size_t po = (size_t) (p0 < npan ? p0 : 0) * 32 * R;
     const int m0 = rt * BM, n0 = ct * BN;
     v16f acc[BI][BJ];
-    ps_tile<BI, BJ, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, acc);
-    float *P = part + (size_t) z * R * R;
-    ps_epilogue<BI, BJ>(acc, [&](int r, int c, float v) {
+    ps_tile<BI, BJ, WN, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, acc);
+    float *Pz = part + (size_t) z * R * R;
+    ps_epilogue<BI, BJ, WN>(acc, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
-        if (m < R && n < R) P[(size_t) m * R + n] = v;
+        if (m < R && n < R) Pz[(size_t) m * R + n] = v;
     });
 }
 
@@ -772,19 +820,25 @@ int yg_gather_split_t(hipStream_t s, int nktiles, const int2 *ktiles, const floa
     return YGPU_OK;
 }
 
-template <int BI, int BJ, int NS>
+template <int BI, int BJ, int WN, int NS>
 static int launch_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr, const int *kofs,
                             const int *order, const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
 {
-    const int ncolt = (naux_pad + 64 * BJ - 1) / (64 * BJ), nrowt = (nrow + 64 * BI - 1) / (64 * BI);
+    using P = PsStage<BI, BJ, WN>;
+    const int ncolt = (naux_pad + P::BN - 1) / P::BN, nrowt = (nrow + P::BM - 1) / P::BM;
     const size_t xpart = (size_t) nbf * nrow * naux_pad;
     // 8 XCD lanes of ceil(nbf / 8) functions each (functions past nbf: empty workgroups)
     const long total = 8L * ((nbf + 7) / 8) * ncolt * nrowt;
+    if (naux_pad % 32) {
+        // the tile stores runs of 8 columns, inside one 32-k group of a row
+        ygpu_set_error(YGPU_ERR_INVALID, "X build: padded aux dimension %d is not a multiple of 32", naux_pad);
+        return YGPU_ERR_INVALID;
+    }
     if (total > 0x7fffffffL) {
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL((k_xbuild_ps<BI, BJ, NS>), dim3((unsigned) total), dim3(256), 0, s, xptr, kofs, order,
+    hipLaunchKernelGGL((k_xbuild_ps<BI, BJ, WN, NS>), dim3((unsigned) total), dim3(64 * P::NW), 0, s, xptr, kofs, order,
                        (const __bf16 *) Cp, cpart, (const __bf16 *) Bt, btpart, (__bf16 *) X, xpart, nrow, naux_pad,
                        ncolt, nrowt, nbf);
     YG_HIP(hipGetLastError());
@@ -795,7 +849,7 @@ int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr
                  const void *Cp, size_t cpart, const void *Bt, size_t btpart, void *X)
 {
     if (nbf <= 0 || nrow <= 0) return YGPU_OK;
-#define YG_XPS(BI) launch_xbuild_ps<BI, 3, 2>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
+#define YG_XPS(BI) launch_xbuild_ps<BI, 2, 4, 2>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
     // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD); two ring
     // stages (a deeper ring measured no faster at 320 x 192)
     if (nrow <= 64) return YG_XPS(1);
@@ -826,7 +880,7 @@ template <bool UPPER>
 static int pgemm_ps(hipStream_t s, int R, long kdim, const __bf16 *A, const __bf16 *B, size_t xpart, DBuf &part,
                     int *nchunk_out)
 {
-    constexpr int BI = 5, BJ = 3, NS = 2, BM = 64 * BI, BN = 64 * BJ;
+    constexpr int BI = 5, BJ = 2, WN = 4, NS = 2, BM = PsStage<BI, BJ, WN>::BM, BN = PsStage<BI, BJ, WN>::BN;
     const int nrowt = (R + BM - 1) / BM, ncolt = (R + BN - 1) / BN;
     int ntiles = 0;
     if (UPPER) {
@@ -846,7 +900,7 @@ static int pgemm_ps(hipStream_t s, int R, long kdim, const __bf16 *A, const __bf
     // (and an R x R FP64 matrix after the partials for the reduction of the difference build)
     YG_TRY(part.ensure((size_t) nchunk * R * R * sizeof(float) + (size_t) R * R * sizeof(double) + 256));
     const long total = 8L * c * ntiles;
-    hipLaunchKernelGGL((k_pgemm_ps<BI, BJ, NS, UPPER>), dim3((unsigned) total), dim3(256), 0, s, A, B, xpart, R, npan,
+    hipLaunchKernelGGL((k_pgemm_ps<BI, BJ, WN, NS, UPPER>), dim3((unsigned) total), dim3(64 * 2 * WN), 0, s, A, B, xpart, R, npan,
                        cp, ncolt, ntiles, part.as<float>());
     YG_HIP(hipGetLastError());
     *nchunk_out = nchunk;
