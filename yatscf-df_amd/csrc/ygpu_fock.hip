@@ -3,10 +3,10 @@
 // J (build_Fock.c:37-130):
 //   pass 1 "aux. J tensor": temp_J[p] = sum_r w_r B[r][p] over the packed rows
 //     r = (i <= j), w_r = D_ii on the diagonal and 2 D_ij off it; a grid of
-//     (64-column tile) x (row group) workgroups streams each row segment once,
-//     reduces its four wavefronts in LDS and leaves one partial per row group,
-//     summed in a fixed order (deterministic; replaces the per-thread buffers
-//     and tree reduction of :15-33, :61-98);
+//     (column block) x (contiguous row range) workgroups streams each row once,
+//     four wavefronts on adjacent column tiles of the same rows, and leaves one
+//     partial per row range, summed in a fixed order (deterministic; replaces
+//     the per-thread buffers and tree reduction of :15-33, :61-98);
 //   pass 2 "J matrix": J_ij = B[r] . temp_J with temp_J staged in LDS, four
 //     rows in flight per wavefront (each temp_J value read once for the four),
 //     cross-lane reductions with shuffles.
@@ -42,73 +42,85 @@ __device__ __forceinline__ void load_cols(const TB *p, double (&v)[cpl<TB>()])
     }
 }
 
-// pass 1: partial[g][c] = sum over the rows of row group g of w_r B[r][c]; grid
-// (column tiles of 64 lanes x cpl columns, row groups); rows are interleaved over
-// (group, wavefront) so every wavefront streams whole 1 KB row segments, and
-// the row index is wave-uniform (scalar loads of the pair indices and of D).
+// w_r = D_ii on the diagonal pairs and 2 D_ij off it, for the packed rows r = (i <= j)
+__global__ void k_pair_weights(long n_up, const int *__restrict__ up_i, const int *__restrict__ up_j,
+                               const double *__restrict__ D, int nbf, double *__restrict__ wts)
+{
+    const long r = (long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_up) return;
+    const int i = up_i[r], j = up_j[r];
+    wts[r] = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int) b, l), hi = __builtin_amdgcn_readlane((int) (b >> 32), l);
+    return __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+
+// pass 1: partial[g][c] = sum of w_r B[r][c] over the contiguous rows r of row group g,
+// [g rpg, min(n_up, (g + 1) rpg)); grid (column blocks, row groups).  The JWAVES wavefronts of a
+// workgroup take adjacent column tiles of 64 lanes x cpl columns of the same rows, so a
+// workgroup reads JWAVES KiB of each row side by side, and the column blocks of one row group,
+// consecutive in the grid, cover its rows whole and walk them in order.  The weights of 64 rows
+// arrive with one load per wavefront and each row takes its own from the lane that holds it, so
+// the row loop carries no dependent load; four rows in flight per wavefront.
 // ld is a multiple of 64, so a lane's cpl columns are all inside or all outside.
 template <class TB>
-__global__ __launch_bounds__(64 * JWAVES) void k_tempj_part(const TB *__restrict__ B, long ld, long n_up,
-                                                            const int *__restrict__ up_i, const int *__restrict__ up_j,
-                                                            const double *__restrict__ D, int nbf,
-                                                            double *__restrict__ part)
+__global__ __launch_bounds__(64 * JWAVES) void k_tempj_part(const TB *__restrict__ B, long ld, long n_up, long rpg,
+                                                            const double *__restrict__ wts, double *__restrict__ part)
 {
     constexpr int C = cpl<TB>();
-    __shared__ double red[JWAVES][64 * C];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const long col = ((long) blockIdx.x * 64 + lane) * C;
-    const long stride = (long) gridDim.y * JWAVES;
-    const long r0 = (long) blockIdx.y * JWAVES + w;
+    const long col0 = ((long) blockIdx.x * JWAVES + w) * 64 * C;
+    if (col0 >= ld) return;                     // a whole wavefront beyond the last column
+    const long col = col0 + (long) lane * C;
+    const bool cin = col < ld;
+    const long r0 = (long) blockIdx.y * rpg, r1 = std::min(n_up, r0 + rpg);
     double a[4][C];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int c = 0; c < C; ++c) a[q][c] = 0.0;
-    const bool cin = col < ld;
-    long r = r0;
-    for (; r + 3 * stride < n_up; r += 4 * stride) {
-        double wt[4], bv[4][C];
+    for (long rb = r0; rb < r1; rb += 64) {
+        const double wv = rb + lane < r1 ? wts[rb + lane] : 0.0;
+        const int nr = (int) std::min<long>(64, r1 - rb);
+        const TB *Bb = B + (size_t) rb * ld + col;
+        int q0 = 0;
+        for (; q0 + 3 < nr; q0 += 4) {
+            double bv[4][C];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const long rq = __builtin_amdgcn_readfirstlane((int) (r + q * stride));
-            const int i = up_i[rq], j = up_j[rq];
-            wt[q] = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
-            if (cin) load_cols<TB>(B + (size_t) rq * ld + col, bv[q]);
+            for (int q = 0; q < 4; ++q) {
+                if (cin) load_cols<TB>(Bb + (size_t) (q0 + q) * ld, bv[q]);
+                else
+#pragma unroll
+                    for (int c = 0; c < C; ++c) bv[q][c] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double wt = readlane_f64(wv, q0 + q);
+#pragma unroll
+                for (int c = 0; c < C; ++c) a[q][c] += wt * bv[q][c];
+            }
+        }
+        for (; q0 < nr; ++q0) {
+            double bv[C];
+            if (cin) load_cols<TB>(Bb + (size_t) q0 * ld, bv);
             else
 #pragma unroll
-                for (int c = 0; c < C; ++c) bv[q][c] = 0.0;
-        }
+                for (int c = 0; c < C; ++c) bv[c] = 0.0;
+            const double wt = readlane_f64(wv, q0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < C; ++c) a[q][c] += wt[q] * bv[q][c];
-    }
-    for (; r < n_up; r += stride) {
-        const long rq = __builtin_amdgcn_readfirstlane((int) r);
-        const int i = up_i[rq], j = up_j[rq];
-        const double wt = (i == j ? 1.0 : 2.0) * D[(size_t) i * nbf + j];
-        double bv[C];
-        if (cin) load_cols<TB>(B + (size_t) rq * ld + col, bv);
-        else
-#pragma unroll
-            for (int c = 0; c < C; ++c) bv[c] = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; ++c) a[0][c] += wt * bv[c];
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) red[w][lane * C + c] = (a[0][c] + a[1][c]) + (a[2][c] + a[3][c]);
-    __syncthreads();
-    if (w == 0 && cin) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            double s = red[0][lane * C + c];
-#pragma unroll
-            for (int q = 1; q < JWAVES; ++q) s += red[q][lane * C + c];
-            part[(size_t) blockIdx.y * ld + col + c] = s;
+            for (int c = 0; c < C; ++c) a[0][c] += wt * bv[c];
         }
     }
+    if (cin)
+#pragma unroll
+        for (int c = 0; c < C; ++c) part[(size_t) blockIdx.y * ld + col + c] = (a[0][c] + a[1][c]) + (a[2][c] + a[3][c]);
 }
 
+// temp_J[c] = sum over the row groups of partial[g][c], in a fixed order (deterministic)
 __global__ void k_tempj_reduce(const double *__restrict__ part, int ngroup, long ld, double *__restrict__ tempJ)
 {
     const long col = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,29 +471,36 @@ int yg_lincomb(hipStream_t s, long len, const double *ys, long ystride, int ny, 
     return YGPU_OK;
 }
 
-long yg_jbuild_groups(long n_up, long naux_pad, bool fp32)
+// column blocks of the pass-1 grid: JWAVES tiles of 64 lanes x cpl columns
+static long tempj_colblocks(long naux_pad, bool fp32)
 {
     const int C = fp32 ? cpl<float>() : cpl<double>();
-    const int ncolt = (int) ((naux_pad + 64 * C - 1) / (64 * C));
-    // row groups: about 2048 workgroups in total, at least 16 rows per wavefront
-    long ngroup = std::max<long>(1, std::min<long>(2048 / ncolt, n_up / (16L * JWAVES)));
+    return (naux_pad + (long) JWAVES * 64 * C - 1) / ((long) JWAVES * 64 * C);
+}
+
+long yg_jbuild_groups(long n_up, long naux_pad, bool fp32)
+{
+    // row groups: about 2048 workgroups in total, at least 64 rows (one load of weights) per group
+    const long ngroup = std::max<long>(1, std::min<long>(2048 / tempj_colblocks(naux_pad, fp32), n_up / 64));
     return std::min<long>(ngroup, 65535);
 }
 
 int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, double *part,
-                 double *tempJ, long ngroup)
+                 double *tempJ, double *wts, long ngroup)
 {
     const long ld = c->naux_pad, n_up = c->n_up;
     const int nbf = c->nbf;
-    const int C = B32 ? cpl<float>() : cpl<double>();
-    const int ncolt = (int) ((ld + 64 * C - 1) / (64 * C));
+    const long ncb = tempj_colblocks(ld, B32 != nullptr), rpg = (n_up + ngroup - 1) / ngroup;
     YG_HIP(hipEventRecord(c->ev[0], s));
+    hipLaunchKernelGGL(k_pair_weights, dim3((unsigned) ((n_up + 255) / 256)), dim3(256), 0, s, n_up,
+                       c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, wts);
+    YG_HIP(hipGetLastError());
     if (B32)
-        hipLaunchKernelGGL(k_tempj_part<float>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s, B32, ld, n_up,
-                           c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, part);
+        hipLaunchKernelGGL(k_tempj_part<float>, dim3((unsigned) ncb, (unsigned) ngroup), dim3(64 * JWAVES), 0, s, B32,
+                           ld, n_up, rpg, wts, part);
     else
-        hipLaunchKernelGGL(k_tempj_part<double>, dim3(ncolt, ngroup), dim3(64 * JWAVES), 0, s,
-                           c->d_B.as<double>(), ld, n_up, c->d_up_i.as<int>(), c->d_up_j.as<int>(), dD, nbf, part);
+        hipLaunchKernelGGL(k_tempj_part<double>, dim3((unsigned) ncb, (unsigned) ngroup), dim3(64 * JWAVES), 0, s,
+                           c->d_B.as<double>(), ld, n_up, rpg, wts, part);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, part, (int) ngroup, ld, tempJ);
     YG_HIP(hipGetLastError());
@@ -509,7 +528,9 @@ int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const fl
     const long ngroup = yg_jbuild_groups(c->n_up, ld, B32 != nullptr);
     YG_TRY(c->d_tempJpart.ensure((size_t) ngroup * ld * sizeof(double)));
     YG_TRY(c->d_tempJ.ensure((size_t) ld * sizeof(double)));
-    YG_TRY(yg_jbuild_ws(c, s, dD, dJ, B32, c->d_tempJpart.as<double>(), c->d_tempJ.as<double>(), ngroup));
+    YG_TRY(c->d_jw.ensure((size_t) c->n_up * sizeof(double)));
+    YG_TRY(yg_jbuild_ws(c, s, dD, dJ, B32, c->d_tempJpart.as<double>(), c->d_tempJ.as<double>(), c->d_jw.as<double>(),
+                        ngroup));
     if (t_tempJ_ms || t_Jmat_ms) {
         YG_HIP(hipEventSynchronize(c->ev[2]));
         if (t_tempJ_ms) YG_HIP(hipEventElapsedTime(t_tempJ_ms, c->ev[0], c->ev[1]));

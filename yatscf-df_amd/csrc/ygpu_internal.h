@@ -136,7 +136,7 @@ struct ygpu_ctx {
     DBuf d_pqA, d_B, d_Jpq;
 
     // Fock / density work buffers
-    DBuf d_D, d_Cocc, d_H, d_J, d_K, d_F, d_tempJ, d_tempJpart, d_X, d_Kpart;
+    DBuf d_D, d_Cocc, d_H, d_J, d_K, d_F, d_tempJ, d_tempJpart, d_jw, d_X, d_Kpart;
     DBuf d_mat0, d_mat1, d_mat2, d_eig, d_info, d_evE;
     DBuf d_gA, d_gB, d_gC;
     DBuf d_S, d_Xo;               // cached overlap and X = S^{-1/2}
@@ -253,9 +253,10 @@ int yg_jbuild(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const fl
               float *t_Jmat_ms);
 // row groups of the J build's first pass (its partials take ngroup * naux_pad doubles)
 long yg_jbuild_groups(long n_up, long naux_pad, bool fp32);
-// yg_jbuild with the pass-1 partials and temp_J in caller-provided buffers (nothing allocated)
+// yg_jbuild with the pass-1 partials, temp_J and the n_up pair weights in caller-provided
+// buffers (nothing allocated)
 int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const float *B32, double *part,
-                 double *tempJ, long ngroup);
+                 double *tempJ, double *wts, long ngroup);
 int yg_fill(hipStream_t s, long n, double *x, double v);                          // x[0..n) = v
 // takes the device address of the no-allocation entry point's static scratch array on `device`
 // (which must be current); once per device, later calls return at once

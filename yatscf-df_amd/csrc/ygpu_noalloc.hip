@@ -96,6 +96,7 @@ try {
     const size_t oC = take((size_t) nbf * n_occ * sizeof(double));
     const size_t oTj = take((size_t) ld * sizeof(double));
     const size_t oTp = take((size_t) ngroup * ld * sizeof(double));
+    const size_t oW = take((size_t) c->n_up * sizeof(double));
     const size_t oX = take((size_t) nbf * kdim * sizeof(double));
     const size_t oKp = take((size_t) nsplit * ms);
     if (off > (size_t) YGPU_NOALLOC_SCRATCH_BYTES) {
@@ -108,7 +109,7 @@ try {
     double *dD = (double *) (base + oD), *dH = (double *) (base + oH), *dJ = (double *) (base + oJ);
     double *dK = (double *) (base + oK), *dF = (double *) (base + oF), *dC = (double *) (base + oC);
     double *dTj = (double *) (base + oTj), *dTp = (double *) (base + oTp), *dX = (double *) (base + oX);
-    double *dKp = (double *) (base + oKp);
+    double *dKp = (double *) (base + oKp), *dW = (double *) (base + oW);
     // this call's inputs overwrite the device hand-off buffers' role: nothing resident stays valid
     c->fock_valid = 0;
     c->density_valid = 0;
@@ -118,7 +119,7 @@ try {
     // J on the side stream (after the uploads), K on the main stream, as in ygpu_build_fock
     YG_HIP(hipEventRecord(c->fork, c->stream));
     YG_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
-    int st = yg_jbuild_ws(c, c->side, dD, dJ, nullptr, dTp, dTj, ngroup);
+    int st = yg_jbuild_ws(c, c->side, dD, dJ, nullptr, dTp, dTj, dW, ngroup);
     if (st == YGPU_OK)
         st = yg_xbuild(c->stream, nbf, n_occ, (int) ld, c->d_xptr.as<int>(), c->d_xup.as<int>(),
                        c->d_xpartner.as<int>(), dC, c->d_B.as<double>(), dX, c->max_row_len);
