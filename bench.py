@@ -306,8 +306,8 @@ def main():
     hist = np.vstack([phase_rows(r, FOCK_MIXED if mixed else FOCK_FP64) for r in runs])
     if len(hist) == 0:
         hist = np.vstack([r["hist"][2:] for r in runs])
-    # J and K run on two streams at once: the J/K figure is the wall time of the pair (start of
-    # J to F assembled); j_ms and k_ms are each timed on their own stream beside the other
+    # the J/K figure is the wall time of the pair (start of J to F assembled); J runs first, each
+    # kernel with the whole GPU (YGPU_J_BESIDE=1 puts it on a second stream beside the K build)
     jk_ms = float(np.median(hist[:, 5]) * 1e3)
     j_ms = float(np.median(hist[:, 0] + hist[:, 1]) * 1e3)
     k_ms = float(np.median(hist[:, 2] + hist[:, 3]) * 1e3)

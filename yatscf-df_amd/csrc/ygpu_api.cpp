@@ -387,6 +387,9 @@ try {
     c->device = device;
     YG_HIP(hipStreamCreate(&c->stream));
     YG_HIP(hipStreamCreate(&c->side));
+    // YGPU_J_BESIDE=1: the J build of a Fock build on the side stream beside the K build
+    // (default: before it on the main stream, each kernel with the whole GPU)
+    if (const char *e = std::getenv("YGPU_J_BESIDE")) c->j_beside = std::atoi(e) != 0;
     YG_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
     YG_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     for (auto &e : c->ev) YG_HIP(hipEventCreate(&e));
@@ -784,7 +787,10 @@ static int unpack_to_reference(ygpu_ctx *c, const double *dsrc, double *host_out
 // error paths too (nothing queued there outlives the call that queued it).
 struct SideDrain {
     hipStream_t s;
-    ~SideDrain() { (void) hipStreamSynchronize(s); }
+    ~SideDrain()
+    {
+        if (s) (void) hipStreamSynchronize(s);
+    }
 };
 
 // side stream s waits for everything queued on the main stream so far
@@ -1066,25 +1072,27 @@ try {
         YG_TRY(c->d_Jprev.ensure(ms));
         YG_TRY(c->d_dD.ensure(ms));
         YG_TRY(c->d_Cocc32.ensure((size_t) nbf * n_occ * sizeof(float)));
-        // J on the side stream, K on the main stream (independent until F is assembled)
-        YG_TRY(fork_side(c, c->side));
-        SideDrain drain{c->side};
-        YG_HIP(hipEventRecord(c->ev[5], c->side));
+        // J first on the main stream, or beside the K build on the side stream (the two are
+        // independent until F is assembled)
+        hipStream_t js = c->j_beside ? c->side : c->stream;
+        if (c->j_beside) YG_TRY(fork_side(c, c->side));
+        SideDrain drain{c->j_beside ? c->side : nullptr};
+        YG_HIP(hipEventRecord(c->ev[5], js));
         if (!c->jinc_valid) {
             // first build of a mixed run: full FP64 J, kept with its D as the base of the increments
-            YG_TRY(yg_jbuild(c, c->side, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
-            YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->side));
-            YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->side));
+            YG_TRY(yg_jbuild(c, js, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+            YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, js));
+            YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, js));
             c->jinc_valid = 1;
         } else {
             // J = J_prev + J(D - D_prev) from the FP32 tensor with FP64 arithmetic: the
             // rounding of the tensor enters only through the small difference density
-            YG_TRY(yg_delta(c->side, nn, c->d_D.as<double>(), c->d_Dprev.as<double>(), c->d_dD.as<double>()));
-            YG_TRY(yg_jbuild(c, c->side, c->d_dD.as<double>(), c->d_J.as<double>(), c->d_B32.as<float>(), nullptr,
+            YG_TRY(yg_delta(js, nn, c->d_D.as<double>(), c->d_Dprev.as<double>(), c->d_dD.as<double>()));
+            YG_TRY(yg_jbuild(c, js, c->d_dD.as<double>(), c->d_J.as<double>(), c->d_B32.as<float>(), nullptr,
                              nullptr));
-            YG_TRY(yg_accum(c->side, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
+            YG_TRY(yg_accum(js, nn, c->d_J.as<double>(), c->d_Jprev.as<double>()));
         }
-        YG_HIP(hipEventRecord(c->ev[3], c->side));
+        YG_HIP(hipEventRecord(c->ev[3], js));
         if (mode == YGPU_FOCK_INCR) {
             // D - D_ref = (P M^T + M P^T) / 2 with P = C1 + C_ref, M = C1 - C_ref: K(D - D_ref) =
             // (X^M (X^P)^T + X^P (X^M)^T) / 2, BF16-split products whose error scales with |M|;
@@ -1104,19 +1112,20 @@ try {
                               c->d_K.as<double>(), c->d_Kpart32));
         }
         YG_HIP(hipEventRecord(c->ev[6], c->stream));
-        YG_TRY(join_side(c, c->side));
+        if (c->j_beside) YG_TRY(join_side(c, c->side));
     } else {
-        // J on the side stream, K on the main stream (independent until F is assembled)
+        // J first on the main stream, or beside the K build on the side stream
         YG_TRY(c->d_Dprev.ensure(ms));
         YG_TRY(c->d_Jprev.ensure(ms));
-        YG_TRY(fork_side(c, c->side));
-        SideDrain drain{c->side};
-        YG_HIP(hipEventRecord(c->ev[5], c->side));
-        YG_TRY(yg_jbuild(c, c->side, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
-        YG_HIP(hipEventRecord(c->ev[3], c->side));
+        hipStream_t js = c->j_beside ? c->side : c->stream;
+        if (c->j_beside) YG_TRY(fork_side(c, c->side));
+        SideDrain drain{c->j_beside ? c->side : nullptr};
+        YG_HIP(hipEventRecord(c->ev[5], js));
+        YG_TRY(yg_jbuild(c, js, c->d_D.as<double>(), c->d_J.as<double>(), nullptr, nullptr, nullptr));
+        YG_HIP(hipEventRecord(c->ev[3], js));
         // this FP64 J and its D are the base of the increments of later FP32-tensor builds
-        YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, c->side));
-        YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, c->side));
+        YG_HIP(hipMemcpyAsync(c->d_Jprev.p, c->d_J.p, ms, hipMemcpyDeviceToDevice, js));
+        YG_HIP(hipMemcpyAsync(c->d_Dprev.p, c->d_D.p, ms, hipMemcpyDeviceToDevice, js));
         c->jinc_valid = 1;
         YG_HIP(hipEventRecord(c->ev[8], c->stream));
         YG_TRY(yg_xbuild(c->stream, nbf, n_occ, (int) c->naux_pad, c->d_xptr.as<int>(), c->d_xup.as<int>(),
@@ -1134,7 +1143,7 @@ try {
         YG_HIP(hipMemcpyAsync(c->d_Kref.p, c->d_K.p, ms, hipMemcpyDeviceToDevice, c->stream));
         c->kref_valid = 1;
         c->kref_nocc = n_occ;
-        YG_TRY(join_side(c, c->side));
+        if (c->j_beside) YG_TRY(join_side(c, c->side));
     }
     YG_TRY(yg_fock_assemble(c->stream, nbf, c->d_H.as<double>(), c->d_J.as<double>(), c->d_K.as<double>(),
                             c->d_F.as<double>()));
@@ -1174,7 +1183,7 @@ try {
     tj2 = elapsed(c->ev[1], c->ev[2]);
     tm.t_tempJ = tj1 * 1e-3;
     tm.t_Jmat = tj2 * 1e-3;
-    tm.t_tempK = elapsed(c->ev[8], c->ev[4]) * 1e-3;   // J (ev 5 .. 3) runs beside the K build
+    tm.t_tempK = elapsed(c->ev[8], c->ev[4]) * 1e-3;   // J: ev 5 .. 3, before or beside the K build
     tm.t_Kmat = elapsed(c->ev[4], c->ev[6]) * 1e-3;
     tm.t_symm = elapsed(c->ev[6], c->ev[7]) * 1e-3;
     tm.t_total = elapsed(c->ev[5], c->ev[7]) * 1e-3;

@@ -91,6 +91,7 @@ struct ygpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;   // independent work beside the main stream (the J build)
+    bool j_beside = false;        // J build on the side stream, beside the K build
     hipEvent_t fork = nullptr, join = nullptr;
     rocblas_handle rb = nullptr;
     hipEvent_t ev[10] = {};
