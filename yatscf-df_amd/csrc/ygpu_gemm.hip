@@ -244,6 +244,97 @@ __global__ __launch_bounds__(NTHR, 5) void k_xbuild(const int *__restrict__ xptr
     });
 }
 
+// ---- X build with 160 x 128 tiles: X_i = Cocc[partners of i]^T B[up rows of i] as above, the
+// 160 rows (occupied orbitals) and 128 columns (aux functions) of a workgroup split over four
+// wavefronts of 80 x 64 (5 x 4 blocks of v_mfma_f64_16x16x4f64, 80 accumulators each in
+// doubles).  2.2x the flops per loaded byte of the 64 x 64 engine, and 300 occupied orbitals
+// pad to 320, not 384.  Both operands are gathered rows with the m / n index contiguous, so a
+// stage is copied into LDS without a transpose: thread t moves 10 doubles of A row t / 16
+// (columns 10 (t % 16) ..) and 8 of B row t / 16, the next stage's loads issued before the
+// current stage's products.  LDS rows are padded to 128 bytes past a multiple of 256, so the
+// two 16-lane halves of an operand read fall in different banks.  The gather indices of each
+// chunk of XK160 k are staged in LDS first.
+constexpr int XT_M = 160, XT_N = 128, XT_BK = 16, XT_PADM = 16, XT_PADN = 16, XK160 = 1024;
+
+__global__ __launch_bounds__(NTHR, 2) void k_xbuild160(const int *__restrict__ xptr, const int *__restrict__ xup,
+                                                    const int *__restrict__ xpartner,
+                                                    const double *__restrict__ Cocc, const double *__restrict__ Bup,
+                                                    double *__restrict__ X, int n_occ, int naux_pad, int ncolt,
+                                                    int nrowt)
+{
+    __shared__ int s_pa[XK160], s_up[XK160];
+    __shared__ double As[XT_BK][XT_M + XT_PADM];
+    __shared__ double Bs[XT_BK][XT_N + XT_PADN];
+    const long t = blockIdx.x;
+    const int ct = (int) (t % ncolt), rt = (int) ((t / ncolt) % nrowt), i = (int) (t / ((long) nrowt * ncolt));
+    const int kb = xptr[i], klen = xptr[i + 1] - kb;
+    const int m0 = rt * XT_M, n0 = ct * XT_N;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 80, wn = (w & 1) * 64;
+    const int lk = tid >> 4, lj = tid & 15;
+    const int am = m0 + 10 * lj, bn = n0 + 8 * lj;
+    v4d acc[5][4];
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    double ra[10], rb[8];
+    // stage k0 (relative to the chunk start kc) of a chunk of kn k
+    auto load = [&](int k0, int kn) {
+        const int k = k0 + lk;
+        const bool kin = k < kn;
+        const double *pa = Cocc + (size_t) (kin ? s_pa[k] : 0) * n_occ;
+        const double *pb = Bup + (size_t) (kin ? s_up[k] : 0) * naux_pad;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) ra[q] = (kin && am + q < n_occ) ? pa[am + q] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) rb[q] = (kin && bn + q < naux_pad) ? pb[bn + q] : 0.0;
+    };
+    for (int kc = 0; kc < klen; kc += XK160) {
+        const int kn = min(XK160, klen - kc);
+        if (kc > 0) __syncthreads();            // the previous chunk's loads are done with the indices
+        for (int k = tid; k < kn; k += NTHR) {
+            s_pa[k] = xpartner[kb + kc + k];
+            s_up[k] = xup[kb + kc + k];
+        }
+        __syncthreads();
+        load(0, kn);
+        for (int k0 = 0; k0 < kn; k0 += XT_BK) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 10; ++q) As[lk][10 * lj + q] = ra[q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[lk][8 * lj + q] = rb[q];
+            __syncthreads();
+            if (k0 + XT_BK < kn) load(k0 + XT_BK, kn);
+#pragma unroll
+            for (int kk = 0; kk < XT_BK; kk += 4) {
+                const int kr = kk + (lane >> 4), c = lane & 15;
+                double a[5], b[4];
+#pragma unroll
+                for (int x = 0; x < 5; ++x) a[x] = As[kr][wm + 16 * x + c];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) b[y] = Bs[kr][wn + 16 * y + c];
+#pragma unroll
+                for (int x = 0; x < 5; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+            }
+        }
+    }
+    double *Xi = X + (size_t) i * n_occ * naux_pad;
+#pragma unroll
+    for (int x = 0; x < 5; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + 16 * x + (lane >> 4) + 4 * v, n = n0 + wn + 16 * y + (lane & 15);
+                if (m < n_occ && n < naux_pad) Xi[(size_t) m * naux_pad + n] = acc[x][y][v];
+            }
+}
+
 // upper tiles of X X^T; blockIdx.x = linear upper-tile id, blockIdx.z = k split
 __global__ __launch_bounds__(NTHR) void k_syrk_part(const double *__restrict__ X, int nbf, long kdim, int ntb,
                                                     long kchunk, double *__restrict__ part)
@@ -511,14 +602,21 @@ int yg_xbuild(hipStream_t s, int nbf, int n_occ, int naux_pad, const int *xptr, 
 {
     (void) max_row_len;
     if (nbf <= 0 || n_occ <= 0) return YGPU_OK;
-    const int ncolt = (naux_pad + BN - 1) / BN, nrowt = (n_occ + BM - 1) / BM;
+    // 160 x 128 tiles from 64 occupied orbitals up (fewer pad rows than 64 x 64 tiles waste below)
+    const bool big = n_occ > 64;
+    const int tm = big ? XT_M : BM, tn = big ? XT_N : BN;
+    const int ncolt = (naux_pad + tn - 1) / tn, nrowt = (n_occ + tm - 1) / tm;
     const long total = (long) ncolt * nrowt * nbf;
     if (total > 0x7fffffffL) {
         ygpu_set_error(YGPU_ERR_INVALID, "X build: %ld workgroups exceed the grid limit", total);
         return YGPU_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_xbuild, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X, n_occ,
-                       naux_pad, ncolt, nrowt);
+    if (big)
+        hipLaunchKernelGGL(k_xbuild160, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X,
+                           n_occ, naux_pad, ncolt, nrowt);
+    else
+        hipLaunchKernelGGL(k_xbuild, dim3((unsigned) total), dim3(NTHR), 0, s, xptr, xup, xpartner, Cocc, B, X, n_occ,
+                           naux_pad, ncolt, nrowt);
     YG_HIP(hipGetLastError());
     return YGPU_OK;
 }
