@@ -231,6 +231,46 @@ def test_difference_density_fock(ygpu):
     assert np.allclose(inc["K"], inc["K"].T, rtol=0, atol=0)
 
 
+def test_j_stream_placement_gives_identical_fock(ygpu, monkeypatch):
+    """The J build runs ahead of the K build on the main stream, or (YGPU_J_BESIDE=1, read when
+    an engine is created) on the second stream beside it: the same kernels on the same data, so
+    FP64, mixed (first and incremental J) and difference builds agree bit for bit."""
+    log("case: J stream placement c10h22")
+    r = ref("c10h22")
+    o = r.o
+    s = r.sys
+    S, X = o.mat("S_mat").copy(), o.mat("X_mat").copy()
+
+    def builds(beside):
+        monkeypatch.setenv("YGPU_J_BESIDE", "1" if beside else "0")
+        e = ygpu.Engine(0)
+        try:
+            e.set_system(s)
+            e.set_sparsity(o.i("bf_pair_mask"), o.i("uniq_sp_lid"), o.i("uniq_sp_rid"), o.d("sp_scrval"),
+                           o.d("df_sp_scrval"))
+            e.set_df_tensor(o.d("df_tensor"))
+            e.set_overlap(S, X)
+            out = []
+            # the last build: the reference's own density with its orbitals rotated (the
+            # eigensolver's S-orthonormal orbitals of state 1), so the alignment runs
+            R, _ = np.linalg.qr(np.random.RandomState(5).randn(s.n_occ, s.n_occ))
+            st1 = r.states[1]
+            for mode, D, C in [(ygpu.FOCK_MIXED, r.states[0]["D"], r.states[0]["C"]),
+                               (ygpu.FOCK_MIXED, st1["D"], st1["C"]), (ygpu.FOCK_FP64, st1["D"], st1["C"]),
+                               (ygpu.FOCK_INCR, st1["D"], st1["C"] @ R)]:
+                e.set_fock_precision(mode)
+                out.append(e.build_fock(D, C, r.H))
+            assert e.fock_last_info()[0] == ygpu.FOCK_INCR
+            return out
+        finally:
+            e.close()
+
+    a, b = builds(False), builds(True)
+    for k, (x, y) in enumerate(zip(a, b)):
+        for name in ("J", "K", "F"):
+            assert np.array_equal(x[name], y[name]), (k, name, np.abs(x[name] - y[name]).max())
+
+
 def test_repeated_solves_reuse_device_blocks(ygpu):
     """Consecutive mixed-precision solves in one process: the second takes its large device
     buffers (FP32 tensor, operand blocks, X, k-chunk partials) from the cache of blocks the
