@@ -850,8 +850,8 @@ int yg_xbuild_ps(hipStream_t s, int nbf, int nrow, int naux_pad, const int *xptr
 {
     if (nbf <= 0 || nrow <= 0) return YGPU_OK;
 #define YG_XPS(BI) launch_xbuild_ps<BI, 2, 4, 2>(s, nbf, nrow, naux_pad, xptr, kofs, order, Cp, cpart, Bt, btpart, X)
-    // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD); two ring
-    // stages (a deeper ring measured no faster at 320 x 192)
+    // every row of X in one workgroup up to 320 (row tiles of 320 beyond, on one XCD), 256 aux
+    // columns per tile on eight wavefronts; two ring stages of 72 KiB (a third does not fit)
     if (nrow <= 64) return YG_XPS(1);
     if (nrow <= 128) return YG_XPS(2);
     if (nrow <= 192) return YG_XPS(3);
