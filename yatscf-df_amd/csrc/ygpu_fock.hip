@@ -120,19 +120,28 @@ __global__ __launch_bounds__(64 * JWAVES) void k_tempj_part(const TB *__restrict
         for (int c = 0; c < C; ++c) part[(size_t) blockIdx.y * ld + col + c] = (a[0][c] + a[1][c]) + (a[2][c] + a[3][c]);
 }
 
-// temp_J[c] = sum over the row groups of partial[g][c], in a fixed order (deterministic)
-__global__ void k_tempj_reduce(const double *__restrict__ part, int ngroup, long ld, double *__restrict__ tempJ)
+// temp_J[c] = sum over the row groups of partial[g][c] in a fixed order (deterministic): a
+// workgroup takes TJR_COLS columns, TJR_SEG threads per column each add a contiguous range of
+// the groups in order, and the TJR_SEG sums are added in order
+constexpr int TJR_COLS = 32, TJR_SEG = 8;
+__global__ __launch_bounds__(TJR_COLS * TJR_SEG) void k_tempj_reduce(const double *__restrict__ part, int ngroup,
+                                                                     long ld, double *__restrict__ tempJ)
 {
-    const long col = (long) blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ld) return;
-    double s0 = 0.0, s1 = 0.0;
-    int g = 0;
-    for (; g + 1 < ngroup; g += 2) {
-        s0 += part[(size_t) g * ld + col];
-        s1 += part[(size_t) (g + 1) * ld + col];
+    __shared__ double red[TJR_SEG][TJR_COLS];
+    const int cl = threadIdx.x % TJR_COLS, sg = threadIdx.x / TJR_COLS;
+    const long col = (long) blockIdx.x * TJR_COLS + cl;
+    const int per = (ngroup + TJR_SEG - 1) / TJR_SEG, g0 = sg * per, g1 = min(ngroup, g0 + per);
+    double s = 0.0;
+    if (col < ld)
+        for (int g = g0; g < g1; ++g) s += part[(size_t) g * ld + col];
+    red[sg][cl] = s;
+    __syncthreads();
+    if (sg == 0 && col < ld) {
+        double t = red[0][cl];
+#pragma unroll
+        for (int q = 1; q < TJR_SEG; ++q) t += red[q][cl];
+        tempJ[col] = t;
     }
-    if (g < ngroup) s0 += part[(size_t) g * ld + col];
-    tempJ[col] = s0 + s1;
 }
 
 constexpr int JM_ROWS = 4;      // pass-2 rows in flight per wavefront
@@ -502,7 +511,8 @@ int yg_jbuild_ws(ygpu_ctx *c, hipStream_t s, const double *dD, double *dJ, const
         hipLaunchKernelGGL(k_tempj_part<double>, dim3((unsigned) ncb, (unsigned) ngroup), dim3(64 * JWAVES), 0, s,
                            c->d_B.as<double>(), ld, n_up, rpg, wts, part);
     YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tempj_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, part, (int) ngroup, ld, tempJ);
+    hipLaunchKernelGGL(k_tempj_reduce, dim3((unsigned) ((ld + TJR_COLS - 1) / TJR_COLS)), dim3(TJR_COLS * TJR_SEG), 0, s,
+                       part, (int) ngroup, ld, tempJ);
     YG_HIP(hipGetLastError());
     YG_HIP(hipEventRecord(c->ev[1], s));
     // J entries of screened-out pairs stay zero (a fill kernel, not a memset node)
