@@ -404,14 +404,15 @@ template <int BI, int BJ, int WN> struct PsStage {
 };
 
 // Panel engine (ring: NS stages of PsStage<BI, BJ, WN>::SLOTS slots in LDS): acc = A B^T for one
-// (64 BI) x (32 BJ WN) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.  A and B are panel-major: panel p of an operand with R rows holds
-// the R rows one after the other, 32 k each, at p * 32 R; its two BF16 parts are apart / bpart
-// elements apart.  2 x WN wavefronts, each with a (32 BI) x (32 BJ) register tile of 32x32 MFMA
-// blocks (WN = 2, BI = 5, BJ = 3: 240 accumulators, one wavefront per SIMD; WN = 4, BJ = 2: 160,
-// two per SIMD), so every 16-byte LDS operand read feeds BJ (or BI) block products.  A stage is copied global -> LDS by the texture
-// path's direct loads (global_load_lds_dwordx4, no registers), each wavefront instruction one
-// contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while stage s is
-// multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
+// (64 BI) x (32 BJ WN) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.
+// A and B are panel-major: panel p of an operand with R rows holds the R rows one after the
+// other, 32 k each, at p * 32 R; its two BF16 parts are apart / bpart elements apart.  2 x WN
+// wavefronts, each with a (32 BI) x (32 BJ) register tile of 32x32 MFMA blocks (WN = 2, BI = 5,
+// BJ = 3: 240 accumulators, one wavefront per SIMD; WN = 4, BJ = 2: 160, two per SIMD), so every
+// 16-byte LDS operand read feeds BJ (or BI) block products.  A stage is copied global -> LDS by
+// the texture path's direct loads (global_load_lds_dwordx4, no registers), each wavefront
+// instruction one contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while
+// stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
 // rows of one operand read over all banks.  Rows beyond an operand are read from its last row
 // (their results are not stored).
 template <int BI, int BJ, int WN, int NS>
@@ -427,9 +428,9 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = (w / WN) * 32 * BI, wn = (w % WN) * 32 * BJ;
     // wavefront copy j of a stage fills slots 64 j .. 64 j + 63: the first SA / (64 NW) of each
-    // wavefront's copies are A's, the rest B's.  Slot t = 64 q + lane of an
-    // operand part holds row t / 4 = 16 q + lane / 4 and, at position t % 4 = lane % 4, its
-    // 8-k group (lane % 4) XOR ((row / 2) % 4) = (lane % 4) XOR ((lane / 8) % 4)
+    // wavefront's copies are A's, the rest B's.  Slot t = 64 q + lane of an operand part holds
+    // row t / 4 = 16 q + lane / 4 and, at position t % 4 = lane % 4, its 8-k group
+    // (lane % 4) XOR ((row / 2) % 4) = (lane % 4) XOR ((lane / 8) % 4)
     const int lr = lane >> 2, lg = 8 * ((lane & 3) ^ ((lane >> 3) & 3));
     auto issue = [&](int slot, int p) {
         v8bf *dst = ring + slot * P::SLOTS;
