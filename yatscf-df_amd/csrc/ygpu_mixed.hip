@@ -27,6 +27,7 @@
 #include "ygpu_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -403,6 +404,17 @@ template <int BI, int BJ, int WN> struct PsStage {
                   "each operand part of a stage is whole wavefront copies of every wavefront");
 };
 
+// column block of wavefront w's register tile.  The hardware places wavefront w of a workgroup on
+// SIMD w % 4, so wavefronts w and w + WN (WN = 4) share a SIMD.  With mir (uniform) the second row
+// half takes the column blocks in reverse, so a SIMD holds column blocks b and WN - 1 - b: the
+// register tiles that the upper-triangle product skips (wholly below the diagonal, at the low
+// column end of the second row half; beyond the columns, at the high end of both) fall on
+// different SIMDs.  (Tiles without skips run ~2 % faster without mir.)
+template <int WN> __device__ __forceinline__ int ps_colblk(int w, bool mir)
+{
+    return (!mir || w < WN) ? w % WN : 2 * WN - 1 - w;
+}
+
 // Panel engine (ring: NS stages of PsStage<BI, BJ, WN>::SLOTS slots in LDS): acc = A B^T for one
 // (64 BI) x (32 BJ WN) tile (rows m0.. of A, n0.. of B) over the 32-deep k panels 0 .. nst-1.
 // A and B are panel-major: panel p of an operand with R rows holds the R rows one after the
@@ -414,11 +426,13 @@ template <int BI, int BJ, int WN> struct PsStage {
 // instruction one contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while
 // stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
 // rows of one operand read over all banks.  Rows beyond an operand are read from its last row
-// (their results are not stored).
+// (their results are not stored).  Wavefront w's register tile is row half w / WN, column block
+// ps_colblk<WN>(w, mir).  A wavefront with act false (uniform) copies its share of
+// every stage and meets every barrier but reads no operands and multiplies nothing (acc stays 0).
 template <int BI, int BJ, int WN, int NS>
 __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A,
                                         size_t apart, int RA, int m0, const __bf16 *__restrict__ B, size_t bpart,
-                                        int RB, int n0, int nst, v16f (&acc)[BI][BJ])
+                                        int RB, int n0, int nst, bool mir, bool act, v16f (&acc)[BI][BJ])
 {
     using P = PsStage<BI, BJ, WN>;
     constexpr int BM = P::BM, BN = P::BN, SA = P::SA, PW = P::PW, NW = P::NW;
@@ -426,7 +440,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
     // the wavefront index as a scalar: the LDS destination and operand panel of each of its
     // copies are uniform, only the row and group within the panel vary by lane
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (w / WN) * 32 * BI, wn = (w % WN) * 32 * BJ;
+    const int wm = (w / WN) * 32 * BI, wn = ps_colblk<WN>(w, mir) * 32 * BJ;
     // wavefront copy j of a stage fills slots 64 j .. 64 j + 63: the first SA / (64 NW) of each
     // wavefront's copies are A's, the rest B's.  Slot t = 64 q + lane of an operand part holds
     // row t / 4 = 16 q + lane / 4 and, at position t % 4 = lane % 4, its 8-k group
@@ -460,58 +474,67 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
     const int rl = lane & 31, hk = lane >> 5;
     for (int p = 0; p < NS - 1 && p < nst; ++p) issue(p, p);
-    for (int st = 0; st < nst; ++st) {
-        // stage st landed (the later stages issued so far may still be in flight), every LDS
-        // read of this wavefront done, then the workgroup: the ring slot written next was read
-        // by all wavefronts in the previous stage
-        const int later = min(NS - 2, nst - 1 - st);
-        asm volatile("" ::: "memory");
-        switch (later) {
+    // the stage loop, with or without the products (two copies of the loop, so the one that
+    // multiplies carries no test of act)
+    auto stages = [&](auto mul) {
+        for (int st = 0; st < nst; ++st) {
+            // stage st landed (the later stages issued so far may still be in flight), every LDS
+            // read of this wavefront done, then the workgroup: the ring slot written next was read
+            // by all wavefronts in the previous stage
+            const int later = min(NS - 2, nst - 1 - st);
+            asm volatile("" ::: "memory");
+            switch (later) {
 #define YG_WAIT(n) \
     case n: __builtin_amdgcn_s_waitcnt((((n) * PW) & 15) | ((((n) * PW) >> 4) << 14) | (7 << 4)); break;
-            YG_WAIT(0) YG_WAIT(1) YG_WAIT(2) YG_WAIT(3)
+                YG_WAIT(0) YG_WAIT(1) YG_WAIT(2) YG_WAIT(3)
 #undef YG_WAIT
-        }
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-        if (st + NS - 1 < nst) issue((st + NS - 1) % NS, st + NS - 1);
-        const v8bf *sb = ring + (st % NS) * P::SLOTS;
-        const v8bf *sa0 = sb, *sa1 = sb + BM * 4, *sb0 = sb + SA, *sb1 = sb0 + BN * 4;
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            v8bf a[BI][2], b[BJ][2];
-#pragma unroll
-            for (int x = 0; x < BI; ++x) {
-                const int r = wm + 32 * x + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
-                a[x][0] = sa0[off];
-                a[x][1] = sa1[off];
             }
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (st + NS - 1 < nst) issue((st + NS - 1) % NS, st + NS - 1);
+            if constexpr (!decltype(mul)::value) continue;
+            const v8bf *sb = ring + (st % NS) * P::SLOTS;
+            const v8bf *sa0 = sb, *sa1 = sb + BM * 4, *sb0 = sb + SA, *sb1 = sb0 + BN * 4;
 #pragma unroll
-            for (int y = 0; y < BJ; ++y) {
-                const int r = wn + 32 * y + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
-                b[y][0] = sb0[off];
-                b[y][1] = sb1[off];
-            }
-            // lo*hi, hi*lo, hi*hi: the order of the other two-part products
+            for (int ks = 0; ks < 2; ++ks) {
+                v8bf a[BI][2], b[BJ][2];
 #pragma unroll
-            for (int x = 0; x < BI; ++x)
+                for (int x = 0; x < BI; ++x) {
+                    const int r = wm + 32 * x + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
+                    a[x][0] = sa0[off];
+                    a[x][1] = sa1[off];
+                }
 #pragma unroll
                 for (int y = 0; y < BJ; ++y) {
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][1], b[y][0], acc[x][y], 0, 0, 0);
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][1], acc[x][y], 0, 0, 0);
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][0], acc[x][y], 0, 0, 0);
+                    const int r = wn + 32 * y + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
+                    b[y][0] = sb0[off];
+                    b[y][1] = sb1[off];
                 }
+                // lo*hi, hi*lo, hi*hi: the order of the other two-part products
+#pragma unroll
+                for (int x = 0; x < BI; ++x)
+#pragma unroll
+                    for (int y = 0; y < BJ; ++y) {
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][1], b[y][0], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][1], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][0], acc[x][y], 0, 0, 0);
+                    }
+            }
         }
-    }
+    };
+    if (act)
+        stages(std::true_type{});
+    else
+        stages(std::false_type{});
 }
 
 // store(m, n, v) for every element of this wavefront's part of a ps_tile result (m, n relative
 // to the workgroup tile)
 template <int BI, int BJ, int WN, class F>
-__device__ __forceinline__ void ps_epilogue(const v16f (&acc)[BI][BJ], F &&store)
+__device__ __forceinline__ void ps_epilogue(const v16f (&acc)[BI][BJ], bool mir, F &&store)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wm = (w / WN) * 32 * BI, wn = (w % WN) * 32 * BJ, rl = lane & 31, hk = lane >> 5;
+    const int wm = (w / WN) * 32 * BI, wn = ps_colblk<WN>(w, mir) * 32 * BJ, rl = lane & 31, hk = lane >> 5;
 #pragma unroll
     for (int x = 0; x < BI; ++x)
 #pragma unroll
@@ -551,7 +574,7 @@ __global__ __launch_bounds__(128 * WN, 1) void k_xbuild_ps(const int *__restrict
     const int m0 = rt * P::BM, n0 = ct * P::BN;
     v16f acc[BI][BJ];
     ps_tile<BI, BJ, WN, NS>(&ring[0][0], Cp + ko * nrow, cpart, nrow, m0, Bt + ko * naux_pad, btpart, naux_pad, n0,
-                            L / 32, acc);
+                            L / 32, false, true, acc);
     // the tile leaves through the (now free) ring: each wavefront writes one 32-row block of its
     // accumulators as FP32 rows into its own part of LDS, then reads them back as runs of 8
     // columns, splits them and stores 16 bytes of each part per lane (four lanes fill one 32-k
@@ -601,10 +624,11 @@ __global__ __launch_bounds__(128 * WN, 1) void k_xbuild_ps(const int *__restrict
 }
 
 // FP32 partials of A B^T over k chunks, A and B panel-major with R rows (the products of the
-// two-part X: the SYRK X X^T and the difference product X^M X^P^T).  Tiles of (64 BI) x (64 BJ);
-// with UPPER only the tiles (rt, ct) holding an element m <= n (rt BM < (ct + 1) BN), else all
-// nrowt x ncolt.  Chunk z covers panels [z cp, min(npan, (z + 1) cp)) and writes its tiles'
-// elements (m, n < R) to part[z] (R x R, FP32).  The chunks are dealt to the XCDs (the hardware
+// two-part X: the SYRK X X^T and the difference product X^M X^P^T).  Tiles of (64 BI) x
+// (32 BJ WN); with UPPER only the tiles (rt, ct) holding an element m <= n (rt BM < (ct + 1) BN),
+// else all nrowt x ncolt.  Chunk z covers panels [z cp, min(npan, (z + 1) cp)) and writes its
+// tiles' elements (m, n < R; with UPPER those of the register tiles holding an element m <= n)
+// to part[z] (R x R, FP32).  The chunks are dealt to the XCDs (the hardware
 // deals workgroup b to XCD b % 8), every tile of chunk z on XCD z % 8 one after the other, so the
 // tiles of a chunk run side by side and read its panels from the one L2 (nchunk is a multiple of 8).
 template <int BI, int BJ, int WN, int NS, bool UPPER>
@@ -636,10 +660,28 @@ __global__ __launch_bounds__(128 * WN, 1) void k_pgemm_ps(const __bf16 *__restri
     const int nst = p1 > p0 ? (int) (p1 - p0) : 0;
     const size_t po = (size_t) (p0 < npan ? p0 : 0) * 32 * R;
     const int m0 = rt * BM, n0 = ct * BN;
+    // UPPER: a wavefront whose register tile lies beyond the rows or columns or wholly below the
+    // diagonal (first row past its last column) stores nothing, so it multiplies nothing; the
+    // column blocks are mirrored in the tiles that have such register tiles.  (Skipping the few
+    // register tiles of the full product beyond the rows or columns did not pay.)
+    bool mir = false, act = true;
+    if constexpr (UPPER) {
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        auto live = [&](int a, int b) {
+            const int ms = m0 + a * 32 * BI, ns = n0 + b * 32 * BJ;
+            return ms < R && ns < R && ms < ns + 32 * BJ;
+        };
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < WN; ++b) mir = mir || !live(a, b);
+        act = live(w / WN, ps_colblk<WN>(w, mir));
+    }
     v16f acc[BI][BJ];
-    ps_tile<BI, BJ, WN, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, acc);
+    ps_tile<BI, BJ, WN, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, mir, act, acc);
+    if (!act) return;
     float *Pz = part + (size_t) z * R * R;
-    ps_epilogue<BI, BJ, WN>(acc, [&](int r, int c, float v) {
+    ps_epilogue<BI, BJ, WN>(acc, mir, [&](int r, int c, float v) {
         const int m = m0 + r, n = n0 + c;
         if (m < R && n < R) Pz[(size_t) m * R + n] = v;
     });
