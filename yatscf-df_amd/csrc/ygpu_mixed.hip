@@ -404,6 +404,13 @@ template <int BI, int BJ, int WN> struct PsStage {
                   "each operand part of a stage is whole wavefront copies of every wavefront");
 };
 
+// f(std::integral_constant<int, t>) for the t in -1 .. sizeof...(Ts) - 2 equal to the uniform t
+template <class F, int... Ts>
+__device__ __forceinline__ void ps_cut_dispatch(int t, F &&f, std::integer_sequence<int, Ts...>)
+{
+    (void) ((t == Ts - 1 ? (f(std::integral_constant<int, Ts - 1>{}), 0) : 0) + ...);
+}
+
 // column block of wavefront w's register tile.  The hardware places wavefront w of a workgroup on
 // SIMD w % 4, so wavefronts w and w + WN (WN = 4) share a SIMD.  With mir (uniform) the second row
 // half takes the column blocks in reverse, so a SIMD holds column blocks b and WN - 1 - b: the
@@ -427,12 +434,15 @@ template <int WN> __device__ __forceinline__ int ps_colblk(int w, bool mir)
 // stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
 // rows of one operand read over all banks.  Rows beyond an operand are read from its last row
 // (their results are not stored).  Wavefront w's register tile is row half w / WN, column block
-// ps_colblk<WN>(w, mir).  A wavefront with act false (uniform) copies its share of
-// every stage and meets every barrier but reads no operands and multiplies nothing (acc stays 0).
+// ps_colblk<WN>(w, mir).  The wavefront multiplies only its 32x32 blocks (x, y) with x - y < cut
+// (cut uniform; cut >= BI: all, the blocks wholly below the diagonal of a register tile whose first
+// row is 32 (cut - 1) before its first column excluded); with cut < 0 it copies its share of every
+// stage and meets every barrier but reads no operands and multiplies nothing.  Blocks not
+// multiplied stay 0.
 template <int BI, int BJ, int WN, int NS>
 __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A,
                                         size_t apart, int RA, int m0, const __bf16 *__restrict__ B, size_t bpart,
-                                        int RB, int n0, int nst, bool mir, bool act, v16f (&acc)[BI][BJ])
+                                        int RB, int n0, int nst, bool mir, int cut, v16f (&acc)[BI][BJ])
 {
     using P = PsStage<BI, BJ, WN>;
     constexpr int BM = P::BM, BN = P::BN, SA = P::SA, PW = P::PW, NW = P::NW;
@@ -474,9 +484,9 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
     const int rl = lane & 31, hk = lane >> 5;
     for (int p = 0; p < NS - 1 && p < nst; ++p) issue(p, p);
-    // the stage loop, with or without the products (two copies of the loop, so the one that
-    // multiplies carries no test of act)
-    auto stages = [&](auto mul) {
+    // the stage loop for one cut T (a copy of the loop per value: no tests inside)
+    auto stages = [&](auto tc) {
+        constexpr int T = decltype(tc)::value;
         for (int st = 0; st < nst; ++st) {
             // stage st landed (the later stages issued so far may still be in flight), every LDS
             // read of this wavefront done, then the workgroup: the ring slot written next was read
@@ -492,7 +502,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
             if (st + NS - 1 < nst) issue((st + NS - 1) % NS, st + NS - 1);
-            if constexpr (!decltype(mul)::value) continue;
+            if constexpr (T < 0) continue;
             const v8bf *sb = ring + (st % NS) * P::SLOTS;
             const v8bf *sa0 = sb, *sa1 = sb + BM * 4, *sb0 = sb + SA, *sb1 = sb0 + BN * 4;
 #pragma unroll
@@ -500,6 +510,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
                 v8bf a[BI][2], b[BJ][2];
 #pragma unroll
                 for (int x = 0; x < BI; ++x) {
+                    if (x - (BJ - 1) >= T) continue;
                     const int r = wm + 32 * x + rl, off = r * 4 + ((2 * ks + hk) ^ ((r >> 1) & 3));
                     a[x][0] = sa0[off];
                     a[x][1] = sa1[off];
@@ -515,6 +526,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
                 for (int x = 0; x < BI; ++x)
 #pragma unroll
                     for (int y = 0; y < BJ; ++y) {
+                        if (x - y >= T) continue;
                         acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][1], b[y][0], acc[x][y], 0, 0, 0);
                         acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][1], acc[x][y], 0, 0, 0);
                         acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x][0], b[y][0], acc[x][y], 0, 0, 0);
@@ -522,10 +534,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
             }
         }
     };
-    if (act)
-        stages(std::true_type{});
-    else
-        stages(std::false_type{});
+    ps_cut_dispatch(std::min(cut, BI), stages, std::make_integer_sequence<int, BI + 2>{});
 }
 
 // store(m, n, v) for every element of this wavefront's part of a ps_tile result (m, n relative
@@ -574,7 +583,7 @@ __global__ __launch_bounds__(128 * WN, 1) void k_xbuild_ps(const int *__restrict
     const int m0 = rt * P::BM, n0 = ct * P::BN;
     v16f acc[BI][BJ];
     ps_tile<BI, BJ, WN, NS>(&ring[0][0], Cp + ko * nrow, cpart, nrow, m0, Bt + ko * naux_pad, btpart, naux_pad, n0,
-                            L / 32, false, true, acc);
+                            L / 32, false, BI, acc);
     // the tile leaves through the (now free) ring: each wavefront writes one 32-row block of its
     // accumulators as FP32 rows into its own part of LDS, then reads them back as runs of 8
     // columns, splits them and stores 16 bytes of each part per lane (four lanes fill one 32-k
@@ -661,10 +670,13 @@ __global__ __launch_bounds__(128 * WN, 1) void k_pgemm_ps(const __bf16 *__restri
     const size_t po = (size_t) (p0 < npan ? p0 : 0) * 32 * R;
     const int m0 = rt * BM, n0 = ct * BN;
     // UPPER: a wavefront whose register tile lies beyond the rows or columns or wholly below the
-    // diagonal (first row past its last column) stores nothing, so it multiplies nothing; the
-    // column blocks are mirrored in the tiles that have such register tiles.  (Skipping the few
-    // register tiles of the full product beyond the rows or columns did not pay.)
+    // diagonal (first row past its last column) stores nothing, so it multiplies nothing, and one
+    // whose register tile crosses the diagonal multiplies only the 32x32 blocks holding an element
+    // m <= n (the reduction reads no other); the column blocks are mirrored in the tiles that have
+    // such register tiles.  (Skipping the few register tiles of the full product beyond the rows
+    // or columns did not pay.)
     bool mir = false, act = true;
+    int cut = BI;
     if constexpr (UPPER) {
         const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         auto live = [&](int a, int b) {
@@ -675,10 +687,14 @@ __global__ __launch_bounds__(128 * WN, 1) void k_pgemm_ps(const __bf16 *__restri
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < WN; ++b) mir = mir || !live(a, b);
-        act = live(w / WN, ps_colblk<WN>(w, mir));
+        const int a = w / WN, b = ps_colblk<WN>(w, mir);
+        act = live(a, b);
+        // block (x, y) holds an element m <= n iff its first row is before its last column:
+        // x - y < (ns - ms) / 32 + 1 (ms, ns multiples of 32; >= 0 for a live register tile)
+        cut = act ? (n0 + b * 32 * BJ - m0 - a * 32 * BI) / 32 + 1 : -1;
     }
     v16f acc[BI][BJ];
-    ps_tile<BI, BJ, WN, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, mir, act, acc);
+    ps_tile<BI, BJ, WN, NS>(&ring[0][0], A + po, xpart, R, m0, B + po, xpart, R, n0, nst, mir, cut, acc);
     if (!act) return;
     float *Pz = part + (size_t) z * R * R;
     ps_epilogue<BI, BJ, WN>(acc, mir, [&](int r, int c, float v) {
