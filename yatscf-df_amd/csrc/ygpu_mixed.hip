@@ -432,8 +432,9 @@ template <int WN> __device__ __forceinline__ int ps_colblk(int w, bool mir)
 // the texture path's direct loads (global_load_lds_dwordx4, no registers), each wavefront
 // instruction one contiguous KiB; NS stages in a ring, the copy of stage s + NS - 1 issued while
 // stage s is multiplied.  In LDS a row's four 8-k groups sit at 4 r + (g XOR (r >> 1) & 3), which spreads the
-// rows of one operand read over all banks.  Rows beyond an operand are read from its last row
-// (their results are not stored).  Wavefront w's register tile is row half w / WN, column block
+// rows of one operand read over all banks.  Rows beyond an operand are read from its last row, and
+// with two stages a copy of 16 rows wholly beyond it is not issued, its slots holding zeros: either
+// way only results that are not stored see them.  Wavefront w's register tile is row half w / WN, column block
 // ps_colblk<WN>(w, mir).  The wavefront multiplies only its 32x32 blocks (x, y) with x - y < cut
 // (cut uniform; cut >= BI: all, the blocks wholly below the diagonal of a register tile whose first
 // row is 32 (cut - 1) before its first column excluded); with cut < 0 it copies its share of every
@@ -456,6 +457,14 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
     // row t / 4 = 16 q + lane / 4 and, at position t % 4 = lane % 4, its 8-k group
     // (lane % 4) XOR ((row / 2) % 4) = (lane % 4) XOR ((lane / 8) % 4)
     const int lr = lane >> 2, lg = 8 * ((lane & 3) ^ ((lane >> 3) & 3));
+    // with two stages (every wait then for all copies in flight) a copy of 16 rows wholly beyond
+    // its operand is not issued
+    auto beyond = [&](int jj) {
+        const int j = NW * jj + w;
+        if (NS != 2) return false;
+        if (jj < SA / (64 * NW)) return m0 + 16 * (j % (BM / 16)) >= RA;
+        return n0 + 16 * ((j - SA / 64) % (BN / 16)) >= RB;
+    };
     auto issue = [&](int slot, int p) {
         v8bf *dst = ring + slot * P::SLOTS;
 #pragma unroll
@@ -463,6 +472,7 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
             const int j = NW * jj + w;
             const __bf16 *base;
             int row;
+            if (beyond(jj)) continue;
             if (jj < SA / (64 * NW)) {
                 const int part = j / (BM / 16), q = j % (BM / 16);
                 base = A + part * apart + (size_t) p * RA * 32;
@@ -476,6 +486,14 @@ __device__ __forceinline__ void ps_tile(v8bf *ring, const __bf16 *__restrict__ A
                                              16, 0, 0);
         }
     };
+    // the copies not issued leave zeros in their slots (written once, here; the first stage's
+    // wait and barrier order them before any read): stale bits there would feed the products
+    // arbitrary operands, which cost clock
+#pragma unroll
+    for (int jj = 0; jj < PW; ++jj)
+        if (beyond(jj))
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) ring[sl * P::SLOTS + 64 * (NW * jj + w) + lane] = v8bf{};
 #pragma unroll
     for (int a = 0; a < BI; ++a)
 #pragma unroll
