@@ -49,12 +49,11 @@ void TinySCF_DIIS(TinySCF_t T)
 {
     ygpu_ctx_t *g = (ygpu_ctx_t *) T->gpu;
     const int ldB = MAX_DIIS + 1;
-    const size_t ms = (size_t) T->mat_size;
     double st = get_wtime_sec();
     if (T->iter <= 1) {
-        /* F = X^T F X only (DIIS.c:30-40) */
-        YS_GPU(ygpu_diis_transform(g, T->F_mat, T->tmp_mat));
-        memcpy(T->F_mat, T->tmp_mat, DBL_SIZE * ms);
+        /* F = X^T F X only (DIIS.c:30-40), in place: on the device stream the copy of F up (when
+         * it is copied) precedes the copy of X^T F X down */
+        YS_GPU(ygpu_diis_transform(g, T->F_mat, T->F_mat));
         T->t_diis = get_wtime_sec() - st;
         return;
     }
